@@ -1,0 +1,260 @@
+"""ctypes binding of libaj.so, the C ABI declared in include/alliancedb/aj.h.
+
+The library is the product: every call here goes straight to the HIP
+implementation.  If libaj.so is missing the import fails loudly; there is no
+CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libaj.so")
+
+# tuple_t of hashing/utils/types.h:67-71: {int32 payloadID @0, int32 key @4}
+TUPLE_DTYPE = np.dtype([("payloadID", "<i4"), ("key", "<i4")])
+# aj_stream_tuple_t: {uint32 key, uint32 payload, uint32 ts}
+STREAM_DTYPE = np.dtype([("key", "<u4"), ("payload", "<u4"), ("ts", "<u4")])
+
+AJ_OK = 0
+AJ_ERR_ARG = -1
+AJ_ERR_NOMEM = -2
+AJ_ERR_HIP = -3
+AJ_ERR_CAPACITY = -4
+AJ_ERR_STATE = -5
+AJ_ERR_NODEV = -6
+AJ_ALGO_PRO = 0
+AJ_ALGO_NPO = 1
+
+
+class relation_t(ctypes.Structure):
+    _fields_ = [("tuples", ctypes.c_void_p), ("num_tuples", ctypes.c_uint64),
+                ("payload", ctypes.c_void_p)]
+
+
+class result_t(ctypes.Structure):
+    _fields_ = [("totalresults", ctypes.c_int64), ("resultlist", ctypes.c_void_p),
+                ("nthreads", ctypes.c_int)]
+
+
+class aj_result_t(ctypes.Structure):
+    _fields_ = [("base", result_t), ("checksum", ctypes.c_uint64), ("npairs", ctypes.c_uint64),
+                ("pairs", ctypes.POINTER(ctypes.c_uint64)), ("status", ctypes.c_int),
+                ("device_ms", ctypes.c_float), ("total_ms", ctypes.c_float)]
+
+
+class aj_join_opts_t(ctypes.Structure):
+    _fields_ = [("algo", ctypes.c_int), ("radix_bits", ctypes.c_int), ("passes", ctypes.c_int),
+                ("cap", ctypes.c_int), ("split", ctypes.c_uint32),
+                ("pairs", ctypes.c_void_p), ("pairs_capacity", ctypes.c_uint64)]
+
+
+class aj_join_out_t(ctypes.Structure):
+    _fields_ = [("count", ctypes.c_uint64), ("checksum", ctypes.c_uint64),
+                ("npairs", ctypes.c_uint64), ("device_ms", ctypes.c_float)]
+
+
+class AjError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"aj status {status}: {msg}")
+        self.status = status
+
+
+def _load() -> ctypes.CDLL:
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is not built; run __graft_entry__.build() (make -C alliancedb_amd/csrc)")
+    lib = ctypes.CDLL(LIB_PATH)
+    c = ctypes
+    P = c.c_void_p
+    sig = {
+        "aj_last_error": (c.c_char_p, []),
+        "aj_last_status": (c.c_int, []),
+        "PRO": (c.POINTER(result_t), [c.POINTER(relation_t), c.POINTER(relation_t), c.c_int]),
+        "NPO": (c.POINTER(result_t), [c.POINTER(relation_t), c.POINTER(relation_t), c.c_int]),
+        "aj_result_ext": (c.POINTER(aj_result_t), [c.POINTER(result_t)]),
+        "aj_set_option": (c.c_int, [c.c_char_p, c.c_int64]),
+        "aj_get_option": (c.c_int64, [c.c_char_p]),
+        "aj_ctx_create": (c.c_int, [c.c_int, c.POINTER(P)]),
+        "aj_ctx_destroy": (c.c_int, [P]),
+        "aj_join_dev": (c.c_int, [P, P, c.c_uint64, P, c.c_uint64, c.POINTER(aj_join_opts_t),
+                                  c.POINTER(aj_join_out_t), P]),
+        "aj_device_alloc": (c.c_int, [c.POINTER(P), c.c_uint64]),
+        "aj_device_free": (c.c_int, [P]),
+        "aj_copy_h2d": (c.c_int, [P, P, c.c_uint64]),
+        "aj_copy_d2h": (c.c_int, [P, P, c.c_uint64]),
+        "aj_device_sync": (c.c_int, []),
+        "aj_gen_unique": (c.c_int, [P, c.c_uint64, c.c_uint64]),
+        "aj_gen_uniform": (c.c_int, [P, c.c_uint64, c.c_uint32, c.c_uint32, c.c_uint64]),
+        "aj_gen_zipf": (c.c_int, [P, c.c_uint64, c.c_uint32, c.c_double, c.c_uint64]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+lib = _load()
+_libc = ctypes.CDLL(None)
+_libc.free.argtypes = [ctypes.c_void_p]
+_libc.free.restype = None
+
+
+def last_error() -> str:
+    return lib.aj_last_error().decode(errors="replace")
+
+
+def check(status: int) -> None:
+    if status != AJ_OK:
+        raise AjError(status, last_error())
+
+
+def set_option(name: str, value: int) -> None:
+    check(lib.aj_set_option(name.encode(), int(value)))
+
+
+def get_option(name: str) -> int:
+    return int(lib.aj_get_option(name.encode()))
+
+
+def _ptr(a: Optional[np.ndarray]) -> Optional[int]:
+    return None if a is None else a.ctypes.data
+
+
+def as_tuples(keys: np.ndarray, payloads: Optional[np.ndarray] = None) -> np.ndarray:
+    t = np.empty(len(keys), dtype=TUPLE_DTYPE)
+    t["key"] = keys
+    t["payloadID"] = np.arange(len(keys)) if payloads is None else payloads
+    return t
+
+
+@dataclass
+class JoinResult:
+    count: int
+    checksum: int
+    pairs: Optional[np.ndarray]
+    device_ms: float
+    total_ms: float = 0.0
+
+
+def _dropin(fn, R: np.ndarray, S: np.ndarray, nthreads: int) -> JoinResult:
+    R = np.ascontiguousarray(R, dtype=TUPLE_DTYPE)
+    S = np.ascontiguousarray(S, dtype=TUPLE_DTYPE)
+    rr = relation_t(R.ctypes.data if len(R) else None, len(R), None)
+    rs = relation_t(S.ctypes.data if len(S) else None, len(S), None)
+    res = fn(ctypes.byref(rr), ctypes.byref(rs), int(nthreads))
+    if not res:
+        raise AjError(lib.aj_last_status(), last_error())
+    try:
+        ext = lib.aj_result_ext(res).contents
+        pairs = None
+        if ext.npairs:
+            pairs = np.ctypeslib.as_array(ext.pairs, shape=(ext.npairs,)).copy()
+        return JoinResult(int(ext.base.totalresults), int(ext.checksum), pairs,
+                          float(ext.device_ms), float(ext.total_ms))
+    finally:
+        _libc.free(ctypes.cast(res, ctypes.c_void_p))
+
+
+def PRO(R: np.ndarray, S: np.ndarray, nthreads: int = 1) -> JoinResult:
+    """Drop-in PRO (radix join) on host relations; see aj.h."""
+    return _dropin(lib.PRO, R, S, nthreads)
+
+
+def NPO(R: np.ndarray, S: np.ndarray, nthreads: int = 1) -> JoinResult:
+    """Drop-in NPO (no-partitioning join) on host relations; see aj.h."""
+    return _dropin(lib.NPO, R, S, nthreads)
+
+
+class Context:
+    """An aj_ctx_t: device scratch reused across joins on one GPU."""
+
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        check(lib.aj_ctx_create(int(device), ctypes.byref(h)))
+        self.handle = h
+
+    def close(self) -> None:
+        if self.handle:
+            lib.aj_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def join(self, dR: int, nR: int, dS: int, nS: int, algo: int = AJ_ALGO_PRO,
+             radix_bits: int = 0, passes: int = 0, cap: int = 0, split: int = 0,
+             pairs_ptr: int = 0, pairs_capacity: int = 0, stream: int = 0) -> aj_join_out_t:
+        """Join device-resident tuple arrays (device pointers as ints)."""
+        o = aj_join_opts_t(algo, radix_bits, passes, cap, split, pairs_ptr or None,
+                           pairs_capacity)
+        out = aj_join_out_t()
+        check(lib.aj_join_dev(self.handle, dR or None, nR, dS or None, nS, ctypes.byref(o),
+                              ctypes.byref(out), stream or None))
+        return out
+
+
+class DeviceArray:
+    """A device allocation made through the C ABI (aj_device_alloc)."""
+
+    def __init__(self, nbytes: int):
+        p = ctypes.c_void_p()
+        check(lib.aj_device_alloc(ctypes.byref(p), int(nbytes)))
+        self.ptr = int(p.value or 0)
+        self.nbytes = int(nbytes)
+
+    @classmethod
+    def from_host(cls, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(a.nbytes)
+        if a.nbytes:
+            check(lib.aj_copy_h2d(d.ptr, a.ctypes.data, a.nbytes))
+        return d
+
+    def to_host(self, dtype, count: int) -> np.ndarray:
+        out = np.empty(count, dtype=dtype)
+        if out.nbytes:
+            check(lib.aj_copy_d2h(out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            lib.aj_device_free(self.ptr)
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def device_sync() -> None:
+    check(lib.aj_device_sync())
+
+
+def gen_unique(n: int, seed: int) -> np.ndarray:
+    t = np.empty(n, dtype=TUPLE_DTYPE)
+    check(lib.aj_gen_unique(_ptr(t), n, seed))
+    return t
+
+
+def gen_uniform(n: int, lo: int, hi: int, seed: int) -> np.ndarray:
+    t = np.empty(n, dtype=TUPLE_DTYPE)
+    check(lib.aj_gen_uniform(_ptr(t), n, lo, hi, seed))
+    return t
+
+
+def gen_zipf(n: int, nkeys: int, theta: float, seed: int) -> np.ndarray:
+    t = np.empty(n, dtype=TUPLE_DTYPE)
+    check(lib.aj_gen_zipf(_ptr(t), n, nkeys, float(theta), seed))
+    return t

@@ -1,0 +1,155 @@
+// aj_internal.h -- shared declarations of the HIP join library (not installed).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstddef>
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/alliancedb/aj.h"
+
+namespace aj {
+
+// ---------------------------------------------------------------- errors
+void set_error(int code, const char *fmt, ...);
+int last_status();
+
+#define AJ_HIP(call)                                                              \
+    do {                                                                          \
+        hipError_t e_ = (call);                                                   \
+        if (e_ != hipSuccess) {                                                   \
+            ::aj::set_error(AJ_ERR_HIP, "%s failed at %s:%d: %s", #call, __FILE__, \
+                            __LINE__, hipGetErrorString(e_));                     \
+            return AJ_ERR_HIP;                                                    \
+        }                                                                         \
+    } while (0)
+
+#define AJ_TRY(expr)                \
+    do {                            \
+        int rc_ = (expr);           \
+        if (rc_ != AJ_OK) return rc_; \
+    } while (0)
+
+// ---------------------------------------------------------------- memory
+// A device buffer that only grows; reused across calls so that repeated joins
+// of the same size do not allocate.
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t bytes);
+    void release();
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+// Allocation hook (tests inject failures / budgets through aj_set_alloc_hook).
+int dev_malloc(void **p, size_t bytes);
+void dev_free(void *p);
+
+// ---------------------------------------------------------------- tuning
+struct ProConfig {
+    int bits1 = 0;        // radix bits of pass 1 (0 = derive from sizes)
+    int bits2 = 0;        // radix bits of pass 2 (0 = single pass)
+    int cap = 8192;       // R tuples per LDS build round
+    int split = 1 << 16;  // S tuples per join task
+    int chunk = 1 << 17;  // tuples per partition work-group
+};
+ProConfig choose_pro_config(uint64_t nR, uint64_t nS, const aj_join_opts_t *opts);
+
+// ---------------------------------------------------------------- context
+struct Ctx {
+    int device = 0;
+    int num_cu = 256;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    // scratch reused across calls
+    DevBuf inR, inS;            // uploaded inputs (drop-in path)
+    DevBuf p1R, p1S, p2R, p2S;  // partitioned copies
+    DevBuf hist, scan_tmp, plan, seg, tasks, acc, pairs, table, misc;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+Ctx *default_ctx(int *status);
+
+// ---------------------------------------------------------------- kernels
+// Segment description of one partition pass, all arrays on the device.
+//   in_begin/in_end[2][nseg]: input ranges of R (0) and S (1)
+//   bits[nseg]: radix bits of each segment for this pass; shift: digit shift
+struct PassDesc {
+    const void *in[2];        // input tuples (8-byte or 12-byte stream tuples)
+    int in_stride;            // 8 or 12 bytes
+    uint64_t *out[2];         // output 8-byte tuples, segment-major / partition-major
+    const uint64_t *in_begin[2];
+    const uint64_t *in_end[2];
+    const uint32_t *bits;     // per segment
+    int nseg;
+    int shift;
+    int max_bits;
+    uint32_t chunk;           // tuples per work-group (multiple of the tile)
+    uint64_t n_total[2];      // total tuples of R and S in this pass
+    uint64_t max_chunks;      // upper bound of chunks over R+S (grid size)
+    uint64_t max_hist;        // upper bound of histogram entries
+    uint64_t max_parts;       // upper bound of partitions per relation
+};
+
+// Workspace carved out of ctx buffers by the pass (device pointers).
+struct PassWork {
+    uint32_t *hist;           // max_hist
+    uint64_t *seg_chunk_pre[2];   // nseg+1 exclusive chunk prefix per relation
+    uint64_t *seg_hist_base[2];   // nseg histogram base per relation
+    uint64_t *seg_out_base[2];    // nseg+1 output base per relation
+    uint32_t *seg_part_base;      // nseg+1 partition base (shared by R and S)
+    uint64_t *header;             // [0]=chunksR [1]=chunksS [2]=histR [3]=histS [4]=parts
+    uint64_t *part_off[2];        // max_parts+1 partition offsets per relation
+    void *scan_tmp;
+    size_t scan_tmp_bytes;
+};
+
+size_t pass_scan_tmp_bytes(uint64_t max_hist);
+int run_partition_pass(const PassDesc &d, const PassWork &w, hipStream_t st);
+
+// Join of partition pairs (R part_off, S part_off) into per-segment results.
+struct JoinDesc {
+    const uint64_t *R, *S;          // partitioned tuples
+    const uint64_t *roff, *soff;    // partition offsets (nparts+1), device
+    const uint64_t *nparts_dev;     // number of partitions (device scalar)
+    uint64_t max_parts;
+    const uint32_t *part_seg_base;  // nseg+1: first partition of each segment
+    int nseg;
+    int hash_shift;                 // unused bits below this are partition bits
+    int cap;                        // R tuples per LDS round
+    uint32_t split;                 // S tuples per task
+    uint64_t *seg_count, *seg_sum;  // nseg accumulators (zeroed by the caller)
+    uint64_t *pairs;                // nullable: materialise pairs
+    uint64_t pairs_cap;
+    uint64_t *pair_cursor;          // device scalar (zeroed by the caller)
+    uint32_t *task_cnt;             // workspace: max_parts+1
+    uint32_t *task_base;            // workspace: max_parts+1
+    uint32_t *task_next;            // device scalar (zeroed by the caller)
+    void *scan_tmp;
+    size_t scan_tmp_bytes;
+    int num_cu;
+};
+size_t join_scan_tmp_bytes(uint64_t max_parts);
+int run_join(const JoinDesc &d, hipStream_t st);
+
+// NPO global table
+struct NpoDesc {
+    const uint64_t *R, *S;
+    uint64_t nR, nS;
+    uint64_t *table;     // capacity slots
+    uint64_t capacity;   // power of two
+    uint64_t *acc;       // [0]=count [1]=sum [2]=sentinel R tuples [3]=pair cursor
+    uint64_t *pairs;
+    uint64_t pairs_cap;
+    int num_cu;
+};
+int run_npo(const NpoDesc &d, hipStream_t st);
+
+// fill kernel (graph-safe replacement of hipMemsetAsync)
+int fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t st);
+int fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st);
+
+}  // namespace aj

@@ -1,0 +1,454 @@
+// api.cpp -- the C ABI: errors, options, contexts, drop-in PRO/NPO, device API.
+#include <atomic>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+
+#include "aj_internal.h"
+
+namespace aj {
+
+int pro_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, uint64_t nS,
+               const ProConfig &cfg, uint64_t *pairs, uint64_t pairs_cap, aj_join_out_t *out,
+               hipStream_t st);
+int npo_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, uint64_t nS,
+               uint64_t *pairs, uint64_t pairs_cap, aj_join_out_t *out, hipStream_t st);
+
+// ---------------------------------------------------------------- errors
+static thread_local char g_err[512] = "";
+static thread_local int g_status = AJ_OK;
+
+void set_error(int code, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    g_status = code;
+}
+int last_status() { return g_status; }
+static void clear_error() {
+    g_err[0] = 0;
+    g_status = AJ_OK;
+}
+
+// ---------------------------------------------------------------- memory
+static std::atomic<uint64_t> g_dev_budget{0};  // 0 = unlimited
+static std::atomic<uint64_t> g_dev_used{0};
+
+int dev_malloc(void **p, size_t bytes) {
+    *p = nullptr;
+    const uint64_t budget = g_dev_budget.load();
+    if (budget && g_dev_used.load() + bytes > budget) {
+        set_error(AJ_ERR_NOMEM, "device budget exceeded: %zu bytes requested, %llu of %llu used",
+                  bytes, (unsigned long long)g_dev_used.load(), (unsigned long long)budget);
+        return AJ_ERR_NOMEM;
+    }
+    hipError_t e = hipMalloc(p, bytes ? bytes : 1);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        set_error(AJ_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+        return AJ_ERR_NOMEM;
+    }
+    g_dev_used += bytes;
+    return AJ_OK;
+}
+
+void dev_free(void *p) {
+    if (p) (void)hipFree(p);
+}
+
+int DevBuf::ensure(size_t bytes) {
+    if (bytes <= cap && p) return AJ_OK;
+    if (p) {
+        dev_free(p);
+        g_dev_used -= cap;
+        p = nullptr;
+        cap = 0;
+    }
+    // grow with headroom so that slowly growing sizes do not reallocate each call
+    size_t want = bytes + bytes / 8;
+    int rc = dev_malloc(&p, want);
+    if (rc != AJ_OK) {
+        rc = dev_malloc(&p, bytes);
+        if (rc != AJ_OK) return rc;
+        want = bytes;
+    }
+    cap = want;
+    return AJ_OK;
+}
+
+void DevBuf::release() {
+    if (p) {
+        dev_free(p);
+        g_dev_used -= cap;
+    }
+    p = nullptr;
+    cap = 0;
+}
+
+// ---------------------------------------------------------------- options
+static std::atomic<int64_t> opt_materialize{0}, opt_radix_bits{0}, opt_passes{0}, opt_device{0},
+    opt_cap{0}, opt_split{0};
+
+static int ctx_init(Ctx *c, int device) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0) {
+        (void)hipGetLastError();
+        set_error(AJ_ERR_NODEV, "no HIP device available");
+        return AJ_ERR_NODEV;
+    }
+    if (device < 0 || device >= n) {
+        set_error(AJ_ERR_ARG, "device %d out of range (%d devices)", device, n);
+        return AJ_ERR_ARG;
+    }
+    c->device = device;
+    AJ_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    AJ_HIP(hipGetDeviceProperties(&prop, device));
+    c->num_cu = prop.multiProcessorCount;
+    AJ_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    AJ_HIP(hipEventCreate(&c->ev0));
+    AJ_HIP(hipEventCreate(&c->ev1));
+    return AJ_OK;
+}
+
+static void ctx_fini(Ctx *c) {
+    (void)hipSetDevice(c->device);
+    DevBuf *bufs[] = {&c->inR, &c->inS, &c->p1R, &c->p1S, &c->p2R, &c->p2S, &c->hist,
+                      &c->scan_tmp, &c->plan, &c->seg, &c->tasks, &c->acc, &c->pairs,
+                      &c->table, &c->misc};
+    for (DevBuf *b : bufs) b->release();
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+static std::mutex g_default_mu;
+static Ctx *g_default = nullptr;
+
+Ctx *default_ctx(int *status) {
+    std::lock_guard<std::mutex> lk(g_default_mu);
+    if (!g_default) {
+        Ctx *c = new Ctx();
+        int rc = ctx_init(c, (int)opt_device.load());
+        if (rc != AJ_OK) {
+            delete c;
+            *status = rc;
+            return nullptr;
+        }
+        g_default = c;
+    }
+    *status = AJ_OK;
+    return g_default;
+}
+
+static ProConfig config_from(uint64_t nR, uint64_t nS, const aj_join_opts_t *o) {
+    aj_join_opts_t eff;
+    memset(&eff, 0, sizeof(eff));
+    if (o) eff = *o;
+    if (!eff.radix_bits) eff.radix_bits = (int)opt_radix_bits.load();
+    if (!eff.passes) eff.passes = (int)opt_passes.load();
+    if (!eff.cap) eff.cap = (int)opt_cap.load();
+    if (!eff.split) eff.split = (uint32_t)opt_split.load();
+    return choose_pro_config(nR, nS, &eff);
+}
+
+static int join_dev_locked(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS,
+                           uint64_t nS, const aj_join_opts_t *opts, aj_join_out_t *out,
+                           hipStream_t st) {
+    memset(out, 0, sizeof(*out));
+    if (nR == 0 || nS == 0) return AJ_OK;
+    const int algo = opts ? opts->algo : AJ_ALGO_PRO;
+    uint64_t *pairs = opts ? opts->pairs : nullptr;
+    const uint64_t pcap = opts ? opts->pairs_capacity : 0;
+    int rc;
+    if (algo == AJ_ALGO_NPO) {
+        rc = npo_device(ctx, dR, nR, dS, nS, pairs, pcap, out, st);
+    } else if (algo == AJ_ALGO_PRO) {
+        rc = pro_device(ctx, dR, nR, dS, nS, config_from(nR, nS, opts), pairs, pcap, out, st);
+    } else {
+        set_error(AJ_ERR_ARG, "unknown algorithm %d", algo);
+        return AJ_ERR_ARG;
+    }
+    if (rc != AJ_OK) return rc;
+    if (pairs && out->count > pcap) {
+        set_error(AJ_ERR_CAPACITY, "pair buffer holds %llu pairs, %llu needed",
+                  (unsigned long long)pcap, (unsigned long long)out->count);
+        return AJ_ERR_CAPACITY;
+    }
+    return AJ_OK;
+}
+
+static int check_relation(const relation_t *r, const char *name) {
+    if (!r) {
+        set_error(AJ_ERR_ARG, "relation %s is NULL", name);
+        return AJ_ERR_ARG;
+    }
+    if (r->num_tuples && !r->tuples) {
+        set_error(AJ_ERR_ARG, "relation %s has %llu tuples but no tuple array", name,
+                  (unsigned long long)r->num_tuples);
+        return AJ_ERR_ARG;
+    }
+    if (r->num_tuples >= (1ull << 31)) {
+        set_error(AJ_ERR_ARG, "relation %s too large (%llu tuples, limit 2^31-1)", name,
+                  (unsigned long long)r->num_tuples);
+        return AJ_ERR_ARG;
+    }
+    return AJ_OK;
+}
+
+static result_t *dropin(int algo, relation_t *relR, relation_t *relS, int nthreads) {
+    clear_error();
+    if (check_relation(relR, "R") || check_relation(relS, "S")) return nullptr;
+    int st = AJ_OK;
+    Ctx *ctx = default_ctx(&st);
+    if (!ctx) return nullptr;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (hipSetDevice(ctx->device) != hipSuccess) {
+        set_error(AJ_ERR_HIP, "hipSetDevice failed");
+        return nullptr;
+    }
+    const uint64_t nR = relR->num_tuples, nS = relS->num_tuples;
+    auto t0 = std::chrono::steady_clock::now();
+    aj_join_out_t out;
+    memset(&out, 0, sizeof(out));
+    const bool mat = opt_materialize.load() != 0;
+    uint64_t npairs = 0;
+    std::vector<uint64_t> host_pairs;
+    int rc = AJ_OK;
+    do {
+        if (nR == 0 || nS == 0) break;
+        if ((rc = ctx->inR.ensure(nR * 8))) break;
+        if ((rc = ctx->inS.ensure(nS * 8))) break;
+        if (hipMemcpyAsync(ctx->inR.p, relR->tuples, nR * 8, hipMemcpyHostToDevice, ctx->stream) ||
+            hipMemcpyAsync(ctx->inS.p, relS->tuples, nS * 8, hipMemcpyHostToDevice, ctx->stream)) {
+            set_error(AJ_ERR_HIP, "host-to-device copy failed");
+            rc = AJ_ERR_HIP;
+            break;
+        }
+        aj_join_opts_t o;
+        memset(&o, 0, sizeof(o));
+        o.algo = algo;
+        rc = join_dev_locked(ctx, ctx->inR.as<uint64_t>(), nR, ctx->inS.as<uint64_t>(), nS, &o,
+                             &out, ctx->stream);
+        if (rc || !mat || out.count == 0) break;
+        if ((rc = ctx->pairs.ensure(out.count * 8))) break;
+        o.pairs = ctx->pairs.as<uint64_t>();
+        o.pairs_capacity = out.count;
+        aj_join_out_t out2;
+        rc = join_dev_locked(ctx, ctx->inR.as<uint64_t>(), nR, ctx->inS.as<uint64_t>(), nS, &o,
+                             &out2, ctx->stream);
+        if (rc) break;
+        npairs = out2.npairs;
+        host_pairs.resize(npairs);
+        if (hipMemcpy(host_pairs.data(), ctx->pairs.p, npairs * 8, hipMemcpyDeviceToHost)) {
+            set_error(AJ_ERR_HIP, "device-to-host copy of pairs failed");
+            rc = AJ_ERR_HIP;
+        }
+    } while (0);
+    if (rc) return nullptr;
+    aj_result_t *res = (aj_result_t *)malloc(sizeof(aj_result_t) + npairs * sizeof(uint64_t));
+    if (!res) {
+        set_error(AJ_ERR_NOMEM, "host allocation of the result failed");
+        return nullptr;
+    }
+    res->base.totalresults = (int64_t)out.count;
+    res->base.resultlist = nullptr;
+    res->base.nthreads = nthreads;
+    res->checksum = out.checksum;
+    res->npairs = npairs;
+    res->pairs = npairs ? reinterpret_cast<uint64_t *>(res + 1) : nullptr;
+    if (npairs) memcpy(res->pairs, host_pairs.data(), npairs * 8);
+    res->status = AJ_OK;
+    res->device_ms = out.device_ms;
+    res->total_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0)
+                        .count();
+    return &res->base;
+}
+
+}  // namespace aj
+
+using namespace aj;
+
+struct aj_ctx {
+    Ctx c;
+};
+
+extern "C" {
+
+const char *aj_last_error(void) { return g_err; }
+int aj_last_status(void) { return g_status; }
+
+result_t *PRO(relation_t *relR, relation_t *relS, int nthreads) {
+    return dropin(AJ_ALGO_PRO, relR, relS, nthreads);
+}
+
+result_t *NPO(relation_t *relR, relation_t *relS, int nthreads) {
+    return dropin(AJ_ALGO_NPO, relR, relS, nthreads);
+}
+
+const aj_result_t *aj_result_ext(const result_t *r) {
+    return reinterpret_cast<const aj_result_t *>(r);
+}
+
+int aj_set_option(const char *name, int64_t value) {
+    clear_error();
+    if (!name) {
+        set_error(AJ_ERR_ARG, "option name is NULL");
+        return AJ_ERR_ARG;
+    }
+    struct {
+        const char *n;
+        std::atomic<int64_t> *v;
+        int64_t lo, hi;
+    } tab[] = {{"materialize", &opt_materialize, 0, 1}, {"radix_bits", &opt_radix_bits, 0, 24},
+               {"passes", &opt_passes, 0, 2},           {"device", &opt_device, 0, 1024},
+               {"cap", &opt_cap, 0, 12288},             {"split", &opt_split, 0, 1ll << 30}};
+    for (auto &e : tab) {
+        if (strcmp(e.n, name) == 0) {
+            if (value < e.lo || value > e.hi) {
+                set_error(AJ_ERR_ARG, "option %s out of range [%lld, %lld]", name,
+                          (long long)e.lo, (long long)e.hi);
+                return AJ_ERR_ARG;
+            }
+            e.v->store(value);
+            return AJ_OK;
+        }
+    }
+    if (strcmp(name, "device_budget") == 0) {
+        if (value < 0) {
+            set_error(AJ_ERR_ARG, "device_budget must be >= 0");
+            return AJ_ERR_ARG;
+        }
+        g_dev_budget.store((uint64_t)value);
+        return AJ_OK;
+    }
+    set_error(AJ_ERR_ARG, "unknown option '%s'", name);
+    return AJ_ERR_ARG;
+}
+
+int64_t aj_get_option(const char *name) {
+    if (!name) return -1;
+    if (!strcmp(name, "materialize")) return opt_materialize.load();
+    if (!strcmp(name, "radix_bits")) return opt_radix_bits.load();
+    if (!strcmp(name, "passes")) return opt_passes.load();
+    if (!strcmp(name, "device")) return opt_device.load();
+    if (!strcmp(name, "cap")) return opt_cap.load();
+    if (!strcmp(name, "split")) return opt_split.load();
+    if (!strcmp(name, "device_budget")) return (int64_t)g_dev_budget.load();
+    if (!strcmp(name, "device_used")) return (int64_t)g_dev_used.load();
+    return -1;
+}
+
+static int ensure_default_device() {
+    int st = AJ_OK;
+    Ctx *c = default_ctx(&st);
+    if (!c) return st;
+    AJ_HIP(hipSetDevice(c->device));
+    return AJ_OK;
+}
+
+int aj_device_alloc(void **ptr, uint64_t bytes) {
+    clear_error();
+    if (!ptr) {
+        set_error(AJ_ERR_ARG, "ptr is NULL");
+        return AJ_ERR_ARG;
+    }
+    *ptr = nullptr;
+    AJ_TRY(ensure_default_device());
+    if (hipMalloc(ptr, bytes ? bytes : 1) != hipSuccess) {
+        (void)hipGetLastError();
+        *ptr = nullptr;
+        set_error(AJ_ERR_NOMEM, "hipMalloc(%llu) failed", (unsigned long long)bytes);
+        return AJ_ERR_NOMEM;
+    }
+    return AJ_OK;
+}
+
+int aj_device_free(void *ptr) {
+    clear_error();
+    if (!ptr) return AJ_OK;
+    AJ_TRY(ensure_default_device());
+    AJ_HIP(hipFree(ptr));
+    return AJ_OK;
+}
+
+int aj_copy_h2d(void *dst, const void *src, uint64_t bytes) {
+    clear_error();
+    if (!bytes) return AJ_OK;
+    if (!dst || !src) {
+        set_error(AJ_ERR_ARG, "NULL pointer");
+        return AJ_ERR_ARG;
+    }
+    AJ_TRY(ensure_default_device());
+    AJ_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return AJ_OK;
+}
+
+int aj_copy_d2h(void *dst, const void *src, uint64_t bytes) {
+    clear_error();
+    if (!bytes) return AJ_OK;
+    if (!dst || !src) {
+        set_error(AJ_ERR_ARG, "NULL pointer");
+        return AJ_ERR_ARG;
+    }
+    AJ_TRY(ensure_default_device());
+    AJ_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return AJ_OK;
+}
+
+int aj_device_sync(void) {
+    clear_error();
+    AJ_TRY(ensure_default_device());
+    AJ_HIP(hipDeviceSynchronize());
+    return AJ_OK;
+}
+
+int aj_ctx_create(int device, aj_ctx_t **out) {
+    clear_error();
+    if (!out) {
+        set_error(AJ_ERR_ARG, "out is NULL");
+        return AJ_ERR_ARG;
+    }
+    *out = nullptr;
+    std::unique_ptr<aj_ctx> c(new aj_ctx());
+    int rc = ctx_init(&c->c, device);
+    if (rc != AJ_OK) return rc;
+    *out = c.release();
+    return AJ_OK;
+}
+
+int aj_ctx_destroy(aj_ctx_t *ctx) {
+    clear_error();
+    if (!ctx) return AJ_OK;
+    {
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        ctx_fini(&ctx->c);
+    }
+    delete ctx;
+    return AJ_OK;
+}
+
+int aj_join_dev(aj_ctx_t *ctx, const tuple_t *dR, uint64_t nR, const tuple_t *dS, uint64_t nS,
+                const aj_join_opts_t *opts, aj_join_out_t *out, void *stream) {
+    clear_error();
+    if (!ctx || !out || (nR && !dR) || (nS && !dS)) {
+        set_error(AJ_ERR_ARG, "aj_join_dev: NULL argument");
+        return AJ_ERR_ARG;
+    }
+    if (nR >= (1ull << 31) || nS >= (1ull << 31)) {
+        set_error(AJ_ERR_ARG, "aj_join_dev: relation too large (limit 2^31-1 tuples)");
+        return AJ_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
+    AJ_HIP(hipSetDevice(ctx->c.device));
+    hipStream_t st = stream ? (hipStream_t)stream : ctx->c.stream;
+    return join_dev_locked(&ctx->c, reinterpret_cast<const uint64_t *>(dR), nR,
+                           reinterpret_cast<const uint64_t *>(dS), nS, opts, out, st);
+}
+
+}  // extern "C"

@@ -1,0 +1,336 @@
+// pipeline.hip -- device pipelines behind the C ABI: PRO (1 or 2 partition
+// passes + LDS join) and NPO (global table), on device-resident 8-byte tuples.
+#include <algorithm>
+#include <cstring>
+
+#include "aj_internal.h"
+#include "device_common.h"
+
+namespace aj {
+
+constexpr uint32_t PART_TILE = 8192;  // must match the partition tile
+
+// Bump layout of many small device arrays inside one DevBuf.
+struct Layout {
+    size_t off = 0;
+    template <class T> size_t add(uint64_t n) {
+        size_t o = off;
+        off += ((n * sizeof(T) + 255) / 256) * 256;
+        return o;
+    }
+};
+template <class T> static T *at(const DevBuf &b, size_t off) {
+    return reinterpret_cast<T *>(static_cast<char *>(b.p) + off);
+}
+
+// Writes the one-segment tables of pass 1 and zeroes the scalars.
+__global__ void k_setup_single(uint64_t *ib0, uint64_t *ie0, uint64_t *ib1, uint64_t *ie1,
+                               uint32_t *bits, uint64_t nR, uint64_t nS, uint32_t b1,
+                               uint32_t *part_seg_base, uint32_t parts, uint64_t *scalars,
+                               int nscalars) {
+    if (threadIdx.x == 0) {
+        ib0[0] = 0;
+        ie0[0] = nR;
+        ib1[0] = 0;
+        ie1[0] = nS;
+        bits[0] = b1;
+        part_seg_base[0] = 0;
+        part_seg_base[1] = parts;
+    }
+    for (int i = threadIdx.x; i < nscalars; i += blockDim.x) scalars[i] = 0;
+}
+
+// Offsets of the "no partitioning" case: one partition holding everything.
+__global__ void k_setup_nopart(uint64_t *roff, uint64_t *soff, uint64_t nR, uint64_t nS,
+                               uint32_t *part_seg_base, uint64_t *scalars, int nscalars) {
+    if (threadIdx.x == 0) {
+        roff[0] = 0;
+        roff[1] = nR;
+        soff[0] = 0;
+        soff[1] = nS;
+        part_seg_base[0] = 0;
+        part_seg_base[1] = 1;
+    }
+    for (int i = threadIdx.x; i < nscalars; i += blockDim.x) scalars[i] = 0;
+}
+
+static uint32_t pick_chunk(uint64_t n_total, int num_cu) {
+    uint64_t want = n_total / ((uint64_t)num_cu * 4);
+    want = (want + PART_TILE - 1) / PART_TILE * PART_TILE;
+    want = std::max<uint64_t>(want, PART_TILE);
+    want = std::min<uint64_t>(want, 1ull << 20);
+    return (uint32_t)want;
+}
+
+ProConfig choose_pro_config(uint64_t nR, uint64_t nS, const aj_join_opts_t *o) {
+    ProConfig c;
+    if (o && o->cap) c.cap = o->cap;
+    if (o && o->split) c.split = o->split;
+    int bits;
+    if (o && o->radix_bits > 0) {
+        bits = o->radix_bits;
+    } else {
+        // average R partition of about half an LDS round
+        bits = 0;
+        const uint64_t target = (uint64_t)c.cap / 2;
+        while (bits < 24 && (nR >> bits) > target) ++bits;
+    }
+    int passes = (o && o->passes > 0) ? o->passes : (bits > 11 ? 2 : 1);
+    if (bits == 0) passes = 0;
+    if (passes == 1) {
+        c.bits1 = bits;
+        c.bits2 = 0;
+    } else if (passes == 2) {
+        c.bits1 = (bits + 1) / 2;
+        c.bits2 = bits - c.bits1;
+    }
+    (void)nS;
+    return c;
+}
+
+struct ProPlan {
+    // pass geometry
+    int b1, b2;
+    uint32_t chunk1, chunk2;
+    uint64_t max_chunks1, max_hist1, max_parts1;
+    uint64_t max_chunks2, max_hist2, max_parts2;
+    uint64_t parts_final;
+    // layout offsets in ctx->plan
+    size_t o_ib[2], o_ie[2], o_bits1, o_bits2;
+    size_t o_cpre[2], o_hb[2], o_ob[2], o_pb, o_hdr, o_poff1[2];
+    size_t o_cpre2[2], o_hb2[2], o_ob2[2], o_pb2, o_hdr2, o_poff2[2];
+    size_t o_tcnt, o_tbase, o_psb, o_scal;
+    size_t bytes;
+    size_t scan_bytes;
+};
+
+static ProPlan plan_pro(uint64_t nR, uint64_t nS, const ProConfig &c, int num_cu) {
+    ProPlan p;
+    memset(&p, 0, sizeof(p));
+    p.b1 = c.bits1;
+    p.b2 = c.bits2;
+    const uint64_t F1 = 1ull << p.b1, F2 = 1ull << p.b2;
+    const bool part = (p.b1 + p.b2) > 0;
+    p.chunk1 = pick_chunk(nR + nS, num_cu);
+    p.max_chunks1 = (nR + p.chunk1 - 1) / p.chunk1 + (nS + p.chunk1 - 1) / p.chunk1;
+    p.max_hist1 = F1 * p.max_chunks1;
+    p.max_parts1 = F1;
+    p.parts_final = part ? F1 : 1;
+    if (p.b2) {
+        p.chunk2 = p.chunk1;
+        p.max_chunks2 = (nR + p.chunk2 - 1) / p.chunk2 + (nS + p.chunk2 - 1) / p.chunk2 + 2 * F1;
+        p.max_hist2 = F2 * p.max_chunks2;
+        p.max_parts2 = F1 * F2;
+        p.parts_final = F1 * F2;
+    }
+    Layout L;
+    for (int r = 0; r < 2; ++r) {
+        p.o_ib[r] = L.add<uint64_t>(1);
+        p.o_ie[r] = L.add<uint64_t>(1);
+    }
+    p.o_bits1 = L.add<uint32_t>(1);
+    p.o_bits2 = L.add<uint32_t>(F1);
+    const uint64_t nseg1 = 1, nseg2 = F1;
+    for (int r = 0; r < 2; ++r) {
+        p.o_cpre[r] = L.add<uint64_t>(nseg1 + 1);
+        p.o_hb[r] = L.add<uint64_t>(nseg1);
+        p.o_ob[r] = L.add<uint64_t>(nseg1 + 1);
+        p.o_poff1[r] = L.add<uint64_t>(p.max_parts1 + 2);
+        p.o_cpre2[r] = L.add<uint64_t>(nseg2 + 1);
+        p.o_hb2[r] = L.add<uint64_t>(nseg2);
+        p.o_ob2[r] = L.add<uint64_t>(nseg2 + 1);
+        p.o_poff2[r] = L.add<uint64_t>(p.max_parts2 + 2);
+    }
+    p.o_pb = L.add<uint32_t>(nseg1 + 1);
+    p.o_pb2 = L.add<uint32_t>(nseg2 + 1);
+    p.o_hdr = L.add<uint64_t>(8);
+    p.o_hdr2 = L.add<uint64_t>(8);
+    const uint64_t jparts = p.parts_final;
+    p.o_tcnt = L.add<uint32_t>(jparts + 1);
+    p.o_tbase = L.add<uint32_t>(jparts + 1);
+    p.o_psb = L.add<uint32_t>(2);
+    p.o_scal = L.add<uint64_t>(8);
+    p.bytes = L.off;
+    p.scan_bytes = std::max({pass_scan_tmp_bytes(std::max<uint64_t>(p.max_hist1, 1)),
+                             pass_scan_tmp_bytes(std::max<uint64_t>(p.max_hist2, 1)),
+                             join_scan_tmp_bytes(jparts)});
+    return p;
+}
+
+// scalars: [0]=count [1]=checksum [2]=pair cursor [3]=task counter (u32 in low half)
+int pro_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, uint64_t nS,
+               const ProConfig &cfg, uint64_t *pairs, uint64_t pairs_cap, aj_join_out_t *out,
+               hipStream_t st) {
+    ProPlan p = plan_pro(nR, nS, cfg, ctx->num_cu);
+    AJ_TRY(ctx->plan.ensure(p.bytes));
+    AJ_TRY(ctx->scan_tmp.ensure(p.scan_bytes));
+    AJ_TRY(ctx->hist.ensure(sizeof(uint32_t) * 2 * std::max(p.max_hist1, p.max_hist2) + 256));
+    const bool part = (p.b1 + p.b2) > 0;
+    if (part) {
+        AJ_TRY(ctx->p1R.ensure(nR * 8 + 256));
+        AJ_TRY(ctx->p1S.ensure(nS * 8 + 256));
+        if (p.b2) {
+            AJ_TRY(ctx->p2R.ensure(nR * 8 + 256));
+            AJ_TRY(ctx->p2S.ensure(nS * 8 + 256));
+        }
+    }
+    const DevBuf &B = ctx->plan;
+    uint64_t *scal = at<uint64_t>(B, p.o_scal);
+    uint32_t *psb = at<uint32_t>(B, p.o_psb);
+
+    AJ_HIP(hipEventRecord(ctx->ev0, st));
+    const uint64_t *jR = dR, *jS = dS, *roff, *soff;
+    if (!part) {
+        roff = at<uint64_t>(B, p.o_poff1[0]);
+        soff = at<uint64_t>(B, p.o_poff1[1]);
+        hipLaunchKernelGGL(k_setup_nopart, dim3(1), dim3(64), 0, st, (uint64_t *)roff,
+                           (uint64_t *)soff, nR, nS, psb, scal, 8);
+    } else {
+        hipLaunchKernelGGL(k_setup_single, dim3(1), dim3(64), 0, st, at<uint64_t>(B, p.o_ib[0]),
+                           at<uint64_t>(B, p.o_ie[0]), at<uint64_t>(B, p.o_ib[1]),
+                           at<uint64_t>(B, p.o_ie[1]), at<uint32_t>(B, p.o_bits1), nR, nS,
+                           (uint32_t)p.b1, psb, (uint32_t)p.parts_final, scal, 8);
+        PassDesc d1;
+        d1.in[0] = dR;
+        d1.in[1] = dS;
+        d1.in_stride = 8;
+        d1.out[0] = ctx->p1R.as<uint64_t>();
+        d1.out[1] = ctx->p1S.as<uint64_t>();
+        for (int r = 0; r < 2; ++r) {
+            d1.in_begin[r] = at<uint64_t>(B, p.o_ib[r]);
+            d1.in_end[r] = at<uint64_t>(B, p.o_ie[r]);
+        }
+        d1.bits = at<uint32_t>(B, p.o_bits1);
+        d1.nseg = 1;
+        d1.shift = 0;
+        d1.max_bits = p.b1;
+        d1.chunk = p.chunk1;
+        d1.n_total[0] = nR;
+        d1.n_total[1] = nS;
+        d1.max_chunks = p.max_chunks1;
+        d1.max_hist = p.max_hist1;
+        d1.max_parts = p.max_parts1;
+        PassWork w1;
+        w1.hist = ctx->hist.as<uint32_t>();
+        for (int r = 0; r < 2; ++r) {
+            w1.seg_chunk_pre[r] = at<uint64_t>(B, p.o_cpre[r]);
+            w1.seg_hist_base[r] = at<uint64_t>(B, p.o_hb[r]);
+            w1.seg_out_base[r] = at<uint64_t>(B, p.o_ob[r]);
+            w1.part_off[r] = at<uint64_t>(B, p.o_poff1[r]);
+        }
+        w1.seg_part_base = at<uint32_t>(B, p.o_pb);
+        w1.header = at<uint64_t>(B, p.o_hdr);
+        w1.scan_tmp = ctx->scan_tmp.p;
+        w1.scan_tmp_bytes = p.scan_bytes;
+        AJ_TRY(run_partition_pass(d1, w1, st));
+        jR = ctx->p1R.as<uint64_t>();
+        jS = ctx->p1S.as<uint64_t>();
+        roff = w1.part_off[0];
+        soff = w1.part_off[1];
+        if (p.b2) {
+            AJ_TRY(fill_u32(at<uint32_t>(B, p.o_bits2), 1ull << p.b1, (uint32_t)p.b2, st));
+            PassDesc d2 = d1;
+            d2.in[0] = jR;
+            d2.in[1] = jS;
+            d2.out[0] = ctx->p2R.as<uint64_t>();
+            d2.out[1] = ctx->p2S.as<uint64_t>();
+            for (int r = 0; r < 2; ++r) {
+                d2.in_begin[r] = w1.part_off[r];
+                d2.in_end[r] = w1.part_off[r] + 1;
+            }
+            d2.bits = at<uint32_t>(B, p.o_bits2);
+            d2.nseg = 1 << p.b1;
+            d2.shift = p.b1;
+            d2.max_bits = p.b2;
+            d2.chunk = p.chunk2;
+            d2.max_chunks = p.max_chunks2;
+            d2.max_hist = p.max_hist2;
+            d2.max_parts = p.max_parts2;
+            PassWork w2 = w1;
+            for (int r = 0; r < 2; ++r) {
+                w2.seg_chunk_pre[r] = at<uint64_t>(B, p.o_cpre2[r]);
+                w2.seg_hist_base[r] = at<uint64_t>(B, p.o_hb2[r]);
+                w2.seg_out_base[r] = at<uint64_t>(B, p.o_ob2[r]);
+                w2.part_off[r] = at<uint64_t>(B, p.o_poff2[r]);
+            }
+            w2.seg_part_base = at<uint32_t>(B, p.o_pb2);
+            w2.header = at<uint64_t>(B, p.o_hdr2);
+            AJ_TRY(run_partition_pass(d2, w2, st));
+            jR = ctx->p2R.as<uint64_t>();
+            jS = ctx->p2S.as<uint64_t>();
+            roff = w2.part_off[0];
+            soff = w2.part_off[1];
+        }
+    }
+    JoinDesc j;
+    j.R = jR;
+    j.S = jS;
+    j.roff = roff;
+    j.soff = soff;
+    j.nparts_dev = nullptr;
+    j.max_parts = p.parts_final;
+    j.part_seg_base = psb;
+    j.nseg = 1;
+    j.hash_shift = p.b1 + p.b2;
+    j.cap = cfg.cap;
+    j.split = cfg.split;
+    j.seg_count = scal + 0;
+    j.seg_sum = scal + 1;
+    j.pairs = pairs;
+    j.pairs_cap = pairs_cap;
+    j.pair_cursor = scal + 2;
+    j.task_cnt = at<uint32_t>(B, p.o_tcnt);
+    j.task_base = at<uint32_t>(B, p.o_tbase);
+    j.task_next = reinterpret_cast<uint32_t *>(scal + 3);
+    j.scan_tmp = ctx->scan_tmp.p;
+    j.scan_tmp_bytes = p.scan_bytes;
+    j.num_cu = ctx->num_cu;
+    AJ_TRY(run_join(j, st));
+    AJ_HIP(hipEventRecord(ctx->ev1, st));
+    uint64_t host[3];
+    AJ_HIP(hipMemcpyAsync(host, scal, sizeof(host), hipMemcpyDeviceToHost, st));
+    AJ_HIP(hipStreamSynchronize(st));
+    float ms = 0;
+    AJ_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    out->count = host[0];
+    out->checksum = host[1];
+    out->npairs = pairs ? host[2] : 0;
+    out->device_ms = ms;
+    return AJ_OK;
+}
+
+int npo_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, uint64_t nS,
+               uint64_t *pairs, uint64_t pairs_cap, aj_join_out_t *out, hipStream_t st) {
+    uint64_t cap = 2;
+    while (cap <= 2 * nR) cap <<= 1;
+    AJ_TRY(ctx->table.ensure(cap * 8));
+    AJ_TRY(ctx->acc.ensure(8 * 8));
+    uint64_t *acc = ctx->acc.as<uint64_t>();
+    AJ_HIP(hipEventRecord(ctx->ev0, st));
+    AJ_TRY(fill_u64(acc, 8, 0, st));
+    NpoDesc d;
+    d.R = dR;
+    d.S = dS;
+    d.nR = nR;
+    d.nS = nS;
+    d.table = ctx->table.as<uint64_t>();
+    d.capacity = cap;
+    d.acc = acc;
+    d.pairs = pairs;
+    d.pairs_cap = pairs_cap;
+    d.num_cu = ctx->num_cu;
+    AJ_TRY(run_npo(d, st));
+    AJ_HIP(hipEventRecord(ctx->ev1, st));
+    uint64_t host[4];
+    AJ_HIP(hipMemcpyAsync(host, acc, sizeof(host), hipMemcpyDeviceToHost, st));
+    AJ_HIP(hipStreamSynchronize(st));
+    float ms = 0;
+    AJ_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    out->count = host[0];
+    out->checksum = host[1];
+    out->npairs = pairs ? host[3] : 0;
+    out->device_ms = ms;
+    return AJ_OK;
+}
+
+}  // namespace aj

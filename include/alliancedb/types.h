@@ -1,0 +1,72 @@
+/*
+ * alliancedb/types.h -- ABI types of the lazy intra-window join path.
+ *
+ * These are byte-identical to the reference's relation/result types so that a
+ * caller built against the reference links against this library unchanged:
+ *
+ *   tuple_t            hashing/utils/types.h:67-71   8 B: {int32 payloadID @0, int32 key @4}
+ *   relation_t         hashing/utils/types.h:77-81
+ *   relation_payload_t hashing/utils/types.h:84-88
+ *   threadresult_t     hashing/utils/types.h:91-95
+ *   result_t           hashing/utils/types.h:98-102
+ *
+ * Read as a little-endian uint64, a tuple is (key << 32) | payloadID.
+ * Keys and payloads are compared and combined as 32-bit bit patterns; the
+ * signed declaration only matters to callers, never to join results.
+ */
+#ifndef ALLIANCEDB_TYPES_H
+#define ALLIANCEDB_TYPES_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t intkey_t;
+typedef int32_t value_t;
+
+typedef struct tuple_t {
+    value_t payloadID; /* row id of the tuple's payload */
+    intkey_t key;      /* join key */
+} tuple_t;
+
+typedef struct relation_payload_t {
+    uint64_t *ts;        /* per-tuple timestamp, indexed by payloadID */
+    uint64_t num_tuples;
+} relation_payload_t;
+
+typedef struct relation_t {
+    tuple_t *tuples;
+    uint64_t num_tuples;
+    relation_payload_t *payload;
+} relation_t;
+
+typedef struct threadresult_t {
+    int64_t nresults;
+    void *results;
+    uint32_t threadid;
+} threadresult_t;
+
+typedef struct result_t {
+    int64_t totalresults;       /* number of matching (r, s) pairs */
+    threadresult_t *resultlist; /* NULL: this library never hands out per-thread lists */
+    int nthreads;
+} result_t;
+
+/*
+ * 12-byte stream tuple used by the window paths (batched windows, serving).
+ * The reference keeps the timestamp in a side array (relation_payload_t.ts);
+ * the window paths keep it inline so one coalesced read gives the window id.
+ */
+typedef struct aj_stream_tuple_t {
+    uint32_t key;
+    uint32_t payload;
+    uint32_t ts; /* arrival time in microseconds */
+} aj_stream_tuple_t;
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ALLIANCEDB_TYPES_H */

@@ -1,0 +1,176 @@
+"""CPU oracle for the lazy equi-join path -- TEST INFRASTRUCTURE.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this package, and only as the checker.  Wraps:
+  liboracle.so     clean-room C restatement (oracle.c): PRO, NPO, aggregate,
+                   sort-merge and brute-force pair lists
+  _ref/libref.so   the reference's own RJ_st / NPO_st compiled from its sources
+                   (present where the reference tree was available at build time)
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_PATH = os.path.join(_HERE, "liboracle.so")
+REF_PATH = os.path.join(_HERE, "_ref", "libref.so")
+
+TUPLE_DTYPE = np.dtype([("payloadID", "<i4"), ("key", "<i4")])
+
+
+class _Out(ctypes.Structure):
+    _fields_ = [("count", ctypes.c_uint64), ("checksum", ctypes.c_uint64),
+                ("npairs", ctypes.c_uint64), ("part_ms", ctypes.c_double),
+                ("join_ms", ctypes.c_double), ("total_ms", ctypes.c_double)]
+
+
+@dataclass
+class OracleResult:
+    count: int
+    checksum: int
+    pairs: Optional[np.ndarray]
+    part_ms: float = 0.0
+    join_ms: float = 0.0
+    total_ms: float = 0.0
+
+
+_lib = None
+_ref = None
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_PATH):
+            raise ImportError(f"{ORACLE_PATH} not built (make -C oracle)")
+        L = ctypes.CDLL(ORACLE_PATH)
+        P, U64 = ctypes.c_void_p, ctypes.c_uint64
+        L.oj_pro.argtypes = [P, U64, P, U64, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                             ctypes.POINTER(_Out), P, U64]
+        L.oj_npo.argtypes = [P, U64, P, U64, ctypes.c_int, ctypes.POINTER(_Out), P, U64]
+        L.oj_aggregate.argtypes = [P, U64, P, U64, ctypes.POINTER(U64), ctypes.POINTER(U64)]
+        L.oj_pairs_sortmerge.argtypes = [P, U64, P, U64, P, U64]
+        L.oj_pairs_sortmerge.restype = ctypes.c_int64
+        L.oj_pairs_bruteforce.argtypes = [P, U64, P, U64, P, U64]
+        L.oj_pairs_bruteforce.restype = ctypes.c_int64
+        L.oj_sort_u64.argtypes = [P, U64]
+        _lib = L
+    return _lib
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_PATH)
+
+
+def ref() -> ctypes.CDLL:
+    global _ref
+    if _ref is None:
+        L = ctypes.CDLL(REF_PATH)
+        P, U64 = ctypes.c_void_p, ctypes.c_uint64
+        L.ref_join.argtypes = [ctypes.c_int, P, U64, P, U64, ctypes.POINTER(ctypes.c_int64), P,
+                               U64, ctypes.POINTER(U64)]
+        _ref = L
+    return _ref
+
+
+def _c(a: np.ndarray) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=TUPLE_DTYPE)
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data if a.size else None
+
+
+def pro(R, S, nthreads=8, radix_bits=8, passes=2, pairs=False) -> OracleResult:
+    R, S = _c(R), _c(S)
+    out = _Out()
+    buf = None
+    cap = 0
+    if pairs:
+        cnt, _ = aggregate(R, S)
+        cap = cnt
+        buf = np.empty(max(cap, 1), np.uint64)
+    rc = lib().oj_pro(_p(R), len(R), _p(S), len(S), nthreads, radix_bits, passes,
+                      ctypes.byref(out), None if buf is None else buf.ctypes.data, cap)
+    if rc:
+        raise RuntimeError(f"oj_pro failed: {rc}")
+    return OracleResult(out.count, out.checksum, None if buf is None else buf[:out.npairs],
+                        out.part_ms, out.join_ms, out.total_ms)
+
+
+def npo(R, S, nthreads=8, pairs=False) -> OracleResult:
+    R, S = _c(R), _c(S)
+    out = _Out()
+    buf = None
+    cap = 0
+    if pairs:
+        cnt, _ = aggregate(R, S)
+        cap = cnt
+        buf = np.empty(max(cap, 1), np.uint64)
+    rc = lib().oj_npo(_p(R), len(R), _p(S), len(S), nthreads, ctypes.byref(out),
+                      None if buf is None else buf.ctypes.data, cap)
+    if rc:
+        raise RuntimeError(f"oj_npo failed: {rc}")
+    return OracleResult(out.count, out.checksum, None if buf is None else buf[:out.npairs],
+                        out.part_ms, out.join_ms, out.total_ms)
+
+
+def aggregate(R, S):
+    """(count, checksum) from key groups alone -- no join algorithm involved."""
+    R, S = _c(R), _c(S)
+    c, x = ctypes.c_uint64(), ctypes.c_uint64()
+    rc = lib().oj_aggregate(_p(R), len(R), _p(S), len(S), ctypes.byref(c), ctypes.byref(x))
+    if rc:
+        raise RuntimeError(f"oj_aggregate failed: {rc}")
+    return int(c.value), int(x.value)
+
+
+def pairs_sortmerge(R, S) -> np.ndarray:
+    R, S = _c(R), _c(S)
+    cnt, _ = aggregate(R, S)
+    buf = np.empty(max(cnt, 1), np.uint64)
+    n = lib().oj_pairs_sortmerge(_p(R), len(R), _p(S), len(S), buf.ctypes.data, cnt)
+    if n < 0:
+        raise RuntimeError("oj_pairs_sortmerge failed")
+    return buf[:n]
+
+
+def pairs_bruteforce(R, S) -> np.ndarray:
+    R, S = _c(R), _c(S)
+    cap = len(R) * len(S)
+    buf = np.empty(max(cap, 1), np.uint64)
+    n = lib().oj_pairs_bruteforce(_p(R), len(R), _p(S), len(S), buf.ctypes.data, cap)
+    if n < 0:
+        raise RuntimeError("oj_pairs_bruteforce failed")
+    return buf[:n]
+
+
+def sort_u64(a: np.ndarray) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.uint64).copy()
+    lib().oj_sort_u64(a.ctypes.data if a.size else None, len(a))
+    return a
+
+
+def ref_join(algo: str, R, S, s_list: bool = False):
+    """The reference's own RJ_st ("RJ_st") or NPO_st ("NPO_st") on copies of R, S.
+    Returns (count, sorted list of (S.key << 32 | S.payloadID) per match or None)."""
+    R, S = _c(R), _c(S)
+    a = {"RJ_st": 0, "NPO_st": 1}[algo]
+    cnt = ctypes.c_int64()
+    n = ctypes.c_uint64()
+    buf = None
+    cap = 0
+    if s_list:
+        cap, _ = aggregate(R, S)
+        buf = np.empty(max(cap, 1), np.uint64)
+    rc = ref().ref_join(a, _p(R), len(R), _p(S), len(S), ctypes.byref(cnt),
+                        None if buf is None else buf.ctypes.data, cap, ctypes.byref(n))
+    if rc:
+        raise RuntimeError(f"ref_join failed: {rc}")
+    lst = None if buf is None else np.sort(buf[:n.value])
+    return int(cnt.value), lst

@@ -1,0 +1,74 @@
+// ref_driver.cpp -- calls the reference's own single-threaded joins
+// (TEST INFRASTRUCTURE, built into oracle/_ref/ by oracle/Makefile from the
+// reference sources where they lie; nothing here is copied from them).
+//
+//   RJ_st   hashing/joins/parallel_radix_join.cpp:1719-1844: two-pass radix
+//           clustering + bucket_chaining_join (the same JoinFunction PRO uses,
+//           parallel_radix_join.cpp:291-354).
+//   NPO_st  hashing/joins/no_partitioning_join.cpp:112-147: the same
+//           allocate_hashtable / probe_hashtable code NPO uses
+//           (common_functions.cpp:74-102, 221-280).
+//
+// The multithreaded PRO/NPO entry points cannot run outside the authors'
+// machine (thread 0 programs Intel PCM MSRs and writes to /data1/xtra without
+// a NULL check, parallel_radix_join.cpp:1036-1047), so these two single-thread
+// variants are what pins the oracle's counts.  The objects are compiled with
+// JOIN_RESULT_MATERIALIZE, so every match also yields the reference's own
+// materialised <S.key, S.payloadID> record (parallel_radix_join.cpp:333-338,
+// common_functions.cpp:252-258), which pins the S half of the pair list.
+#include "benchmark.h"
+#include "joins/no_partitioning_join.h"
+#include "joins/parallel_radix_join.h"
+#include "utils/tuple_buffer.h"
+
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+
+extern "C" {
+
+// algo: 0 = RJ_st, 1 = NPO_st.  R/S are arrays of 8-byte tuples; they are
+// copied first because RJ_st's second pass writes into its inputs.
+// s_list (nullable) receives (S.key << 32 | S.payloadID) of every match.
+// Returns 0, or -1 on bad arguments, -3 if s_list is too small.
+int ref_join(int algo, const void *R, uint64_t nR, const void *S, uint64_t nS,
+             int64_t *count, uint64_t *s_list, uint64_t cap, uint64_t *nlist) {
+    if (algo < 0 || algo > 1 || !count) return -1;
+    const size_t pad = 64 * 1024;
+    relation_t r, s;
+    r.tuples = (tuple_t *)malloc(nR * sizeof(tuple_t) + pad);
+    s.tuples = (tuple_t *)malloc(nS * sizeof(tuple_t) + pad);
+    if (nR) memcpy(r.tuples, R, nR * sizeof(tuple_t));
+    if (nS) memcpy(s.tuples, S, nS * sizeof(tuple_t));
+    r.num_tuples = nR;
+    s.num_tuples = nS;
+    r.payload = s.payload = nullptr;
+    param_t p;
+    memset(&p, 0, sizeof(p));
+    result_t *res = algo == 0 ? RJ_st(&r, &s, p) : NPO_st(&r, &s, p);
+    *count = res->totalresults;
+    int rc = 0;
+    uint64_t n = 0;
+    chainedtuplebuffer_t *cb = (chainedtuplebuffer_t *)res->resultlist[0].results;
+    // newest buffer first, holding writepos records; older buffers are full
+    uint32_t len = cb->writepos;
+    for (tuplebuffer_t *b = cb->buf; b; b = b->next, len = CHAINEDBUFF_NUMTUPLESPERBUF) {
+        for (uint32_t i = 0; i < len; i++) {
+            if (s_list) {
+                if (n >= cap) { rc = -3; break; }
+                s_list[n] = ((uint64_t)(uint32_t)b->tuples[i].key << 32) |
+                            (uint32_t)b->tuples[i].payloadID;
+            }
+            n++;
+        }
+    }
+    if (nlist) *nlist = n;
+    chainedtuplebuffer_free(cb);
+    free(res->resultlist);
+    free(res);
+    free(r.tuples);
+    free(s.tuples);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,130 @@
+"""GPU parity: PRO and NPO through the C ABI vs the CPU oracle.
+
+Bit-exact bar: count, 64-bit wrapping checksum of (R.payload << 32 | S.payload),
+and the sorted pair list.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SEED_R, SEED_S = 12345, 54321
+
+
+def _inputs(aj, kind, n):
+    if kind == "unique_uniform":  # BASELINE workload shape
+        return aj.gen_unique(n, SEED_R), aj.gen_uniform(n, 1, max(n, 1), SEED_S)
+    if kind == "dups":  # duplicate factor ~8 on both sides
+        k = max(n // 8, 1)
+        return aj.gen_uniform(n, 1, k, 1), aj.gen_uniform(n, 1, k, 2)
+    if kind == "one_key":
+        return aj.as_tuples(np.full(n, 7)), aj.as_tuples(np.full(n, 7))
+    if kind == "disjoint":
+        return aj.as_tuples(np.arange(n) * 2), aj.as_tuples(np.arange(n) * 2 + 1)
+    if kind == "negative":  # full 32-bit range incl. negative keys
+        rng = np.random.default_rng(3)
+        keys = rng.integers(-2**31, 2**31, max(n // 4, 1), dtype=np.int64).astype(np.int32)
+        return (aj.as_tuples(rng.choice(keys, n)), aj.as_tuples(rng.choice(keys, n)))
+    if kind == "all_ones":  # tuples whose bit pattern is all ones (key -1, payload -1)
+        R = aj.as_tuples(np.full(n, -1), np.full(n, -1))
+        R["key"][::3] = 5
+        S = aj.as_tuples(np.full(n, -1))
+        S["key"][1::2] = 5
+        return R, S
+    if kind == "zipf":
+        return aj.gen_unique(n, SEED_R), aj.gen_zipf(4 * n, n, 1.0, SEED_S)
+    raise ValueError(kind)
+
+
+KINDS = ["unique_uniform", "dups", "one_key", "disjoint", "negative", "all_ones", "zipf"]
+
+
+@pytest.mark.parametrize("algo", ["PRO", "NPO"])
+@pytest.mark.parametrize("kind", KINDS)
+@pytest.mark.parametrize("n", [1, 1000, 1 << 16])
+def test_dropin_count_checksum(aj, oracle, algo, kind, n):
+    if kind == "one_key" and n > 1000:
+        n = 1000  # |R|*|S| pairs
+    R, S = _inputs(aj, kind, n)
+    print(f"case {algo} {kind} n={n}", flush=True)
+    got = getattr(aj, algo)(R, S, 8)
+    assert (got.count, got.checksum) == oracle.aggregate(R, S)
+
+
+@pytest.mark.parametrize("algo", ["PRO", "NPO"])
+def test_empty_relations(aj, algo):
+    R = aj.gen_unique(100, 1)
+    E = np.empty(0, dtype=aj.TUPLE_DTYPE)
+    for a, b in ((R, E), (E, R), (E, E)):
+        got = getattr(aj, algo)(a, b, 4)
+        assert (got.count, got.checksum) == (0, 0)
+
+
+@pytest.mark.parametrize("bits,passes", [(0, 1), (4, 1), (8, 1), (13, 1), (8, 2), (14, 2),
+                                         (18, 2)])
+@pytest.mark.parametrize("cap", [4096, 8192, 12288])
+def test_pro_geometry_invariance(aj, oracle, bits, passes, cap):
+    """Results do not depend on radix bits, pass count or LDS round size."""
+    n = 1 << 18
+    R, S = aj.gen_uniform(n, 1, n // 2, 11), aj.gen_uniform(n, 1, n // 2, 12)
+    want = oracle.aggregate(R, S)
+    dR, dS = aj.DeviceArray.from_host(R), aj.DeviceArray.from_host(S)
+    print(f"case bits={bits} passes={passes} cap={cap}", flush=True)
+    with aj.Context(0) as ctx:
+        out = ctx.join(dR.ptr, n, dS.ptr, n, aj.AJ_ALGO_PRO, radix_bits=bits,
+                       passes=passes, cap=cap)
+    assert (out.count, out.checksum) == want
+
+
+@pytest.mark.parametrize("algo", ["PRO", "NPO"])
+@pytest.mark.parametrize("kind", ["unique_uniform", "dups", "all_ones"])
+def test_materialized_pairs(aj, oracle, algo, kind):
+    R, S = _inputs(aj, kind, 5000)
+    aj.set_option("materialize", 1)
+    try:
+        got = getattr(aj, algo)(R, S, 8)
+    finally:
+        aj.set_option("materialize", 0)
+    ref = oracle.pairs_sortmerge(R, S)
+    assert got.pairs is not None and len(got.pairs) == got.count == len(ref)
+    assert np.array_equal(np.sort(got.pairs), ref)
+
+
+def test_pro_split_heavy_partition(aj, oracle):
+    """A hot key puts most of S in one partition: S is split over many tasks."""
+    n = 1 << 18
+    R = aj.gen_unique(n, 5)
+    S = aj.gen_zipf(1 << 21, n, 1.5, 6)
+    want = oracle.aggregate(R, S)
+    dR, dS = aj.DeviceArray.from_host(R), aj.DeviceArray.from_host(S)
+    with aj.Context(0) as ctx:
+        for split in (1024, 65536):
+            out = ctx.join(dR.ptr, len(R), dS.ptr, len(S), aj.AJ_ALGO_PRO, split=split)
+            assert (out.count, out.checksum) == want
+
+
+def test_device_pairs_capacity_error(aj, oracle):
+    R, S = _inputs(aj, "dups", 4096)
+    cnt, _ = oracle.aggregate(R, S)
+    dR, dS = aj.DeviceArray.from_host(R), aj.DeviceArray.from_host(S)
+    buf = aj.DeviceArray(8 * (cnt // 2))
+    with aj.Context(0) as ctx:
+        with pytest.raises(aj.AjError) as e:
+            ctx.join(dR.ptr, len(R), dS.ptr, len(S), aj.AJ_ALGO_PRO,
+                     pairs_ptr=buf.ptr, pairs_capacity=cnt // 2)
+        assert e.value.status == aj.AJ_ERR_CAPACITY
+
+
+def test_baseline_shape_closed_form(aj):
+    """|R|=|S|=2^24 of the BASELINE key distribution: every S tuple matches the one R
+    tuple with its key, so count = |S| and the checksum follows from R's inverse."""
+    n = 1 << 24
+    R, S = aj.gen_unique(n, SEED_R), aj.gen_uniform(n, 1, n, SEED_S)
+    inv = np.empty(n + 1, np.uint64)
+    inv[R["key"]] = R["payloadID"].astype(np.uint64)
+    want_sum = int((inv[S["key"]] << np.uint64(32)).sum(dtype=np.uint64)
+                   + S["payloadID"].astype(np.uint64).sum(dtype=np.uint64)) % (1 << 64)
+    for fn in (aj.PRO, aj.NPO):
+        got = fn(R, S, 8)
+        assert got.count == n
+        assert got.checksum == want_sum
