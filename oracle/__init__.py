@@ -170,6 +170,8 @@ def ref_join(algo: str, R, S, s_list: bool = False):
         buf = np.empty(max(cap, 1), np.uint64)
     rc = ref().ref_join(a, _p(R), len(R), _p(S), len(S), ctypes.byref(cnt),
                         None if buf is None else buf.ctypes.data, cap, ctypes.byref(n))
+    if rc == -5:
+        raise ValueError("the reference's NPO_st is undefined for |R| < 2")
     if rc:
         raise RuntimeError(f"ref_join failed: {rc}")
     lst = None if buf is None else np.sort(buf[:n.value])

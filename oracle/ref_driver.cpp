@@ -34,6 +34,10 @@ extern "C" {
 int ref_join(int algo, const void *R, uint64_t nR, const void *S, uint64_t nS,
              int64_t *count, uint64_t *s_list, uint64_t cap, uint64_t *nlist) {
     if (algo < 0 || algo > 1 || !count) return -1;
+    // NPO_st sizes its table as NEXT_POW_2(|R| / BUCKET_SIZE): for |R| < 2 that
+    // is zero buckets and the reference dereferences outside the table
+    // (no_partitioning_join.cpp:118, common_functions.cpp:80-101).
+    if (algo == 1 && nR < 2) return -5;
     const size_t pad = 64 * 1024;
     relation_t r, s;
     r.tuples = (tuple_t *)malloc(nR * sizeof(tuple_t) + pad);
