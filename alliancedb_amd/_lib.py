@@ -54,9 +54,23 @@ class aj_join_opts_t(ctypes.Structure):
                 ("pairs", ctypes.c_void_p), ("pairs_capacity", ctypes.c_uint64)]
 
 
+AJ_NPHASES = 12
+PHASE_NAMES = ["plan1", "hist1", "scan1", "scatter1", "plan2", "hist2", "scan2", "scatter2",
+               "tasks", "join", "npo_build", "npo_probe"]
+
+
 class aj_join_out_t(ctypes.Structure):
     _fields_ = [("count", ctypes.c_uint64), ("checksum", ctypes.c_uint64),
-                ("npairs", ctypes.c_uint64), ("device_ms", ctypes.c_float)]
+                ("npairs", ctypes.c_uint64), ("device_ms", ctypes.c_float),
+                ("phase_ms", ctypes.c_float * AJ_NPHASES)]
+
+    def phases(self) -> dict:
+        return {n: float(self.phase_ms[i]) for i, n in enumerate(PHASE_NAMES)}
+
+
+class aj_batch_opts_t(ctypes.Structure):
+    _fields_ = [("cap", ctypes.c_int), ("split", ctypes.c_uint32), ("target", ctypes.c_uint32),
+                ("chunk", ctypes.c_uint32), ("no_graph", ctypes.c_int)]
 
 
 class AjError(RuntimeError):
@@ -89,6 +103,12 @@ def _load() -> ctypes.CDLL:
         "aj_copy_h2d": (c.c_int, [P, P, c.c_uint64]),
         "aj_copy_d2h": (c.c_int, [P, P, c.c_uint64]),
         "aj_device_sync": (c.c_int, []),
+        "aj_batch_plan_create": (c.c_int, [P, c.c_uint64, c.c_uint64, c.c_int, c.c_uint64,
+                                           c.POINTER(aj_batch_opts_t), c.POINTER(P)]),
+        "aj_batch_plan_destroy": (c.c_int, [P]),
+        "aj_batch_run": (c.c_int, [P, P, c.c_uint64, P, c.c_uint64, P, c.c_int, P, P, P]),
+        "aj_gen_window_streams": (c.c_int, [P, c.c_uint64, c.c_int, P, c.c_double, c.c_uint32,
+                                            P, c.c_uint64, P, c.POINTER(c.c_uint64)]),
         "aj_gen_unique": (c.c_int, [P, c.c_uint64, c.c_uint64]),
         "aj_gen_uniform": (c.c_int, [P, c.c_uint64, c.c_uint32, c.c_uint32, c.c_uint64]),
         "aj_gen_zipf": (c.c_int, [P, c.c_uint64, c.c_uint32, c.c_double, c.c_uint64]),
@@ -201,6 +221,47 @@ class Context:
         check(lib.aj_join_dev(self.handle, dR or None, nR, dS or None, nS, ctypes.byref(o),
                               ctypes.byref(out), stream or None))
         return out
+
+
+class BatchPlan:
+    """aj_batch_plan_t: batched tumbling-window joins captured in one hipGraph."""
+
+    def __init__(self, ctx: "Context", max_R: int, max_S: int, max_windows: int,
+                 max_window_tuples: int = 0, cap: int = 0, split: int = 0, target: int = 0,
+                 chunk: int = 0, graph: bool = True):
+        self.ctx = ctx
+        o = aj_batch_opts_t(cap, split, target, chunk, 0 if graph else 1)
+        h = ctypes.c_void_p()
+        check(lib.aj_batch_plan_create(ctx.handle, max_R, max_S, max_windows,
+                                       max_window_tuples, ctypes.byref(o), ctypes.byref(h)))
+        self.handle = h
+
+    def run(self, dR: int, nR: int, dS: int, nS: int, d_bounds: int, nwin: int, d_count: int,
+            d_checksum: int, stream: int = 0) -> None:
+        check(lib.aj_batch_run(self.handle, dR or None, nR, dS or None, nS, d_bounds, nwin,
+                               d_count, d_checksum, stream or None))
+
+    def close(self) -> None:
+        if self.handle:
+            lib.aj_batch_plan_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def gen_window_streams(ctx: "Context", seed: int, sizes: np.ndarray, rate_per_us: float,
+                       dup: int, d_out: int, capacity: int, d_bounds: int) -> int:
+    """Device stream of 12-byte tuples for tumbling windows of the given sizes."""
+    sizes = np.ascontiguousarray(sizes, dtype=np.uint32)
+    n = ctypes.c_uint64()
+    check(lib.aj_gen_window_streams(ctx.handle, seed, len(sizes), sizes.ctypes.data,
+                                    float(rate_per_us), dup, d_out, capacity, d_bounds,
+                                    ctypes.byref(n)))
+    return int(n.value)
 
 
 class DeviceArray:

@@ -70,8 +70,20 @@ struct Ctx {
     DevBuf p1R, p1S, p2R, p2S;  // partitioned copies
     DevBuf hist, scan_tmp, plan, seg, tasks, acc, pairs, table, misc;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t evk[AJ_NPHASES + 1] = {};  // per-kernel timing (option kernel_timing)
+};
+bool kernel_timing_enabled();
+
+// Optional event marks between kernels: mark(i) records event i on the stream.
+struct Marks {
+    hipEvent_t *ev = nullptr;  // nullptr = timing off
+    hipStream_t st = nullptr;
+    void mark(int i) const {
+        if (ev) (void)hipEventRecord(ev[i], st);
+    }
 };
 Ctx *default_ctx(int *status);
+Ctx *ctx_of(aj_ctx_t *c);
 
 // ---------------------------------------------------------------- kernels
 // Segment description of one partition pass, all arrays on the device.
@@ -108,7 +120,10 @@ struct PassWork {
 };
 
 size_t pass_scan_tmp_bytes(uint64_t max_hist);
-int run_partition_pass(const PassDesc &d, const PassWork &w, hipStream_t st);
+// marks (nullable): m0 after planning, m0+1 after the histogram, m0+2 after the
+// scan, m0+3 after the scatter
+int run_partition_pass(const PassDesc &d, const PassWork &w, hipStream_t st,
+                       const Marks *marks = nullptr, int m0 = 0);
 
 // Join of partition pairs (R part_off, S part_off) into per-segment results.
 struct JoinDesc {
@@ -133,7 +148,8 @@ struct JoinDesc {
     int num_cu;
 };
 size_t join_scan_tmp_bytes(uint64_t max_parts);
-int run_join(const JoinDesc &d, hipStream_t st);
+// marks (nullable): m0 after task planning, m0+1 after the join kernel
+int run_join(const JoinDesc &d, hipStream_t st, const Marks *marks = nullptr, int m0 = 0);
 
 // NPO global table
 struct NpoDesc {
@@ -146,7 +162,7 @@ struct NpoDesc {
     uint64_t pairs_cap;
     int num_cu;
 };
-int run_npo(const NpoDesc &d, hipStream_t st);
+int run_npo(const NpoDesc &d, hipStream_t st, const Marks *marks = nullptr, int m0 = 0);
 
 // fill kernel (graph-safe replacement of hipMemsetAsync)
 int fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t st);

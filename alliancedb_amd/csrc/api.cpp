@@ -91,7 +91,9 @@ void DevBuf::release() {
 
 // ---------------------------------------------------------------- options
 static std::atomic<int64_t> opt_materialize{0}, opt_radix_bits{0}, opt_passes{0}, opt_device{0},
-    opt_cap{0}, opt_split{0};
+    opt_cap{0}, opt_split{0}, opt_kernel_timing{0};
+
+bool kernel_timing_enabled() { return opt_kernel_timing.load() != 0; }
 
 static int ctx_init(Ctx *c, int device) {
     int n = 0;
@@ -124,6 +126,8 @@ static void ctx_fini(Ctx *c) {
     for (DevBuf *b : bufs) b->release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    for (hipEvent_t &e : c->evk)
+        if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
 }
 
@@ -278,6 +282,10 @@ struct aj_ctx {
     Ctx c;
 };
 
+namespace aj {
+Ctx *ctx_of(aj_ctx_t *c) { return &c->c; }
+}  // namespace aj
+
 extern "C" {
 
 const char *aj_last_error(void) { return g_err; }
@@ -307,7 +315,8 @@ int aj_set_option(const char *name, int64_t value) {
         int64_t lo, hi;
     } tab[] = {{"materialize", &opt_materialize, 0, 1}, {"radix_bits", &opt_radix_bits, 0, 24},
                {"passes", &opt_passes, 0, 2},           {"device", &opt_device, 0, 1024},
-               {"cap", &opt_cap, 0, 12288},             {"split", &opt_split, 0, 1ll << 30}};
+               {"cap", &opt_cap, 0, 12288},             {"split", &opt_split, 0, 1ll << 30},
+               {"kernel_timing", &opt_kernel_timing, 0, 1}};
     for (auto &e : tab) {
         if (strcmp(e.n, name) == 0) {
             if (value < e.lo || value > e.hi) {
@@ -339,6 +348,7 @@ int64_t aj_get_option(const char *name) {
     if (!strcmp(name, "device")) return opt_device.load();
     if (!strcmp(name, "cap")) return opt_cap.load();
     if (!strcmp(name, "split")) return opt_split.load();
+    if (!strcmp(name, "kernel_timing")) return opt_kernel_timing.load();
     if (!strcmp(name, "device_budget")) return (int64_t)g_dev_budget.load();
     if (!strcmp(name, "device_used")) return (int64_t)g_dev_used.load();
     return -1;

@@ -30,6 +30,14 @@ namespace aj {
 constexpr int JBLOCK = 512;
 constexpr int JUNROLL = 4;
 
+// Bucket count of a full round: the smallest power of two nb with 2*nb >= CAP
+// (build_round's rule at m = CAP); the bucket-start array holds nb+1 entries.
+constexpr uint32_t max_buckets(uint32_t cap) {
+    uint32_t nb = 1;
+    while (2 * nb < cap) nb *= 2;
+    return nb;
+}
+
 struct JoinArgs {
     const uint64_t *R, *S, *roff, *soff;
     const uint32_t *part_seg_base;
@@ -137,7 +145,7 @@ template <int CAP, bool PAIRS>
 __global__ __launch_bounds__(JBLOCK) void k_join(JoinArgs a) {
     extern __shared__ uint64_t jsm[];
     uint64_t *tab = jsm;                                     // CAP tuples
-    uint32_t *bst = reinterpret_cast<uint32_t *>(tab + CAP);  // CAP/2 + 1 bucket starts
+    uint32_t *bst = reinterpret_cast<uint32_t *>(tab + CAP);  // max_buckets(CAP) + 1 starts
     __shared__ uint32_t s_task;
     __shared__ uint64_t red[JBLOCK / 64];
     __shared__ uint32_t scan_tmp[JBLOCK / 64];
@@ -194,7 +202,7 @@ size_t join_scan_tmp_bytes(uint64_t max_parts) {
 }
 
 template <int CAP, bool PAIRS> static int launch_join(const JoinArgs &a, int num_cu, hipStream_t st) {
-    const size_t lds = sizeof(uint64_t) * CAP + sizeof(uint32_t) * (CAP / 2 + 2);
+    const size_t lds = sizeof(uint64_t) * CAP + sizeof(uint32_t) * (max_buckets(CAP) + 2);
     static bool attr = false;
     if (!attr) {
         AJ_HIP(hipFuncSetAttribute((const void *)k_join<CAP, PAIRS>,
@@ -208,7 +216,7 @@ template <int CAP, bool PAIRS> static int launch_join(const JoinArgs &a, int num
     return AJ_OK;
 }
 
-int run_join(const JoinDesc &d, hipStream_t st) {
+int run_join(const JoinDesc &d, hipStream_t st, const Marks *mk, int m0) {
     if (d.split == 0 || d.max_parts >= (1ull << 31)) {
         set_error(AJ_ERR_ARG, "bad join split/partition count");
         return AJ_ERR_ARG;
@@ -239,6 +247,7 @@ int run_join(const JoinDesc &d, hipStream_t st) {
         set_error(AJ_ERR_HIP, "task scan failed: %s", hipGetErrorString(e));
         return AJ_ERR_HIP;
     }
+    if (mk) mk->mark(m0);
     const bool pairs = d.pairs != nullptr;
     int rc;
     switch (d.cap) {
@@ -250,6 +259,7 @@ int run_join(const JoinDesc &d, hipStream_t st) {
             return AJ_ERR_ARG;
     }
     if (rc) return rc;
+    if (mk) mk->mark(m0 + 1);
     AJ_HIP(hipGetLastError());
     return AJ_OK;
 }

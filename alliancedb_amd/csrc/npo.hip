@@ -153,16 +153,18 @@ __global__ __launch_bounds__(NBLOCK) void k_npo_probe(const uint64_t *S, uint64_
     }
 }
 
-int run_npo(const NpoDesc &d, hipStream_t st) {
+int run_npo(const NpoDesc &d, hipStream_t st, const Marks *mk, int m0) {
     if (d.capacity < 2 || (d.capacity & (d.capacity - 1)) || d.capacity <= d.nR) {
         set_error(AJ_ERR_ARG, "NPO table capacity must be a power of two above |R|");
         return AJ_ERR_ARG;
     }
     AJ_TRY(fill_u64(d.table, d.capacity, EMPTY_SLOT, st));
     const uint64_t mask = d.capacity - 1;
+    if (mk) mk->mark(m0);
     if (d.nR)
         hipLaunchKernelGGL(k_npo_build, dim3(grid_for(d.nR, d.num_cu)), dim3(NBLOCK), 0, st, d.R,
                            d.nR, d.table, mask, d.acc);
+    if (mk) mk->mark(m0 + 1);
     if (d.nS) {
         if (d.pairs)
             hipLaunchKernelGGL(k_npo_probe<true>, dim3(grid_for(d.nS, d.num_cu)), dim3(NBLOCK), 0,
@@ -173,6 +175,7 @@ int run_npo(const NpoDesc &d, hipStream_t st) {
                                st, d.S, d.nS, (const uint64_t *)d.table, mask, d.acc, nullptr,
                                (uint64_t)0);
     }
+    if (mk) mk->mark(m0 + 2);
     AJ_HIP(hipGetLastError());
     return AJ_OK;
 }

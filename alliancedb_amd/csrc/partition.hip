@@ -182,7 +182,8 @@ __global__ __launch_bounds__(PBLOCK) void k_scatter(PassArgs a) {
     uint32_t *lofs = cursor + F;                               // F + 1
     const void *in = L.rel ? a.in1 : a.in0;
     uint64_t *out = L.rel ? a.out1 : a.out0;
-    const uint32_t rel_base = L.rel ? (uint32_t)a.n_total0 : 0u;
+    // S offsets follow every R tuple that lies in a segment
+    const uint32_t rel_base = L.rel ? (uint32_t)a.obase0[a.nseg] : 0u;
     const uint64_t hb = (L.rel ? a.hbase1 : a.hbase0)[L.seg];
     for (uint32_t d = threadIdx.x; d < F; d += PBLOCK)
         cursor[d] = a.hist_scan[hb + d * L.nchunk + L.c] - rel_base;
@@ -231,7 +232,7 @@ __global__ void k_part_off(PassArgs a) {
     for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p <= a.max_parts;
          p += (uint64_t)gridDim.x * blockDim.x) {
         if (p > parts) {
-            a.poff0[p] = a.n_total0;
+            a.poff0[p] = a.obase0[a.nseg];
             a.poff1[p] = a.obase1[a.nseg];
             continue;
         }
@@ -247,7 +248,7 @@ __global__ void k_part_off(PassArgs a) {
             const uint64_t C = cpre[k + 1] - cpre[k];
             const uint64_t hb = (rel ? a.hbase1 : a.hbase0)[k];
             const uint64_t ob = (rel ? a.obase1 : a.obase0)[k];
-            const uint64_t rb = rel ? a.n_total0 : 0;
+            const uint64_t rb = rel ? a.obase0[a.nseg] : 0;
             uint64_t *po = rel ? a.poff1 : a.poff0;
             po[p] = C ? (uint64_t)a.hist_scan[hb + d * C] - rb : ob;
         }
@@ -272,7 +273,8 @@ static int lds_limit_set(const void *fn, size_t bytes) {
     return AJ_OK;
 }
 
-int run_partition_pass(const PassDesc &d, const PassWork &w, hipStream_t st) {
+int run_partition_pass(const PassDesc &d, const PassWork &w, hipStream_t st, const Marks *mk,
+                       int m0) {
     if (d.max_bits > MAX_PASS_BITS || d.max_bits < 0) {
         set_error(AJ_ERR_ARG, "radix bits per pass must be in [0, %d], got %d", MAX_PASS_BITS,
                   d.max_bits);
@@ -328,8 +330,10 @@ int run_partition_pass(const PassDesc &d, const PassWork &w, hipStream_t st) {
 
     hipLaunchKernelGGL(k_plan, dim3(1), dim3(PLAN_BLOCK), 0, st, a);
     hipLaunchKernelGGL(k_zero_tail, dim3(256), dim3(256), 0, st, a.hist, a.header, d.max_hist);
+    if (mk) mk->mark(m0);
     if (d.max_chunks)
         hipLaunchKernelGGL(k_hist, dim3((unsigned)d.max_chunks), dim3(PBLOCK), lds_hist, st, a);
+    if (mk) mk->mark(m0 + 1);
     size_t tmp_bytes = w.scan_tmp_bytes;
     if (d.max_hist) {
         hipError_t e = hipcub::DeviceScan::ExclusiveSum(w.scan_tmp, tmp_bytes, a.hist,
@@ -339,9 +343,11 @@ int run_partition_pass(const PassDesc &d, const PassWork &w, hipStream_t st) {
             return AJ_ERR_HIP;
         }
     }
+    if (mk) mk->mark(m0 + 2);
     if (d.max_chunks)
         hipLaunchKernelGGL(k_scatter, dim3((unsigned)d.max_chunks), dim3(PBLOCK), lds_scatter,
                            st, a);
+    if (mk) mk->mark(m0 + 3);
     const unsigned pgrid = (unsigned)std::min<uint64_t>((d.max_parts + 256) / 256, 4096);
     hipLaunchKernelGGL(k_part_off, dim3(pgrid), dim3(256), 0, st, a);
     AJ_HIP(hipGetLastError());

@@ -157,6 +157,23 @@ static ProPlan plan_pro(uint64_t nR, uint64_t nS, const ProConfig &c, int num_cu
     return p;
 }
 
+static int ensure_phase_events(Ctx *c) {
+    for (hipEvent_t &e : c->evk)
+        if (!e) AJ_HIP(hipEventCreate(&e));
+    return AJ_OK;
+}
+
+static void read_phases(Ctx *c, aj_join_out_t *out, int first, int last) {
+    for (int i = first; i < last; ++i) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, c->evk[i], c->evk[i + 1]) != hipSuccess) {
+            (void)hipGetLastError();
+            ms = 0;
+        }
+        out->phase_ms[i] = ms;
+    }
+}
+
 // scalars: [0]=count [1]=checksum [2]=pair cursor [3]=task counter (u32 in low half)
 int pro_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, uint64_t nS,
                const ProConfig &cfg, uint64_t *pairs, uint64_t pairs_cap, aj_join_out_t *out,
@@ -179,6 +196,15 @@ int pro_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
     uint32_t *psb = at<uint32_t>(B, p.o_psb);
 
     AJ_HIP(hipEventRecord(ctx->ev0, st));
+    Marks mk;
+    const Marks *mkp = nullptr;
+    if (kernel_timing_enabled()) {
+        AJ_TRY(ensure_phase_events(ctx));
+        mk.ev = ctx->evk;
+        mk.st = st;
+        mkp = &mk;
+        mk.mark(0);
+    }
     const uint64_t *jR = dR, *jS = dS, *roff, *soff;
     if (!part) {
         roff = at<uint64_t>(B, p.o_poff1[0]);
@@ -222,7 +248,7 @@ int pro_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
         w1.header = at<uint64_t>(B, p.o_hdr);
         w1.scan_tmp = ctx->scan_tmp.p;
         w1.scan_tmp_bytes = p.scan_bytes;
-        AJ_TRY(run_partition_pass(d1, w1, st));
+        AJ_TRY(run_partition_pass(d1, w1, st, mkp, 1));
         jR = ctx->p1R.as<uint64_t>();
         jS = ctx->p1S.as<uint64_t>();
         roff = w1.part_off[0];
@@ -255,12 +281,15 @@ int pro_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
             }
             w2.seg_part_base = at<uint32_t>(B, p.o_pb2);
             w2.header = at<uint64_t>(B, p.o_hdr2);
-            AJ_TRY(run_partition_pass(d2, w2, st));
+            AJ_TRY(run_partition_pass(d2, w2, st, mkp, 5));
             jR = ctx->p2R.as<uint64_t>();
             jS = ctx->p2S.as<uint64_t>();
             roff = w2.part_off[0];
             soff = w2.part_off[1];
         }
+    }
+    if (mkp) {  // phases that did not run take zero time
+        for (int i = part ? (p.b2 ? 9 : 5) : 1; i < 9; ++i) mk.mark(i);
     }
     JoinDesc j;
     j.R = jR;
@@ -285,7 +314,7 @@ int pro_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
     j.scan_tmp = ctx->scan_tmp.p;
     j.scan_tmp_bytes = p.scan_bytes;
     j.num_cu = ctx->num_cu;
-    AJ_TRY(run_join(j, st));
+    AJ_TRY(run_join(j, st, mkp, 9));
     AJ_HIP(hipEventRecord(ctx->ev1, st));
     uint64_t host[3];
     AJ_HIP(hipMemcpyAsync(host, scal, sizeof(host), hipMemcpyDeviceToHost, st));
@@ -296,6 +325,7 @@ int pro_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
     out->checksum = host[1];
     out->npairs = pairs ? host[2] : 0;
     out->device_ms = ms;
+    if (mkp) read_phases(ctx, out, AJ_PH_PLAN1, AJ_PH_JOIN + 1);
     return AJ_OK;
 }
 
@@ -319,7 +349,15 @@ int npo_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
     d.pairs = pairs;
     d.pairs_cap = pairs_cap;
     d.num_cu = ctx->num_cu;
-    AJ_TRY(run_npo(d, st));
+    Marks mk;
+    const Marks *mkp = nullptr;
+    if (kernel_timing_enabled()) {
+        AJ_TRY(ensure_phase_events(ctx));
+        mk.ev = ctx->evk;
+        mk.st = st;
+        mkp = &mk;
+    }
+    AJ_TRY(run_npo(d, st, mkp, AJ_PH_NPO_BUILD));
     AJ_HIP(hipEventRecord(ctx->ev1, st));
     uint64_t host[4];
     AJ_HIP(hipMemcpyAsync(host, acc, sizeof(host), hipMemcpyDeviceToHost, st));
@@ -330,6 +368,7 @@ int npo_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
     out->checksum = host[1];
     out->npairs = pairs ? host[3] : 0;
     out->device_ms = ms;
+    if (mkp) read_phases(ctx, out, AJ_PH_NPO_BUILD, AJ_PH_NPO_PROBE + 1);
     return AJ_OK;
 }
 

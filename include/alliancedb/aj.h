@@ -86,6 +86,7 @@ const aj_result_t *aj_result_ext(const result_t *r);
  *   "radix_bits"   0..24 total radix bits (0 = tuned default)
  *   "passes"       0..2  partition passes (0 = tuned default)
  *   "device"       >=0   HIP device of the default context
+ *   "kernel_timing" 0/1  record per-kernel device times (aj_join_out_t.phase_ms)
  */
 int aj_set_option(const char *name, int64_t value);
 int64_t aj_get_option(const char *name);
@@ -118,11 +119,30 @@ typedef struct aj_join_opts_t {
     uint64_t pairs_capacity;
 } aj_join_opts_t;
 
+/* Per-kernel device times of the last join, filled when the option
+ * "kernel_timing" is 1 (HIP events around each launch, on the join stream). */
+enum aj_phase {
+    AJ_PH_PLAN1 = 0,    /* PRO pass-1 planning (segment tables) */
+    AJ_PH_HIST1 = 1,    /* PRO pass-1 histogram kernel */
+    AJ_PH_SCAN1 = 2,    /* PRO pass-1 histogram scan */
+    AJ_PH_SCATTER1 = 3, /* PRO pass-1 scatter kernel */
+    AJ_PH_PLAN2 = 4,
+    AJ_PH_HIST2 = 5,
+    AJ_PH_SCAN2 = 6,
+    AJ_PH_SCATTER2 = 7,
+    AJ_PH_TASKS = 8,    /* PRO join task planning */
+    AJ_PH_JOIN = 9,     /* PRO build/probe kernel */
+    AJ_PH_NPO_BUILD = 10,
+    AJ_PH_NPO_PROBE = 11,
+    AJ_NPHASES = 12
+};
+
 typedef struct aj_join_out_t {
     uint64_t count;
     uint64_t checksum;
     uint64_t npairs;
     float device_ms;
+    float phase_ms[AJ_NPHASES];
 } aj_join_out_t;
 
 /*
@@ -135,10 +155,53 @@ int aj_join_dev(aj_ctx_t *ctx, const tuple_t *dR, uint64_t nR, const tuple_t *dS
                 uint64_t nS, const aj_join_opts_t *opts, aj_join_out_t *out,
                 void *stream);
 
+/* ------------------------------------------------------- batched windows */
+/*
+ * Many tumbling windows of two timestamp-ordered streams of 12-byte
+ * aj_stream_tuple_t, joined window by window in one graph launch.  Window w
+ * holds the tuples with d_bounds[w] <= ts < d_bounds[w+1]; per-window count
+ * and checksum land in d_count[w] / d_checksum[w] (device arrays of nwin).
+ * Each window's result equals PRO on that window's tuples (the reference runs
+ * one window per process, benchmark.cpp:164-257; there is no batched
+ * counterpart in the reference).  The plan owns the scratch for its maximum
+ * sizes; the kernel sequence is captured once into a hipGraph and replayed.
+ */
+typedef struct aj_batch_opts_t {
+    int cap;          /* R tuples per LDS round (4096/8192/12288), 0 = 8192 */
+    uint32_t split;   /* S tuples per join task, 0 = 65536 */
+    uint32_t target;  /* target R tuples per partition, 0 = cap/2 */
+    uint32_t chunk;   /* tuples per partition work-group, 0 = derived */
+    int no_graph;     /* 1 = enqueue kernels directly instead of a graph */
+} aj_batch_opts_t;
+
+typedef struct aj_batch_plan aj_batch_plan_t;
+
+int aj_batch_plan_create(aj_ctx_t *ctx, uint64_t max_R, uint64_t max_S, int max_windows,
+                         uint64_t max_window_tuples, const aj_batch_opts_t *opts,
+                         aj_batch_plan_t **out);
+int aj_batch_plan_destroy(aj_batch_plan_t *plan);
+/* Asynchronous on `stream` (NULL = the context's stream). */
+int aj_batch_run(aj_batch_plan_t *plan, const aj_stream_tuple_t *dR, uint64_t nR,
+                 const aj_stream_tuple_t *dS, uint64_t nS, const uint32_t *d_bounds, int nwin,
+                 uint64_t *d_count, uint64_t *d_checksum, void *stream);
+
+/*
+ * Device stream generator for window workloads: Poisson arrivals at
+ * rate_per_us tuples per microsecond (exponential gaps, prefix-summed), window
+ * w lasting sizes[w] / rate microseconds, keys uniform on
+ * [1, max(1, sizes[w] / dup)] of the tuple's window, payload = row id.
+ * Writes up to `capacity` tuples to d_out, nwin+1 boundaries to d_bounds and
+ * the tuple count to *n_out.  Blocks until done.
+ */
+int aj_gen_window_streams(aj_ctx_t *ctx, uint64_t seed, int nwin, const uint32_t *sizes,
+                          double rate_per_us, uint32_t dup, aj_stream_tuple_t *d_out,
+                          uint64_t capacity, uint32_t *d_bounds, uint64_t *n_out);
+
 /* ------------------------------------------------------------ generators */
 /*
- * Seeded host generators (splitmix64-seeded xoshiro256**), writing tuple_t
- * arrays with payloadID = row id (0-based).
+ * Seeded host generators (counter-based splitmix64 stream: value i is output
+ * number i of splitmix64 started at the seed, so the result does not depend on
+ * the thread count), writing tuple_t arrays with payloadID = row id (0-based).
  *   unique:  keys are a seeded pseudo-random permutation of 1..n
  *   uniform: keys uniform on [lo, hi]
  *   zipf:    keys are ranks 1..nkeys drawn Zipf(theta) by rejection-inversion

@@ -1,0 +1,106 @@
+"""GPU parity of batched tumbling-window joins: every window's count and checksum
+equals the CPU oracle's PRO on that window's tuples, alone or in the batch, with
+or without graph capture, over repeated replays."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _streams(aj, ctx, nwin, lo, hi, seed, dup=4, rate=1.6):
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(lo), np.log(hi), nwin)).astype(np.uint32)
+    cap = int(sizes.sum() * 1.01 + 8 * np.sqrt(sizes.sum()) + 2048)
+    out = {}
+    for side, s in (("R", seed * 2 + 1), ("S", seed * 2 + 2)):
+        d = aj.DeviceArray(12 * cap)
+        b = aj.DeviceArray(4 * (nwin + 1))
+        n = aj.gen_window_streams(ctx, s, sizes, rate, dup, d.ptr, cap, b.ptr)
+        out[side] = (d, n, b)
+    assert np.array_equal(out["R"][2].to_host(np.uint32, nwin + 1),
+                          out["S"][2].to_host(np.uint32, nwin + 1))
+    return sizes, out
+
+
+def _expected(aj, oracle, out, nwin):
+    bounds = out["R"][2].to_host(np.uint32, nwin + 1)
+    host = {}
+    for side in ("R", "S"):
+        d, n, _ = out[side]
+        host[side] = d.to_host(aj.STREAM_DTYPE, n)
+    want = []
+    for w in range(nwin):
+        win = []
+        for side in ("R", "S"):
+            t = host[side]
+            lo, hi = np.searchsorted(t["ts"], [bounds[w], bounds[w + 1]], side="left")
+            win.append(aj.as_tuples(t["key"][lo:hi].view(np.int32),
+                                    t["payload"][lo:hi].view(np.int32)))
+        want.append(oracle.aggregate(*win))
+    return bounds, host, want
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_batch_windows_match_oracle(aj, oracle, graph):
+    nwin = 96
+    with aj.Context(0) as ctx:
+        sizes, out = _streams(aj, ctx, nwin, 1 << 10, 1 << 17, seed=3)
+        bounds, host, want = _expected(aj, oracle, out, nwin)
+        assert np.all(np.diff(host["R"]["ts"].astype(np.int64)) >= 0)
+        cnt, chk = aj.DeviceArray(8 * nwin), aj.DeviceArray(8 * nwin)
+        (dR, nR, bR), (dS, nS, _) = out["R"], out["S"]
+        print(f"case graph={graph} nR={nR} nS={nS}", flush=True)
+        with aj.BatchPlan(ctx, nR, nS, nwin, int(sizes.max() * 1.1), graph=graph) as plan:
+            for rep in range(3):  # replays, with host work and syncs between them
+                plan.run(dR.ptr, nR, dS.ptr, nS, bR.ptr, nwin, cnt.ptr, chk.ptr)
+                aj.device_sync()
+                got = list(zip(cnt.to_host(np.uint64, nwin).tolist(),
+                               chk.to_host(np.uint64, nwin).tolist()))
+                assert got == [(int(c), int(x)) for c, x in want], rep
+                # a plain join on the same context between replays
+                one = ctx.join(dR.ptr, 0, dS.ptr, 0)
+                assert one.count == 0
+
+
+def test_batch_graph_update_on_new_inputs(aj, oracle):
+    """A second input set of other sizes updates the captured graph in place."""
+    with aj.Context(0) as ctx:
+        runs = []
+        for seed, nwin in ((5, 40), (6, 25)):
+            sizes, out = _streams(aj, ctx, nwin, 1 << 11, 1 << 16, seed=seed)
+            runs.append((nwin, sizes, out, _expected(aj, oracle, out, nwin)[2]))
+        max_R = max(r[2]["R"][1] for r in runs)
+        max_S = max(r[2]["S"][1] for r in runs)
+        with aj.BatchPlan(ctx, max_R, max_S, 64, 1 << 17) as plan:
+            for rep in range(2):
+                for nwin, sizes, out, want in runs:
+                    cnt, chk = aj.DeviceArray(8 * nwin), aj.DeviceArray(8 * nwin)
+                    (dR, nR, bR), (dS, nS, _) = out["R"], out["S"]
+                    plan.run(dR.ptr, nR, dS.ptr, nS, bR.ptr, nwin, cnt.ptr, chk.ptr)
+                    aj.device_sync()
+                    got = list(zip(cnt.to_host(np.uint64, nwin).tolist(),
+                                   chk.to_host(np.uint64, nwin).tolist()))
+                    assert got == [(int(c), int(x)) for c, x in want], (rep, nwin)
+
+
+def test_batch_window_equals_single_join(aj):
+    """A window's batched result equals a plain device PRO join of its tuples."""
+    nwin = 12
+    with aj.Context(0) as ctx:
+        sizes, out = _streams(aj, ctx, nwin, 1 << 12, 1 << 15, seed=9)
+        bounds = out["R"][2].to_host(np.uint32, nwin + 1)
+        (dR, nR, bR), (dS, nS, _) = out["R"], out["S"]
+        cnt, chk = aj.DeviceArray(8 * nwin), aj.DeviceArray(8 * nwin)
+        with aj.BatchPlan(ctx, nR, nS, nwin) as plan:
+            plan.run(dR.ptr, nR, dS.ptr, nS, bR.ptr, nwin, cnt.ptr, chk.ptr)
+            aj.device_sync()
+        got_c, got_x = cnt.to_host(np.uint64, nwin), chk.to_host(np.uint64, nwin)
+        hR, hS = dR.to_host(aj.STREAM_DTYPE, nR), dS.to_host(aj.STREAM_DTYPE, nS)
+        for w in range(nwin):
+            parts = []
+            for t in (hR, hS):
+                lo, hi = np.searchsorted(t["ts"], [bounds[w], bounds[w + 1]])
+                parts.append(aj.as_tuples(t["key"][lo:hi].view(np.int32),
+                                          t["payload"][lo:hi].view(np.int32)))
+            single = aj.PRO(parts[0], parts[1], 1)
+            assert (single.count, single.checksum) == (int(got_c[w]), int(got_x[w]))
