@@ -103,6 +103,7 @@ def _load() -> ctypes.CDLL:
         "aj_copy_h2d": (c.c_int, [P, P, c.c_uint64]),
         "aj_copy_d2h": (c.c_int, [P, P, c.c_uint64]),
         "aj_device_sync": (c.c_int, []),
+        "aj_copy_d2d": (c.c_int, [P, P, c.c_uint64]),
         "aj_batch_plan_create": (c.c_int, [P, c.c_uint64, c.c_uint64, c.c_int, c.c_uint64,
                                            c.POINTER(aj_batch_opts_t), c.POINTER(P)]),
         "aj_batch_plan_destroy": (c.c_int, [P]),

@@ -143,11 +143,14 @@ struct JoinDesc {
     uint32_t *task_cnt;             // workspace: max_parts+1
     uint32_t *task_base;            // workspace: max_parts+1
     uint32_t *task_next;            // device scalar (zeroed by the caller)
+    uint4 *task_info;               // workspace: 2 * join_max_tasks() entries
+    ulonglong2 *task_res;           // workspace: join_max_tasks() partial results
     void *scan_tmp;
     size_t scan_tmp_bytes;
     int num_cu;
 };
 size_t join_scan_tmp_bytes(uint64_t max_parts);
+uint64_t join_max_tasks(uint64_t max_parts, uint64_t max_S, uint32_t split);
 // marks (nullable): m0 after task planning, m0+1 after the join kernel
 int run_join(const JoinDesc &d, hipStream_t st, const Marks *marks = nullptr, int m0 = 0);
 
@@ -163,9 +166,11 @@ struct NpoDesc {
     int num_cu;
 };
 int run_npo(const NpoDesc &d, hipStream_t st, const Marks *marks = nullptr, int m0 = 0);
+size_t npo_acc_words(int num_cu);  // u64 words of NpoDesc::acc
 
 // fill kernel (graph-safe replacement of hipMemsetAsync)
 int fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t st);
 int fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st);
+int copy_d2d(void *dst, const void *src, uint64_t bytes, int num_cu, hipStream_t st);
 
 }  // namespace aj

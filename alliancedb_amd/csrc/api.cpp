@@ -411,6 +411,22 @@ int aj_copy_d2h(void *dst, const void *src, uint64_t bytes) {
     return AJ_OK;
 }
 
+int aj_copy_d2d(void *dst, const void *src, uint64_t bytes) {
+    clear_error();
+    if (!bytes) return AJ_OK;
+    if (!dst || !src) {
+        set_error(AJ_ERR_ARG, "NULL pointer");
+        return AJ_ERR_ARG;
+    }
+    int st = AJ_OK;
+    Ctx *c = default_ctx(&st);
+    if (!c) return st;
+    AJ_HIP(hipSetDevice(c->device));
+    AJ_TRY(copy_d2d(dst, src, bytes, c->num_cu, nullptr));
+    AJ_HIP(hipDeviceSynchronize());
+    return AJ_OK;
+}
+
 int aj_device_sync(void) {
     clear_error();
     AJ_TRY(ensure_default_device());

@@ -46,6 +46,26 @@ static unsigned grid_for(uint64_t n, int num_cu) {
     return (unsigned)(g ? g : 1);
 }
 
+// Plain streaming copy of 16-byte words (the traffic-calibration kernel).
+__global__ __launch_bounds__(NBLOCK) void aj_copy_kernel(const uint4 *src, uint4 *dst, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+int copy_d2d(void *dst, const void *src, uint64_t bytes, int num_cu, hipStream_t st) {
+    if ((bytes & 15) || ((uintptr_t)dst & 15) || ((uintptr_t)src & 15)) {
+        set_error(AJ_ERR_ARG, "aj_copy_d2d needs 16-byte aligned pointers and sizes");
+        return AJ_ERR_ARG;
+    }
+    const uint64_t n = bytes / 16;
+    if (n)
+        hipLaunchKernelGGL(aj_copy_kernel, dim3(grid_for(n, num_cu)), dim3(NBLOCK), 0, st,
+                           (const uint4 *)src, (uint4 *)dst, n);
+    AJ_HIP(hipGetLastError());
+    return AJ_OK;
+}
+
 int fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t st) {
     if (!n) return AJ_OK;
     hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n, 256)), dim3(NBLOCK), 0, st, p, n, v);
@@ -147,11 +167,30 @@ __global__ __launch_bounds__(NBLOCK) void k_npo_probe(const uint64_t *S, uint64_
     }
     cnt = block_sum<NBLOCK>(cnt, red);
     sum = block_sum<NBLOCK>(sum, red);
-    if (threadIdx.x == 0 && cnt) {
-        atomicAdd((unsigned long long *)&acc[0], (unsigned long long)cnt);
-        atomicAdd((unsigned long long *)&acc[1], (unsigned long long)sum);
+    // per-block partials (acc[8 + 2b], acc[9 + 2b]), added by k_npo_reduce: a
+    // shared accumulator would serialise one atomic per block
+    if (threadIdx.x == 0) {
+        acc[8 + 2 * blockIdx.x] = cnt;
+        acc[9 + 2 * blockIdx.x] = sum;
     }
 }
+
+__global__ __launch_bounds__(NBLOCK) void k_npo_reduce(uint64_t *acc, unsigned nblocks) {
+    __shared__ uint64_t red[NBLOCK / 64];
+    uint64_t c = 0, x = 0;
+    for (unsigned b = threadIdx.x; b < nblocks; b += NBLOCK) {
+        c += acc[8 + 2 * b];
+        x += acc[9 + 2 * b];
+    }
+    c = block_sum<NBLOCK>(c, red);
+    x = block_sum<NBLOCK>(x, red);
+    if (threadIdx.x == 0) {
+        acc[0] = c;
+        acc[1] = x;
+    }
+}
+
+size_t npo_acc_words(int num_cu) { return 8 + 2 * (size_t)num_cu * 16; }
 
 int run_npo(const NpoDesc &d, hipStream_t st, const Marks *mk, int m0) {
     if (d.capacity < 2 || (d.capacity & (d.capacity - 1)) || d.capacity <= d.nR) {
@@ -166,14 +205,14 @@ int run_npo(const NpoDesc &d, hipStream_t st, const Marks *mk, int m0) {
                            d.nR, d.table, mask, d.acc);
     if (mk) mk->mark(m0 + 1);
     if (d.nS) {
+        const unsigned g = grid_for(d.nS, d.num_cu);  // <= num_cu * 16 (npo_acc_words)
         if (d.pairs)
-            hipLaunchKernelGGL(k_npo_probe<true>, dim3(grid_for(d.nS, d.num_cu)), dim3(NBLOCK), 0,
-                               st, d.S, d.nS, (const uint64_t *)d.table, mask, d.acc, d.pairs,
-                               d.pairs_cap);
+            hipLaunchKernelGGL(k_npo_probe<true>, dim3(g), dim3(NBLOCK), 0, st, d.S, d.nS,
+                               (const uint64_t *)d.table, mask, d.acc, d.pairs, d.pairs_cap);
         else
-            hipLaunchKernelGGL(k_npo_probe<false>, dim3(grid_for(d.nS, d.num_cu)), dim3(NBLOCK), 0,
-                               st, d.S, d.nS, (const uint64_t *)d.table, mask, d.acc, nullptr,
-                               (uint64_t)0);
+            hipLaunchKernelGGL(k_npo_probe<false>, dim3(g), dim3(NBLOCK), 0, st, d.S, d.nS,
+                               (const uint64_t *)d.table, mask, d.acc, nullptr, (uint64_t)0);
+        hipLaunchKernelGGL(k_npo_reduce, dim3(1), dim3(NBLOCK), 0, st, d.acc, g);
     }
     if (mk) mk->mark(m0 + 2);
     AJ_HIP(hipGetLastError());

@@ -9,12 +9,14 @@
 //
 //   k_plan      per segment: chunk counts, histogram bases, output bases,
 //               partition bases (one work-group, block scans)
-//   k_hist      one WG per chunk: LDS histogram of the digit (wave-aggregated
-//               LDS atomics) written column-major: hist[base + d*C + c]
+//   k_hist      one WG per chunk: LDS histogram of the digit (LDS atomics;
+//               the LDS serialises same-address lanes of one instruction, so
+//               a skewed digit costs at most ~1 cycle per tuple) written
+//               column-major: hist[base + d*C + c]
 //   scan        one flat exclusive scan over all histogram entries gives every
 //               (segment, digit, chunk) its output offset -- the
 //               partition-major, chunk-minor order of the reference
-//   k_scatter   one WG per chunk, tiles of 8192 tuples: wave-level ranking,
+//   k_scatter   one WG per chunk, tiles of 8192 tuples: rank by LDS atomic,
 //               LDS counting sort of the tile, then each digit's run is
 //               written with consecutive lanes on consecutive addresses
 //   k_part_off  partition start offsets for the join
@@ -140,7 +142,63 @@ __device__ __forceinline__ bool locate_chunk(const PassArgs &a, uint64_t g, Chun
     return true;
 }
 
+// Tile of PTILE tuples starting at `tb` (tb even for 8-byte tuples): element
+// j of this thread is tuple idx[j]; v[j] says whether it lies in [lo, hi).
+// 8-byte tuples are read two per lane with 16-byte loads (consecutive lanes on
+// consecutive 16 bytes); a load never reaches past `hi`, so nothing outside
+// the segment's array is touched.  12-byte stream tuples are read one per lane.
+__device__ __forceinline__ void load_tile(const void *in, int stride, uint64_t tb, uint64_t lo,
+                                          uint64_t hi, uint64_t (&t)[PITEMS],
+                                          bool (&v)[PITEMS]) {
+    if (stride == 8) {
+        typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
+        const uint64_t *p = static_cast<const uint64_t *>(in);
+        if (tb >= lo && tb + PTILE <= hi) {  // whole tile inside: unconditional 16-byte loads
+#pragma unroll
+            for (int j = 0; j < PITEMS / 2; ++j) {
+                const u64x2 q = __builtin_nontemporal_load(
+                    reinterpret_cast<const u64x2 *>(p + tb + 2 * ((uint64_t)j * PBLOCK + threadIdx.x)));
+                t[2 * j] = q.x;
+                t[2 * j + 1] = q.y;
+                v[2 * j] = v[2 * j + 1] = true;
+            }
+            return;
+        }
+#pragma unroll
+        for (int j = 0; j < PITEMS / 2; ++j) {
+            const uint64_t i0 = tb + 2 * ((uint64_t)j * PBLOCK + threadIdx.x);
+            uint64_t a = 0, b = 0;
+            if (i0 + 1 < hi) {
+                typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
+                const u64x2 q = __builtin_nontemporal_load(reinterpret_cast<const u64x2 *>(p + i0));
+                a = q.x;
+                b = q.y;
+            } else if (i0 < hi) {
+                a = p[i0];
+            }
+            t[2 * j] = a;
+            t[2 * j + 1] = b;
+            v[2 * j] = i0 >= lo && i0 < hi;
+            v[2 * j + 1] = i0 + 1 >= lo && i0 + 1 < hi;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < PITEMS; ++j) {
+            const uint64_t i = tb + (uint64_t)j * PBLOCK + threadIdx.x;
+            v[j] = i >= lo && i < hi;
+            t[j] = v[j] ? load_tuple(in, stride, i) : 0;
+        }
+    }
+}
+
+// first tile start of a chunk: 8-byte tuples start on an even index so that
+// 16-byte loads are aligned (the element before `begin` is masked out)
+__device__ __forceinline__ uint64_t tile_start(uint64_t begin, int stride) {
+    return stride == 8 ? (begin & ~1ull) : begin;
+}
+
 // ---------------------------------------------------------------- k_hist
+template <int STRIDE>
 __global__ __launch_bounds__(PBLOCK) void k_hist(PassArgs a) {
     extern __shared__ uint32_t cnt[];
     ChunkLoc L;
@@ -150,18 +208,14 @@ __global__ __launch_bounds__(PBLOCK) void k_hist(PassArgs a) {
     const void *in = L.rel ? a.in1 : a.in0;
     for (uint32_t i = threadIdx.x; i < F; i += PBLOCK) cnt[i] = 0;
     __syncthreads();
-    for (uint64_t base = L.begin; base < L.end; base += PTILE) {
+    for (uint64_t base = tile_start(L.begin, STRIDE); base < L.end; base += PTILE) {
         uint64_t t[PITEMS];
+        bool v[PITEMS];
+        load_tile(in, STRIDE, base, L.begin, L.end, t, v);
 #pragma unroll
         for (int j = 0; j < PITEMS; ++j) {
-            const uint64_t i = base + (uint64_t)j * PBLOCK + threadIdx.x;
-            t[j] = i < L.end ? load_tuple(in, a.stride, i) : 0;
-        }
-#pragma unroll
-        for (int j = 0; j < PITEMS; ++j) {
-            const uint64_t i = base + (uint64_t)j * PBLOCK + threadIdx.x;
             const uint32_t d = (key_of(t[j]) >> a.shift) & mask;
-            wave_count_add(cnt, d, i < L.end, bits);
+            if (v[j]) atomicAdd(&cnt[d], 1u);
         }
     }
     __syncthreads();
@@ -170,6 +224,7 @@ __global__ __launch_bounds__(PBLOCK) void k_hist(PassArgs a) {
 }
 
 // ---------------------------------------------------------------- k_scatter
+template <int STRIDE>
 __global__ __launch_bounds__(PBLOCK) void k_scatter(PassArgs a) {
     extern __shared__ uint64_t smem[];
     __shared__ uint32_t scan_tmp[PBLOCK / 64];
@@ -188,28 +243,23 @@ __global__ __launch_bounds__(PBLOCK) void k_scatter(PassArgs a) {
     for (uint32_t d = threadIdx.x; d < F; d += PBLOCK)
         cursor[d] = a.hist_scan[hb + d * L.nchunk + L.c] - rel_base;
 
-    for (uint64_t base = L.begin; base < L.end; base += PTILE) {
+    for (uint64_t base = tile_start(L.begin, STRIDE); base < L.end; base += PTILE) {
         for (uint32_t d = threadIdx.x; d <= F; d += PBLOCK) lofs[d] = 0;
         uint64_t t[PITEMS];
-#pragma unroll
-        for (int j = 0; j < PITEMS; ++j) {
-            const uint64_t i = base + (uint64_t)j * PBLOCK + threadIdx.x;
-            t[j] = i < L.end ? load_tuple(in, a.stride, i) : 0;
-        }
+        bool v[PITEMS];
+        load_tile(in, STRIDE, base, L.begin, L.end, t, v);
         __syncthreads();
         uint32_t rank[PITEMS];
 #pragma unroll
         for (int j = 0; j < PITEMS; ++j) {
-            const uint64_t i = base + (uint64_t)j * PBLOCK + threadIdx.x;
             const uint32_t d = (key_of(t[j]) >> a.shift) & mask;
-            rank[j] = wave_rank_add(lofs, d, i < L.end, bits);
+            rank[j] = v[j] ? atomicAdd(&lofs[d], 1u) : 0u;
         }
         __syncthreads();
         const uint32_t tile_len = lds_exclusive_scan<PBLOCK>(lofs, F + 1, scan_tmp);
 #pragma unroll
         for (int j = 0; j < PITEMS; ++j) {
-            const uint64_t i = base + (uint64_t)j * PBLOCK + threadIdx.x;
-            if (i < L.end) {
+            if (v[j]) {
                 const uint32_t d = (key_of(t[j]) >> a.shift) & mask;
                 buf[lofs[d] + rank[j]] = t[j];
             }
@@ -263,12 +313,18 @@ size_t pass_scan_tmp_bytes(uint64_t max_hist) {
     return bytes;
 }
 
+// Raises a kernel's dynamic-LDS limit once to the largest size requested so far
+// (callers hold the context lock).
 static int lds_limit_set(const void *fn, size_t bytes) {
-    static size_t set_hist = 0, set_scatter = 0;
-    size_t *s = fn == (const void *)k_hist ? &set_hist : &set_scatter;
-    if (bytes > *s) {
+    static const void *fns[8] = {};
+    static size_t set[8] = {};
+    int i = 0;
+    while (i < 8 && fns[i] && fns[i] != fn) ++i;
+    if (i == 8) i = 7;
+    if (!fns[i]) fns[i] = fn;
+    if (bytes > set[i]) {
         AJ_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
-        *s = bytes;
+        set[i] = bytes;
     }
     return AJ_OK;
 }
@@ -325,14 +381,25 @@ int run_partition_pass(const PassDesc &d, const PassWork &w, hipStream_t st, con
     const uint32_t F = 1u << d.max_bits;
     const size_t lds_hist = sizeof(uint32_t) * F;
     const size_t lds_scatter = sizeof(uint64_t) * PTILE + sizeof(uint32_t) * (2 * F + 1);
-    AJ_TRY(lds_limit_set((const void *)k_hist, lds_hist));
-    AJ_TRY(lds_limit_set((const void *)k_scatter, lds_scatter));
+    if (d.in_stride != 8 && d.in_stride != 12) {
+        set_error(AJ_ERR_ARG, "tuple stride must be 8 or 12 bytes");
+        return AJ_ERR_ARG;
+    }
+    const bool s8 = d.in_stride == 8;
+    const void *fh = s8 ? (const void *)k_hist<8> : (const void *)k_hist<12>;
+    const void *fs = s8 ? (const void *)k_scatter<8> : (const void *)k_scatter<12>;
+    AJ_TRY(lds_limit_set(fh, lds_hist));
+    AJ_TRY(lds_limit_set(fs, lds_scatter));
 
     hipLaunchKernelGGL(k_plan, dim3(1), dim3(PLAN_BLOCK), 0, st, a);
     hipLaunchKernelGGL(k_zero_tail, dim3(256), dim3(256), 0, st, a.hist, a.header, d.max_hist);
     if (mk) mk->mark(m0);
-    if (d.max_chunks)
-        hipLaunchKernelGGL(k_hist, dim3((unsigned)d.max_chunks), dim3(PBLOCK), lds_hist, st, a);
+    if (d.max_chunks) {
+        if (s8)
+            hipLaunchKernelGGL(k_hist<8>, dim3((unsigned)d.max_chunks), dim3(PBLOCK), lds_hist, st, a);
+        else
+            hipLaunchKernelGGL(k_hist<12>, dim3((unsigned)d.max_chunks), dim3(PBLOCK), lds_hist, st, a);
+    }
     if (mk) mk->mark(m0 + 1);
     size_t tmp_bytes = w.scan_tmp_bytes;
     if (d.max_hist) {
@@ -344,9 +411,14 @@ int run_partition_pass(const PassDesc &d, const PassWork &w, hipStream_t st, con
         }
     }
     if (mk) mk->mark(m0 + 2);
-    if (d.max_chunks)
-        hipLaunchKernelGGL(k_scatter, dim3((unsigned)d.max_chunks), dim3(PBLOCK), lds_scatter,
-                           st, a);
+    if (d.max_chunks) {
+        if (s8)
+            hipLaunchKernelGGL(k_scatter<8>, dim3((unsigned)d.max_chunks), dim3(PBLOCK),
+                               lds_scatter, st, a);
+        else
+            hipLaunchKernelGGL(k_scatter<12>, dim3((unsigned)d.max_chunks), dim3(PBLOCK),
+                               lds_scatter, st, a);
+    }
     if (mk) mk->mark(m0 + 3);
     const unsigned pgrid = (unsigned)std::min<uint64_t>((d.max_parts + 256) / 256, 4096);
     hipLaunchKernelGGL(k_part_off, dim3(pgrid), dim3(256), 0, st, a);

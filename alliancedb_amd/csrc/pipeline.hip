@@ -70,12 +70,14 @@ ProConfig choose_pro_config(uint64_t nR, uint64_t nS, const aj_join_opts_t *o) {
     if (o && o->radix_bits > 0) {
         bits = o->radix_bits;
     } else {
-        // average R partition of about half an LDS round
+        // average R partition of about one LDS round (measured best at 2^27:
+        // larger partitions halve the join tasks; the overflow takes a 2nd round)
         bits = 0;
-        const uint64_t target = (uint64_t)c.cap / 2;
+        const uint64_t target = (uint64_t)c.cap;
         while (bits < 24 && (nR >> bits) > target) ++bits;
     }
-    int passes = (o && o->passes > 0) ? o->passes : (bits > 11 ? 2 : 1);
+    // one pass only while the scatter keeps runs of >= 16 tuples per 8K tile
+    int passes = (o && o->passes > 0) ? o->passes : (bits > 9 ? 2 : 1);
     if (bits == 0) passes = 0;
     if (passes == 1) {
         c.bits1 = bits;
@@ -99,7 +101,7 @@ struct ProPlan {
     size_t o_ib[2], o_ie[2], o_bits1, o_bits2;
     size_t o_cpre[2], o_hb[2], o_ob[2], o_pb, o_hdr, o_poff1[2];
     size_t o_cpre2[2], o_hb2[2], o_ob2[2], o_pb2, o_hdr2, o_poff2[2];
-    size_t o_tcnt, o_tbase, o_psb, o_scal;
+    size_t o_tcnt, o_tbase, o_tinfo, o_tres, o_psb, o_scal;
     size_t bytes;
     size_t scan_bytes;
 };
@@ -148,6 +150,8 @@ static ProPlan plan_pro(uint64_t nR, uint64_t nS, const ProConfig &c, int num_cu
     const uint64_t jparts = p.parts_final;
     p.o_tcnt = L.add<uint32_t>(jparts + 1);
     p.o_tbase = L.add<uint32_t>(jparts + 1);
+    p.o_tinfo = L.add<uint4>(2 * join_max_tasks(jparts, nS, (uint32_t)c.split));
+    p.o_tres = L.add<ulonglong2>(join_max_tasks(jparts, nS, (uint32_t)c.split));
     p.o_psb = L.add<uint32_t>(2);
     p.o_scal = L.add<uint64_t>(8);
     p.bytes = L.off;
@@ -311,6 +315,8 @@ int pro_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
     j.task_cnt = at<uint32_t>(B, p.o_tcnt);
     j.task_base = at<uint32_t>(B, p.o_tbase);
     j.task_next = reinterpret_cast<uint32_t *>(scal + 3);
+    j.task_info = at<uint4>(B, p.o_tinfo);
+    j.task_res = at<ulonglong2>(B, p.o_tres);
     j.scan_tmp = ctx->scan_tmp.p;
     j.scan_tmp_bytes = p.scan_bytes;
     j.num_cu = ctx->num_cu;
@@ -334,7 +340,7 @@ int npo_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
     uint64_t cap = 2;
     while (cap <= 2 * nR) cap <<= 1;
     AJ_TRY(ctx->table.ensure(cap * 8));
-    AJ_TRY(ctx->acc.ensure(8 * 8));
+    AJ_TRY(ctx->acc.ensure(8 * npo_acc_words(ctx->num_cu)));
     uint64_t *acc = ctx->acc.as<uint64_t>();
     AJ_HIP(hipEventRecord(ctx->ev0, st));
     AJ_TRY(fill_u64(acc, 8, 0, st));

@@ -79,7 +79,7 @@ __global__ void k_window_bits(const uint64_t *rb, const uint64_t *re, int nwin, 
 }
 
 struct BatchLayout {
-    size_t rb, re, sb, se, bits, cpre[2], hb[2], ob[2], pb, hdr, poff[2], tcnt, tbase, scal;
+    size_t rb, re, sb, se, bits, cpre[2], hb[2], ob[2], pb, hdr, poff[2], tcnt, tbase, tinfo, tres, scal;
     size_t bytes;
 };
 
@@ -143,6 +143,8 @@ static int plan_init(BatchPlan *p) {
     L.hdr = lay(off, 64);
     L.tcnt = lay(off, 4 * (p->max_parts + 1));
     L.tbase = lay(off, 4 * (p->max_parts + 1));
+    L.tinfo = lay(off, 2 * sizeof(uint4) * join_max_tasks(p->max_parts, p->max_S, p->split));
+    L.tres = lay(off, sizeof(ulonglong2) * join_max_tasks(p->max_parts, p->max_S, p->split));
     L.scal = lay(off, 64);
     L.bytes = off;
     p->scan_bytes = std::max(pass_scan_tmp_bytes(std::max<uint64_t>(p->max_hist, 1)),
@@ -227,6 +229,8 @@ static int enqueue_batch(BatchPlan *p, const aj_stream_tuple_t *dR, uint64_t nR,
     j.task_cnt = W_<uint32_t>(p, L.tcnt);
     j.task_base = W_<uint32_t>(p, L.tbase);
     j.task_next = reinterpret_cast<uint32_t *>(W_<uint64_t>(p, L.scal) + 3);
+    j.task_info = W_<uint4>(p, L.tinfo);
+    j.task_res = W_<ulonglong2>(p, L.tres);
     j.scan_tmp = p->scan_tmp.p;
     j.scan_tmp_bytes = p->scan_bytes;
     j.num_cu = p->ctx->num_cu;

@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--cap", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-log2n", type=int, default=24)
+    ap.add_argument("--calib", action="store_true",
+                    help="also run aj_copy_kernel over 1 GiB (traffic-counter calibration)")
     return ap.parse_args()
 
 
@@ -70,6 +72,20 @@ def cpu_baseline(aj, log2n: int):
                       f"2 passes (reference defaults), median of 3 (temporaries included)"}
 
 
+def measured_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 traffic summary of
+    this workload, or None when none was measured for it."""
+    path = os.path.join(ROOT, "profiles", "pro_2e27_traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+    except OSError:
+        return None
+    if d.get("workload") != workload:
+        return None
+    return d["hbm_bytes_per_launch"].get(kernel)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -86,6 +102,10 @@ def main():
     dR, dS = aj.DeviceArray.from_host(R), aj.DeviceArray.from_host(S)
     aj.device_sync()
     del R, S
+    if args.calib:
+        dst = aj.DeviceArray(dR.nbytes)
+        aj.check(aj.lib.aj_copy_d2d(dst.ptr, dR.ptr, dR.nbytes))
+        dst.free()
     algo = aj.AJ_ALGO_PRO if args.algo == "PRO" else aj.AJ_ALGO_NPO
     ctx = aj.Context(0)
     kw = dict(radix_bits=args.radix_bits, passes=args.passes, cap=args.cap)
@@ -139,6 +159,9 @@ def main():
                      "kernel_ms": {k: round(v, 4) for k, v in mean_ph.items() if v > 0}},
         "checksum": str(out.checksum),
     }
+    default_geom = not (args.radix_bits or args.passes or args.cap)
+    if default_geom:
+        line["roofline"]["traffic"] = measured_traffic(KERNEL_NAMES[dom], line["config"]["workload"])
     ctx.close()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(aj, args.cpu_log2n)

@@ -100,6 +100,9 @@ int aj_device_alloc(void **ptr, uint64_t bytes);
 int aj_device_free(void *ptr);
 int aj_copy_h2d(void *dst_device, const void *src_host, uint64_t bytes);
 int aj_copy_d2h(void *dst_host, const void *src_device, uint64_t bytes);
+/* Device-to-device copy by a plain streaming kernel (aj_copy_kernel), also
+ * the reference point for calibrating memory-traffic counters. */
+int aj_copy_d2d(void *dst_device, const void *src_device, uint64_t bytes);
 int aj_device_sync(void);
 
 typedef struct aj_ctx aj_ctx_t;
