@@ -1,0 +1,149 @@
+// ajbench.cpp -- benchmark driver: builds R and S, selects the join by name and
+// runs it, like the reference's `hashing` binary (main -> parse_args ->
+// benchmark(), hashing/main.cpp:256-296, 444-705; hashing/benchmark.cpp:164-257).
+//
+//   ajbench -a PRO|NPO [-r |R|] [-s |S|] [-x r_seed] [-y s_seed]
+//           [-d key distribution: 0 unique/uniform, 2 zipf] [-z zipf theta]
+//           [-w window -e step -l interval] [-n nthreads] [-m] [-R repeats]
+//
+// Sizes: with -w/-e/-l, |R| = |S| = (window / interval) * step as in the
+// reference (benchmark.cpp:179-180); otherwise -r/-s (default 128M as in the
+// reference's help text, main.cpp:403-404).  R keys are a seeded permutation of
+// 1..|R|; S keys are uniform on 1..|R| (-d 0) or Zipf(theta) over R's keys
+// (-d 2).  Prints "Results = N" like benchmark.cpp:249, plus the checksum and
+// the device and end-to-end times.
+#include <getopt.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "../../include/alliancedb/aj.h"
+
+namespace {
+
+struct algo_t {
+    const char *name;
+    result_t *(*joinAlgo)(relation_t *, relation_t *, int);
+};
+
+// the algorithms of this library, selected by name (hashing/main.cpp:306-338)
+const algo_t algos[] = {
+    {"PRO", PRO},
+    {"NPO", NPO},
+    {nullptr, nullptr},
+};
+
+struct params {
+    const algo_t *algo = &algos[0];
+    uint64_t r_size = 128000000, s_size = 128000000;
+    uint64_t r_seed = 12345, s_seed = 54321;  // main.cpp:351-352
+    int key_distribution = 0;
+    double skew = 0.0;
+    long window = 0, step = 0, interval = 0;
+    int nthreads = 1;
+    int materialize = 0;
+    int repeats = 1;
+};
+
+void usage(const char *prog) {
+    printf("Usage: %s -a <PRO|NPO> [-r R] [-s S] [-x seed] [-y seed] [-d 0|2] [-z theta]\n"
+           "          [-w window -e step -l interval] [-n nthreads] [-m] [-R repeats]\n",
+           prog);
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+    params p;
+    static option longopts[] = {{"algo", required_argument, 0, 'a'},
+                                {"r-size", required_argument, 0, 'r'},
+                                {"s-size", required_argument, 0, 's'},
+                                {"r-seed", required_argument, 0, 'x'},
+                                {"s-seed", required_argument, 0, 'y'},
+                                {"key_distribution", required_argument, 0, 'd'},
+                                {"skew", required_argument, 0, 'z'},
+                                {"window-size", required_argument, 0, 'w'},
+                                {"step-size", required_argument, 0, 'e'},
+                                {"interval", required_argument, 0, 'l'},
+                                {"nthreads", required_argument, 0, 'n'},
+                                {"materialize", no_argument, 0, 'm'},
+                                {"repeats", required_argument, 0, 'R'},
+                                {"help", no_argument, 0, 'h'},
+                                {0, 0, 0, 0}};
+    int c;
+    while ((c = getopt_long(argc, argv, "a:r:s:x:y:d:z:w:e:l:n:mR:h", longopts, nullptr)) != -1) {
+        switch (c) {
+            case 'a': {
+                const algo_t *a = algos;
+                while (a->name && strcmp(a->name, optarg) != 0) ++a;
+                if (!a->name) {
+                    fprintf(stderr, "[ERROR] Join algorithm named `%s' does not exist!\n", optarg);
+                    usage(argv[0]);
+                    return EXIT_FAILURE;
+                }
+                p.algo = a;
+                break;
+            }
+            case 'r': p.r_size = strtoull(optarg, nullptr, 10); break;
+            case 's': p.s_size = strtoull(optarg, nullptr, 10); break;
+            case 'x': p.r_seed = strtoull(optarg, nullptr, 10); break;
+            case 'y': p.s_seed = strtoull(optarg, nullptr, 10); break;
+            case 'd': p.key_distribution = atoi(optarg); break;
+            case 'z': p.skew = atof(optarg); break;
+            case 'w': p.window = atol(optarg); break;
+            case 'e': p.step = atol(optarg); break;
+            case 'l': p.interval = atol(optarg); break;
+            case 'n': p.nthreads = atoi(optarg); break;
+            case 'm': p.materialize = 1; break;
+            case 'R': p.repeats = atoi(optarg); break;
+            case 'h': usage(argv[0]); return EXIT_SUCCESS;
+            default: usage(argv[0]); return EXIT_FAILURE;
+        }
+    }
+    if (p.window > 0 && p.step > 0 && p.interval > 0) {
+        p.r_size = p.s_size = (uint64_t)(p.window / p.interval) * (uint64_t)p.step;
+    }
+    if (p.key_distribution != 0 && p.key_distribution != 2) {
+        fprintf(stderr, "[ERROR] key distribution must be 0 (unique/uniform) or 2 (zipf)\n");
+        return EXIT_FAILURE;
+    }
+    printf("[INFO ] Creating relations R=%llu S=%llu (seeds %llu/%llu)\n",
+           (unsigned long long)p.r_size, (unsigned long long)p.s_size,
+           (unsigned long long)p.r_seed, (unsigned long long)p.s_seed);
+    std::vector<tuple_t> R(p.r_size), S(p.s_size);
+    int rc = aj_gen_unique(R.data(), p.r_size, p.r_seed);
+    if (rc == AJ_OK) {
+        if (p.key_distribution == 2)
+            rc = aj_gen_zipf(S.data(), p.s_size, (uint32_t)p.r_size, p.skew, p.s_seed);
+        else
+            rc = aj_gen_uniform(S.data(), p.s_size, 1, (uint32_t)(p.r_size ? p.r_size : 1), p.s_seed);
+    }
+    if (rc != AJ_OK) {
+        fprintf(stderr, "[ERROR] generator failed (%d)\n", rc);
+        return EXIT_FAILURE;
+    }
+    aj_set_option("materialize", p.materialize);
+    relation_t relR = {R.data(), p.r_size, nullptr};
+    relation_t relS = {S.data(), p.s_size, nullptr};
+    printf("[INFO ] Running join algorithm %s ...\n", p.algo->name);
+    for (int rep = 0; rep < p.repeats; ++rep) {
+        auto t0 = std::chrono::steady_clock::now();
+        result_t *res = p.algo->joinAlgo(&relR, &relS, p.nthreads);
+        double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (!res) {
+            fprintf(stderr, "[ERROR] %s failed: %s\n", p.algo->name, aj_last_error());
+            return EXIT_FAILURE;
+        }
+        const aj_result_t *x = aj_result_ext(res);
+        printf("[INFO ] Results = %lld. checksum = %llu pairs = %llu device_ms = %.3f "
+               "total_ms = %.3f wall_ms = %.3f\n",
+               (long long)res->totalresults, (unsigned long long)x->checksum,
+               (unsigned long long)x->npairs, x->device_ms, x->total_ms, ms);
+        free(res);
+    }
+    printf("[INFO ] DONE.\n");
+    return EXIT_SUCCESS;
+}

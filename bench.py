@@ -1,15 +1,21 @@
-"""Benchmark: GPU PRO join throughput on 1 MI355X.
+"""Benchmark of the MI355X lazy equi-join path.  Prints one JSON line.
 
-Workload (default): |R| = |S| = 2^27 tuples of 8 bytes {payloadID, key};
-R keys a seeded permutation of 1..2^27, S keys uniform on 1..2^27, payload = row
-id.  A step is one PRO join of the device-resident relations (partition passes
-+ LDS build/probe + count/checksum reduction).  Prints one JSON line.
+Default workload (`--workload pro`): PRO on |R| = |S| = 2^27 tuples of 8 bytes
+{payloadID, key}.  R keys are a seeded permutation of 1..2^27, S keys uniform on
+1..2^27, payload = row id.  A step is one PRO join of the device-resident
+relations: partition passes, LDS build/probe, count/checksum reduction.
 
 Roofline: per-kernel device times come from HIP events recorded around every
 launch on the join stream (library option kernel_timing) during the timed
-steps; the dominant kernel's algorithmic bytes per launch (hist 8 B/tuple,
+steps.  The dominant kernel's algorithmic bytes per launch (hist 8 B/tuple,
 scatter 16 B/tuple, build/probe 8 B/tuple) over its mean duration gives
 `achieved`.
+
+Other workloads (each prints its own line):
+  --workload skew     |R| = 2^24 unique keys, |S| = 2^28 Zipf(--theta) over R's keys
+  --workload windows  1024 tumbling windows of two Poisson streams (1.6 tuples/us),
+                      per-side sizes log-uniform in [4K, 4M], duplicate factor 4;
+                      one graph-captured batched launch vs per-window launches
 """
 from __future__ import annotations
 
@@ -27,7 +33,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 PRO_BYTES_PER_TUPLE = 32  # histogram read 8 + scatter read 8 + write 8 + build/probe read 8
 NPO_BYTES_PER_TUPLE = 16  # R read 8 + slot write 8 / S read 8 + slot read 8
-# algorithmic bytes per input tuple of each kernel
+WINDOW_BYTES_PER_TUPLE = 40  # 12 hist read + 12 scatter read + 8 write + 8 build/probe read
+# algorithmic bytes per input tuple of each kernel (8-byte tuples)
 KERNEL_BYTES = {"hist1": 8, "scatter1": 16, "hist2": 8, "scatter2": 16, "join": 8,
                 "npo_build": 16, "npo_probe": 16}
 KERNEL_NAMES = {"hist1": "k_hist (pass 1)", "scatter1": "k_scatter (pass 1)",
@@ -40,8 +47,11 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", choices=["pro", "skew", "windows"], default="pro")
     ap.add_argument("--algo", choices=["PRO", "NPO"], default="PRO")
     ap.add_argument("--log2n", type=int, default=27)
+    ap.add_argument("--theta", type=float, default=1.0)
+    ap.add_argument("--nwin", type=int, default=1024)
     ap.add_argument("--radix-bits", type=int, default=0)
     ap.add_argument("--passes", type=int, default=0)
     ap.add_argument("--cap", type=int, default=0)
@@ -86,55 +96,46 @@ def measured_traffic(kernel: str, workload: str):
     return d["hbm_bytes_per_launch"].get(kernel)
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              f"(this path does not shard across GPUs)", file=sys.stderr)
-        sys.exit(2)
-    import alliancedb_amd as aj
-    n = 1 << args.log2n
-    t0 = time.time()
-    R = aj.gen_unique(n, 12345)
-    S = aj.gen_uniform(n, 1, n, 54321)
-    print(f"[bench] generated 2x{n} tuples in {time.time() - t0:.1f}s", file=sys.stderr,
-          flush=True)
-    dR, dS = aj.DeviceArray.from_host(R), aj.DeviceArray.from_host(S)
-    aj.device_sync()
-    del R, S
-    if args.calib:
-        dst = aj.DeviceArray(dR.nbytes)
-        aj.check(aj.lib.aj_copy_d2d(dst.ptr, dR.ptr, dR.nbytes))
-        dst.free()
-    algo = aj.AJ_ALGO_PRO if args.algo == "PRO" else aj.AJ_ALGO_NPO
-    ctx = aj.Context(0)
-    kw = dict(radix_bits=args.radix_bits, passes=args.passes, cap=args.cap)
-    for _ in range(args.warmup):
-        out = ctx.join(dR.ptr, n, dS.ptr, n, algo, **kw)
-    assert out.count == n, out.count
+def closed_form_fk(R: np.ndarray, S: np.ndarray):
+    """count/checksum of a foreign-key join where R's keys are a permutation of
+    1..|R| and every S key is one of them: count = |S| and the checksum follows
+    from R's inverse permutation."""
+    inv = np.empty(len(R) + 1, np.uint64)
+    inv[R["key"]] = R["payloadID"].astype(np.uint64)
+    x = (inv[S["key"]] << np.uint64(32)).sum(dtype=np.uint64)
+    x = (int(x) + int(S["payloadID"].astype(np.uint64).sum(dtype=np.uint64))) % (1 << 64)
+    return len(S), x
+
+
+def timed_joins(aj, ctx, dR, nR, dS, nS, algo, kw, steps, warmup, want):
+    for _ in range(warmup):
+        out = ctx.join(dR.ptr, nR, dS.ptr, nS, algo, **kw)
+        assert (out.count, out.checksum) == want, (out.count, out.checksum, want)
     aj.device_sync()
     print("[bench] warmup done", file=sys.stderr, flush=True)
     aj.set_option("kernel_timing", 1)
     dev_ms, phases = [], []
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        out = ctx.join(dR.ptr, n, dS.ptr, n, algo, **kw)
+    for _ in range(steps):
+        out = ctx.join(dR.ptr, nR, dS.ptr, nS, algo, **kw)
         dev_ms.append(out.device_ms)
         phases.append(out.phases())
-        assert out.count == n, out.count
+        assert (out.count, out.checksum) == want
     aj.device_sync()
     wall = time.perf_counter() - t0
     aj.set_option("kernel_timing", 0)
-    ms_step = wall * 1e3 / args.steps
-    dev = float(np.mean(dev_ms))
-    tuples = 2 * n
+    return out, float(np.mean(dev_ms)), wall * 1e3 / steps, phases
+
+
+def join_line(aj, args, workload_name, nR, nS, algo, out, dev, ms_step, phases, metric):
+    tuples = nR + nS
     mean_ph = {k: float(np.mean([p[k] for p in phases])) for k in phases[0]}
     dom = max(KERNEL_BYTES, key=lambda k: mean_ph.get(k, 0.0))
     achieved = KERNEL_BYTES[dom] * tuples / (mean_ph[dom] * 1e-3) / 1e9
     bpt = PRO_BYTES_PER_TUPLE if algo == aj.AJ_ALGO_PRO else NPO_BYTES_PER_TUPLE
     pipeline_gbs = bpt * tuples / (dev * 1e-3) / 1e9
-    line = {
-        "metric": "join throughput M tuples/s (|R|+|S|)",
+    return {
+        "metric": metric,
         "value": round(tuples / (dev * 1e-3) / 1e6, 1),
         "unit": "M tuples/s",
         "n_gpus": 1,
@@ -146,9 +147,7 @@ def main():
         "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic",
-        "config": {"workload": f"{args.algo} |R|=|S|=2^{args.log2n} 8-byte tuples, "
-                               f"unique R keys x uniform S keys",
-                   "n_r": n, "n_s": n},
+        "config": {"workload": workload_name, "n_r": nR, "n_s": nS},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None, "kernel": KERNEL_NAMES[dom],
@@ -159,12 +158,163 @@ def main():
                      "kernel_ms": {k: round(v, 4) for k, v in mean_ph.items() if v > 0}},
         "checksum": str(out.checksum),
     }
-    default_geom = not (args.radix_bits or args.passes or args.cap)
-    if default_geom:
-        line["roofline"]["traffic"] = measured_traffic(KERNEL_NAMES[dom], line["config"]["workload"])
-    ctx.close()
+
+
+def run_pro(aj, args):
+    n = 1 << args.log2n
+    t0 = time.time()
+    R = aj.gen_unique(n, 12345)
+    S = aj.gen_uniform(n, 1, n, 54321)
+    want = closed_form_fk(R, S)
+    print(f"[bench] generated 2x{n} tuples in {time.time() - t0:.1f}s", file=sys.stderr,
+          flush=True)
+    dR, dS = aj.DeviceArray.from_host(R), aj.DeviceArray.from_host(S)
+    aj.device_sync()
+    del R, S
+    if args.calib:
+        dst = aj.DeviceArray(dR.nbytes)
+        aj.check(aj.lib.aj_copy_d2d(dst.ptr, dR.ptr, dR.nbytes))
+        dst.free()
+    algo = aj.AJ_ALGO_PRO if args.algo == "PRO" else aj.AJ_ALGO_NPO
+    kw = dict(radix_bits=args.radix_bits, passes=args.passes, cap=args.cap)
+    with aj.Context(0) as ctx:
+        out, dev, ms_step, phases = timed_joins(aj, ctx, dR, n, dS, n, algo, kw, args.steps,
+                                                args.warmup, want)
+    name = (f"{args.algo} |R|=|S|=2^{args.log2n} 8-byte tuples, "
+            f"unique R keys x uniform S keys")
+    line = join_line(aj, args, name, n, n, algo, out, dev, ms_step, phases,
+                     "join throughput M tuples/s (|R|+|S|)")
+    if not (args.radix_bits or args.passes or args.cap):
+        line["roofline"]["traffic"] = measured_traffic(line["roofline"]["kernel"], name)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(aj, args.cpu_log2n)
+    return line
+
+
+def run_skew(aj, args):
+    nR, nS = 1 << 24, 1 << 28
+    R = aj.gen_unique(nR, 12345)
+    S = aj.gen_zipf(nS, nR, args.theta, 54321)
+    want = closed_form_fk(R, S)
+    print(f"[bench] generated skewed relations (theta={args.theta})", file=sys.stderr,
+          flush=True)
+    dR, dS = aj.DeviceArray.from_host(R), aj.DeviceArray.from_host(S)
+    aj.device_sync()
+    del R, S
+    algo = aj.AJ_ALGO_PRO if args.algo == "PRO" else aj.AJ_ALGO_NPO
+    kw = dict(radix_bits=args.radix_bits, passes=args.passes, cap=args.cap)
+    with aj.Context(0) as ctx:
+        out, dev, ms_step, phases = timed_joins(aj, ctx, dR, nR, dS, nS, algo, kw, args.steps,
+                                                args.warmup, want)
+    name = (f"{args.algo} skewed foreign-key join |R|=2^24 unique x |S|=2^28 "
+            f"Zipf(theta={args.theta})")
+    return join_line(aj, args, name, nR, nS, algo, out, dev, ms_step, phases,
+                     "join throughput M tuples/s (|R|+|S|)")
+
+
+def run_windows(aj, args):
+    import oracle
+    nwin = args.nwin
+    rng = np.random.default_rng(2024)
+    sizes = np.exp(rng.uniform(np.log(4096), np.log(4194304), nwin)).astype(np.uint32)
+    cap = int(sizes.sum() * 1.001 + 8 * np.sqrt(sizes.sum()) + 4096)
+    with aj.Context(0) as ctx:
+        streams = {}
+        for side, seed in (("R", 101), ("S", 202)):
+            d = aj.DeviceArray(12 * cap)
+            b = aj.DeviceArray(4 * (nwin + 1))
+            n = aj.gen_window_streams(ctx, seed, sizes, 1.6, 4, d.ptr, cap, b.ptr)
+            streams[side] = (d, n, b)
+        (dR, nR, bR), (dS, nS, _) = streams["R"], streams["S"]
+        bounds = bR.to_host(np.uint32, nwin + 1)
+        print(f"[bench] streams: {nR} + {nS} tuples, {nwin} windows", file=sys.stderr,
+              flush=True)
+        cnt, chk = aj.DeviceArray(8 * nwin), aj.DeviceArray(8 * nwin)
+        with aj.BatchPlan(ctx, nR, nS, nwin, int(sizes.max() * 1.05)) as plan:
+            for _ in range(args.warmup):
+                plan.run(dR.ptr, nR, dS.ptr, nS, bR.ptr, nwin, cnt.ptr, chk.ptr)
+            aj.device_sync()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                plan.run(dR.ptr, nR, dS.ptr, nS, bR.ptr, nwin, cnt.ptr, chk.ptr)
+            aj.device_sync()
+            t_batch = (time.perf_counter() - t0) / args.steps
+        batch_res = np.stack([cnt.to_host(np.uint64, nwin), chk.to_host(np.uint64, nwin)])
+        # per-window launches: one plain (not batched, not graph) join call per window
+        cnt1, chk1 = aj.DeviceArray(8 * nwin), aj.DeviceArray(8 * nwin)
+        with aj.BatchPlan(ctx, nR, nS, 1, int(sizes.max() * 1.05), graph=False) as one:
+            for w in range(min(nwin, 8)):  # warm
+                one.run(dR.ptr, nR, dS.ptr, nS, bR.ptr + 4 * w, 1, cnt1.ptr + 8 * w,
+                        chk1.ptr + 8 * w)
+            aj.device_sync()
+            t0 = time.perf_counter()
+            for w in range(nwin):
+                one.run(dR.ptr, nR, dS.ptr, nS, bR.ptr + 4 * w, 1, cnt1.ptr + 8 * w,
+                        chk1.ptr + 8 * w)
+                aj.device_sync()
+            t_single = time.perf_counter() - t0
+        single_res = np.stack([cnt1.to_host(np.uint64, nwin), chk1.to_host(np.uint64, nwin)])
+        assert np.array_equal(batch_res, single_res), "batched != per-window results"
+        # CPU oracle on a sample of windows (smallest ones, to stay bounded)
+        hR, hS = dR.to_host(aj.STREAM_DTYPE, nR), dS.to_host(aj.STREAM_DTYPE, nS)
+        checked = 0
+        for w in np.argsort(sizes)[:16]:
+            parts = []
+            for t in (hR, hS):
+                lo, hi = np.searchsorted(t["ts"], [bounds[w], bounds[w + 1]])
+                parts.append(aj.as_tuples(t["key"][lo:hi].view(np.int32),
+                                          t["payload"][lo:hi].view(np.int32)))
+            want = oracle.pro(parts[0], parts[1], nthreads=8)
+            assert (want.count, want.checksum) == (int(batch_res[0, w]), int(batch_res[1, w]))
+            checked += 1
+        lo = np.searchsorted(hR["ts"], bounds[0])
+        hi = np.searchsorted(hR["ts"], bounds[-1])
+        in_r = int(hi - lo)
+        lo = np.searchsorted(hS["ts"], bounds[0])
+        hi = np.searchsorted(hS["ts"], bounds[-1])
+        in_s = int(hi - lo)
+    tuples = in_r + in_s
+    gbs = WINDOW_BYTES_PER_TUPLE * tuples / t_batch / 1e9
+    return {
+        "metric": "batched window join throughput M tuples/s (|R|+|S| over all windows)",
+        "value": round(tuples / t_batch / 1e6, 1),
+        "unit": "M tuples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_batch * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": f"{nwin} tumbling windows, two Poisson streams at 1.6 tuples/us, "
+                               f"per-side sizes log-uniform [4K, 4M], duplicate factor 4, "
+                               f"12-byte tuples; one hipGraph launch per step",
+                   "windows": nwin, "tuples_in_windows": tuples},
+        "per_window_launches": {"value": round(tuples / t_single / 1e6, 1),
+                                "unit": "M tuples/s", "ms_total": round(t_single * 1e3, 2),
+                                "speedup_of_batched": round(t_single / t_batch, 2)},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "whole batched pipeline", "bytes_per_tuple": WINDOW_BYTES_PER_TUPLE},
+        "parity": {"batched_equals_per_window": True, "oracle_windows_checked": checked},
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              f"(this path does not shard across GPUs)", file=sys.stderr)
+        sys.exit(2)
+    import alliancedb_amd as aj
+    if args.workload == "pro":
+        line = run_pro(aj, args)
+    elif args.workload == "skew":
+        line = run_skew(aj, args)
+    else:
+        line = run_windows(aj, args)
     print(json.dumps(line), flush=True)
 
 

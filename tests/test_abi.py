@@ -133,3 +133,13 @@ def test_zipf_theta0_is_uniform(aj):
     counts = np.bincount(t["key"], minlength=257)[1:]
     exp = (1 << 18) / 256
     assert ((counts - exp) ** 2 / exp).sum() < 350
+
+
+def test_driver_registry_by_name():
+    """ajbench selects joins by name like the reference's -a (main.cpp:521-539)."""
+    import subprocess
+    exe = os.path.join(ROOT, "alliancedb_amd", "ajbench")
+    r = subprocess.run([exe, "-a", "NO_SUCH_JOIN"], capture_output=True, text=True)
+    assert r.returncode != 0 and "does not exist" in r.stderr
+    r = subprocess.run([exe, "-h"], capture_output=True, text=True)
+    assert r.returncode == 0 and "PRO|NPO" in r.stdout

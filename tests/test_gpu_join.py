@@ -156,3 +156,37 @@ def test_golden_fixtures(aj, algo):
             assert xxhash.xxh64(pairs.tobytes()).hexdigest() == e["pairs_digest"], name
     finally:
         aj.set_option("materialize", 0)
+
+
+@pytest.mark.parametrize("algo", ["PRO", "NPO"])
+def test_driver_runs_join(aj, oracle, algo):
+    """The C++ driver joins generated relations and prints the reference's result line."""
+    import os
+    import re
+    import subprocess
+    exe = os.path.join(os.path.dirname(aj.LIB_PATH), "ajbench")
+    r = subprocess.run([exe, "-a", algo, "-r", "200000", "-s", "300000", "-d", "2", "-z", "1.0",
+                        "-R", "2"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = re.findall(r"Results = (\d+)\. checksum = (\d+)", r.stdout)
+    assert len(got) == 2
+    R = aj.gen_unique(200000, 12345)
+    S = aj.gen_zipf(300000, 200000, 1.0, 54321)
+    want = oracle.aggregate(R, S)
+    for c, x in got:
+        assert (int(c), int(x)) == want
+
+
+@pytest.mark.parametrize("theta", [0.0, 0.5, 1.0, 1.5])
+@pytest.mark.parametrize("algo", [0, 1])
+def test_skewed_foreign_key_join(aj, oracle, theta, algo):
+    """Skewed workload shape at 1/16 size: |R| = 2^20 unique keys, |S| = 2^24 Zipf over R's keys."""
+    nR, nS = 1 << 20, 1 << 24
+    R = aj.gen_unique(nR, 77)
+    S = aj.gen_zipf(nS, nR, theta, 78)
+    want = oracle.aggregate(R, S)
+    dR, dS = aj.DeviceArray.from_host(R), aj.DeviceArray.from_host(S)
+    print(f"case skew theta={theta} algo={algo}", flush=True)
+    with aj.Context(0) as ctx:
+        out = ctx.join(dR.ptr, nR, dS.ptr, nS, algo)
+    assert (out.count, out.checksum) == want
