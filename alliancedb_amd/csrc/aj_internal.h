@@ -53,9 +53,10 @@ void dev_free(void *p);
 struct ProConfig {
     int bits1 = 0;        // radix bits of pass 1 (0 = derive from sizes)
     int bits2 = 0;        // radix bits of pass 2 (0 = single pass)
-    int cap = 8192;       // R tuples per LDS build round
-    int split = 1 << 16;  // S tuples per join task
-    int chunk = 1 << 17;  // tuples per partition work-group
+    int cap = 8704;       // R tuples per LDS build round
+    int split = 0;        // S tuples per join task (0 = cap; at most cap)
+    int chunk = 0;        // tuples per partition work-group (0 = derived from sizes)
+    int block = 512;      // lanes per partition work-group (256 or 512)
 };
 ProConfig choose_pro_config(uint64_t nR, uint64_t nS, const aj_join_opts_t *opts);
 
@@ -99,7 +100,8 @@ struct PassDesc {
     int nseg;
     int shift;
     int max_bits;
-    uint32_t chunk;           // tuples per work-group (multiple of the tile)
+    uint32_t chunk;           // tuples per work-group (multiple of the largest tile)
+    int block = 512;          // lanes per histogram/scatter work-group: 256 or 512
     uint64_t n_total[2];      // total tuples of R and S in this pass
     uint64_t max_chunks;      // upper bound of chunks over R+S (grid size)
     uint64_t max_hist;        // upper bound of histogram entries
@@ -150,6 +152,9 @@ struct JoinDesc {
     int num_cu;
 };
 size_t join_scan_tmp_bytes(uint64_t max_parts);
+// S tuples per join task actually used: at most one LDS round (`cap`), so a
+// task's S slice fits the work-group's registers; 0 = cap.
+uint32_t join_task_split(uint32_t split, int cap);
 uint64_t join_max_tasks(uint64_t max_parts, uint64_t max_S, uint32_t split);
 // marks (nullable): m0 after task planning, m0+1 after the join kernel
 int run_join(const JoinDesc &d, hipStream_t st, const Marks *marks = nullptr, int m0 = 0);

@@ -91,7 +91,7 @@ void DevBuf::release() {
 
 // ---------------------------------------------------------------- options
 static std::atomic<int64_t> opt_materialize{0}, opt_radix_bits{0}, opt_passes{0}, opt_device{0},
-    opt_cap{0}, opt_split{0}, opt_kernel_timing{0};
+    opt_cap{0}, opt_split{0}, opt_kernel_timing{0}, opt_part_block{0}, opt_part_chunk{0};
 
 bool kernel_timing_enabled() { return opt_kernel_timing.load() != 0; }
 
@@ -158,7 +158,10 @@ static ProConfig config_from(uint64_t nR, uint64_t nS, const aj_join_opts_t *o) 
     if (!eff.passes) eff.passes = (int)opt_passes.load();
     if (!eff.cap) eff.cap = (int)opt_cap.load();
     if (!eff.split) eff.split = (uint32_t)opt_split.load();
-    return choose_pro_config(nR, nS, &eff);
+    ProConfig c = choose_pro_config(nR, nS, &eff);
+    if (opt_part_block.load()) c.block = (int)opt_part_block.load();
+    if (opt_part_chunk.load()) c.chunk = (int)opt_part_chunk.load();
+    return c;
 }
 
 static int join_dev_locked(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS,
@@ -316,12 +319,22 @@ int aj_set_option(const char *name, int64_t value) {
     } tab[] = {{"materialize", &opt_materialize, 0, 1}, {"radix_bits", &opt_radix_bits, 0, 24},
                {"passes", &opt_passes, 0, 2},           {"device", &opt_device, 0, 1024},
                {"cap", &opt_cap, 0, 12288},             {"split", &opt_split, 0, 1ll << 30},
-               {"kernel_timing", &opt_kernel_timing, 0, 1}};
+               {"kernel_timing", &opt_kernel_timing, 0, 1},
+               {"part_block", &opt_part_block, 0, 512},
+               {"part_chunk", &opt_part_chunk, 0, 1ll << 24}};
     for (auto &e : tab) {
         if (strcmp(e.n, name) == 0) {
             if (value < e.lo || value > e.hi) {
                 set_error(AJ_ERR_ARG, "option %s out of range [%lld, %lld]", name,
                           (long long)e.lo, (long long)e.hi);
+                return AJ_ERR_ARG;
+            }
+            if (e.v == &opt_part_block && value && value != 256 && value != 512) {
+                set_error(AJ_ERR_ARG, "option part_block must be 0, 256 or 512");
+                return AJ_ERR_ARG;
+            }
+            if (e.v == &opt_part_chunk && value % 8192) {
+                set_error(AJ_ERR_ARG, "option part_chunk must be a multiple of 8192");
                 return AJ_ERR_ARG;
             }
             e.v->store(value);
@@ -349,6 +362,8 @@ int64_t aj_get_option(const char *name) {
     if (!strcmp(name, "cap")) return opt_cap.load();
     if (!strcmp(name, "split")) return opt_split.load();
     if (!strcmp(name, "kernel_timing")) return opt_kernel_timing.load();
+    if (!strcmp(name, "part_block")) return opt_part_block.load();
+    if (!strcmp(name, "part_chunk")) return opt_part_chunk.load();
     if (!strcmp(name, "device_budget")) return (int64_t)g_dev_budget.load();
     if (!strcmp(name, "device_used")) return (int64_t)g_dev_used.load();
     return -1;

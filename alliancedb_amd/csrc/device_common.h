@@ -4,8 +4,21 @@
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <type_traits>
+#include <utility>
 
 namespace aj {
+
+// f(std::integral_constant<int, q>) for q = 0..N-1 as straight-line code (a
+// fold expression, not a loop), so per-lane register arrays indexed by q are
+// always indexed by constants and stay in registers.
+template <class F, int... Q>
+__device__ __forceinline__ void unroll_seq(F &&f, std::integer_sequence<int, Q...>) {
+    (f(std::integral_constant<int, Q>{}), ...);
+}
+template <int N, class F> __device__ __forceinline__ void unroll(F &&f) {
+    unroll_seq(f, std::make_integer_sequence<int, N>{});
+}
 
 constexpr uint64_t EMPTY_SLOT = ~0ull;
 
@@ -127,6 +140,27 @@ __device__ __forceinline__ uint32_t lds_exclusive_scan(uint32_t *a, uint32_t n, 
     for (uint32_t i = b; i < e; ++i) {
         uint32_t v = a[i];
         a[i] = pre;
+        pre += v;
+    }
+    __syncthreads();
+    return total;
+}
+
+// Exclusive scan of n LDS counters `a` (u32) into 16-bit starts `out` (the
+// caller guarantees every prefix fits 16 bits).  Ends with a barrier.
+template <int BLOCK>
+__device__ __forceinline__ uint32_t lds_exclusive_scan_u16(const uint32_t *a, uint16_t *out,
+                                                           uint32_t n, uint32_t *tmp) {
+    const uint32_t per = (n + BLOCK - 1) / BLOCK;
+    const uint32_t b = threadIdx.x * per;
+    const uint32_t e = min(b + per, n);
+    uint32_t s = 0;
+    for (uint32_t i = b; i < e; ++i) s += a[i];
+    uint32_t total;
+    uint32_t pre = block_exclusive_scan<BLOCK>(s, tmp, &total);
+    for (uint32_t i = b; i < e; ++i) {
+        const uint32_t v = a[i];
+        out[i] = (uint16_t)pre;
         pre += v;
     }
     __syncthreads();

@@ -7,23 +7,26 @@
 // one partition to several threads (parallel_radix_join.cpp:1299-1321).
 //
 // CDNA4 form:
-//   k_task_count  tasks per partition = ceil(|S_p| / split) if |R_p|,|S_p| > 0
+//   k_task_count  tasks per partition = ceil(|S_p| / split) if |R_p|,|S_p| > 0;
+//                 split <= CAP, so a task's S slice fits the registers
 //   scan          task bases (flat exclusive scan)
 //   k_task_fill   task -> (R range, S slice, segment) records, so a worker
 //                 finds its task with one load
 //   k_join        persistent work-groups take tasks round-robin (t = wg,
-//                 wg + grid, ...).  A task first loads part of its S slice
-//                 into registers, then takes R_p
-//                 in rounds of `cap` tuples; each round counting-sorts R by
-//                 bucket into LDS (bucket starts + sorted tuples, about four
-//                 tuples per bucket) and probes every S tuple against one
-//                 bucket.  Count and checksum are reduced per work-group and
-//                 added once to the segment's 64-bit accumulators.  With pairs
-//                 enabled each round probes twice: count, reserve a range with
-//                 one atomic, write.
+//                 wg + grid, ...).  A task's first CAP tuples of R_p and its
+//                 whole S slice arrive in registers before the task starts
+//                 (loaded during the previous task); the build counting-sorts
+//                 R by bucket into LDS (16-bit bucket starts, about two
+//                 tuples per bucket) straight from the registers, the next
+//                 task's R is requested, and every S tuple probes one bucket
+//                 (six S tuples of a lane step through their buckets together).  R partitions larger than
+//                 CAP take further rounds read from memory.  Each task stores
+//                 its count and checksum; k_seg_reduce adds them per segment.
+//                 With pairs enabled each round probes twice: count, reserve a
+//                 range with one atomic, write.
 //
-// Bytes per tuple: R and S are each read once (S again per extra R round when
-// the slice does not fit the registers).
+// Bytes per tuple: R and S are each read once (R partitions above CAP: the
+// extra rounds read their R tuples twice).
 #include <hipcub/hipcub.hpp>
 
 #include "aj_internal.h"
@@ -31,20 +34,30 @@
 
 namespace aj {
 
-constexpr int SREG = 4;  // S tuples per thread loaded ahead of the build
-constexpr int JUNROLL = 4;
+constexpr int MAX_BUCKET_LG = 12;  // at most 4096 buckets per round
 
-// Buckets of a round of m tuples: nb = max(1, next_pow2(m) / 4).
+// Buckets of a round of m tuples: the power of two nb >= m/2 (about two
+// tuples per bucket), at most 2^MAX_BUCKET_LG.
 __host__ __device__ constexpr uint32_t max_buckets(uint32_t cap) {
     uint32_t p = 1;
-    while (p < cap) p *= 2;
-    return p >= 4 ? p / 4 : 1;
+    while (2 * p < cap && p < (1u << MAX_BUCKET_LG)) p *= 2;
+    return p;
 }
 __device__ __forceinline__ int bucket_lg(uint32_t m) {
     int lg = 0;
-    while ((4u << lg) < m) ++lg;
+    while ((2u << lg) < m && lg < MAX_BUCKET_LG) ++lg;
     return lg;
 }
+
+// LDS bytes of one round's table: the tuples in bucket order (8 bytes each)
+// plus one spare slot, and nb+1 16-bit bucket starts.
+__host__ __device__ constexpr size_t join_lds_bytes(uint32_t cap) {
+    return sizeof(uint64_t) * (cap + 1) + sizeof(uint16_t) * (max_buckets(cap) + 2);
+}
+
+// S tuples per lane probed together: their candidate tuples are read from LDS
+// in one batch per step, so the LDS latency is paid once per step and group.
+constexpr int PGROUP = 6;
 
 struct JoinArgs {
     const uint64_t *R, *S, *roff, *soff;
@@ -132,89 +145,149 @@ __global__ void k_task_fill(JoinArgs a) {
     }
 }
 
-__device__ __forceinline__ uint32_t bucket_of(uint32_t key, int lg) {
-    return lg ? (key * 0x9E3779B1u) >> (32 - lg) : 0u;
+__device__ __forceinline__ uint32_t umin32(uint32_t x, uint32_t y) { return x < y ? x : y; }
+
+// threadIdx.x behind an empty asm statement: the per-tuple indices derived from
+// it are recomputed where they are used instead of being hoisted out of the
+// task loop and kept live (or spilled) across every phase.
+__device__ __forceinline__ uint32_t opaque_tid() {
+    uint32_t t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    return t;
 }
 
-// Counting-sort m (<= CAP) tuples of R into `tab` by bucket; `bst` receives
-// nb+1 bucket starts.  Only the ranks stay in registers across the scan; the
-// tuples are read again (cache-hot) to place them.  Returns log2(nb).
+// Bucket of a key among 2^lg (0 <= lg <= 31): the top lg bits of a
+// multiplicative hash (high word of hash * 2^lg, which is 0 for lg = 0).
+__device__ __forceinline__ uint32_t bucket_of(uint32_t key, int lg) {
+    return __umulhi(key * 0x9E3779B1u, 1u << lg);
+}
+
+// Exclusive scan of the nb (a power of two, at most 8*JB) bucket counters
+// `cnt` into 16-bit starts bst[0..nb]; bst[nb] = total.  Each lane takes
+// nb/JB consecutive counters (at least one), read in one batch.  Ends with a
+// barrier.
+template <int JB>
+__device__ __forceinline__ void bucket_scan(const uint32_t *cnt, uint16_t *bst, uint32_t nb,
+                                            uint32_t *scan_tmp, uint32_t tid) {
+    constexpr int MAXPER = 8;
+    const uint32_t per = nb > JB ? nb / JB : 1u;
+    const uint32_t b = tid * per;
+    uint32_t v[MAXPER];
+    uint32_t s = 0;
+    unroll<MAXPER>([&](auto j) {
+        const bool ok = j < per && b + j < nb;
+        const uint32_t x = cnt[ok ? b + j : 0];
+        v[j] = ok ? x : 0u;
+        s += v[j];
+    });
+    uint32_t total;
+    uint32_t pre = block_exclusive_scan<JB>(s, scan_tmp, &total);
+    unroll<MAXPER>([&](auto j) {
+        if (j < per && b + j < nb) bst[b + j] = (uint16_t)pre;
+        pre += v[j];
+    });
+    if (tid == 0) bst[nb] = (uint16_t)total;
+    __syncthreads();
+}
+
+// Counting-sort the m (<= CAP) tuples of R held in registers (element q of
+// `rv` is tuple tid + q*JB, valid below m) by bucket into `tab`.  Count
+// (counters in `tab`), scan into the 16-bit bucket starts `bst`, then place
+// each tuple at a position taken by an LDS atomic on its bucket's 16-bit
+// cursor (two cursors share one 32-bit word; no carry crosses the halves since
+// every cursor stays below 2^16).  After the build bst[h] is the END of bucket
+// h: bucket h spans [h ? bst[h-1] : 0, bst[h]).  No per-tuple rank is kept in
+// registers.  Branch-free: lanes without a tuple count into the spare counter
+// nb+1, advance the spare cursor nb+1 and store into the spare slot tab[CAP].
+// Returns log2(nb).
 template <int JB, int CAP>
-__device__ __forceinline__ int build_round(const uint64_t *R, uint32_t m, uint64_t *tab,
-                                           uint32_t *bst, uint32_t *scan_tmp) {
+__device__ __forceinline__ int build_regs(const uint64_t (&rv)[CAP / JB], uint32_t m,
+                                          uint64_t *tab, uint16_t *bst, uint32_t *scan_tmp,
+                                          uint32_t tid) {
     constexpr int ITEMS = CAP / JB;
+    static_assert(max_buckets(CAP) <= 8u * JB, "bucket scan takes at most 8 counters per lane");
     const int lg = bucket_lg(m);
     const uint32_t nb = 1u << lg;
-    const uint32_t *Rk = reinterpret_cast<const uint32_t *>(R) + 1;  // key word of each tuple
-    uint32_t k[ITEMS];
-#pragma unroll
-    for (int q = 0; q < ITEMS; ++q) {
-        const uint32_t i = threadIdx.x + q * JB;
-        k[q] = i < m ? Rk[2 * i] : 0u;
-    }
-    for (uint32_t i = threadIdx.x; i <= nb; i += JB) bst[i] = 0;
+    uint32_t *cnt = reinterpret_cast<uint32_t *>(tab);
+    for (uint32_t i = threadIdx.x; i < nb + 2; i += JB) cnt[i] = 0;
+    if (threadIdx.x == 0) bst[nb + 1] = 0;
     __syncthreads();
-#pragma unroll
-    for (int q = 0; q < ITEMS; ++q) {
-        const uint32_t i = threadIdx.x + q * JB;
-        if (i < m) k[q] = atomicAdd(&bst[bucket_of(k[q], lg)], 1u);  // key -> rank
-    }
+    unroll<ITEMS>([&](auto q) {
+        const uint32_t h = bucket_of(key_of(rv[q]), lg);
+        atomicAdd(&cnt[tid + q * JB < m ? h : nb + 1], 1u);
+    });
     __syncthreads();
-    lds_exclusive_scan<JB>(bst, nb + 1, scan_tmp);
-#pragma unroll
-    for (int q = 0; q < ITEMS; ++q) {
-        const uint32_t i = threadIdx.x + q * JB;
-        if (i < m) {
-            const uint64_t t = R[i];
-            tab[bst[bucket_of(key_of(t), lg)] + k[q]] = t;
-        }
-    }
+    bucket_scan<JB>(cnt, bst, nb, scan_tmp, tid);
+    uint32_t *cur = reinterpret_cast<uint32_t *>(bst);
+    unroll<ITEMS>([&](auto q) {
+        const bool ok = tid + q * JB < m;
+        const uint32_t hb = bucket_of(key_of(rv[q]), lg);
+        const uint32_t h = ok ? hb : nb + 1;
+        const uint32_t sh = (h & 1u) * 16;
+        const uint32_t old = atomicAdd(&cur[h >> 1], 1u << sh);
+        tab[ok ? (old >> sh) & 0xffffu : (uint32_t)CAP] = rv[q];
+    });
     __syncthreads();
     return lg;
 }
 
-template <bool WRITE>
-__device__ __forceinline__ void probe_one(uint64_t s, const uint64_t *tab, const uint32_t *bst,
-                                          int lg, uint64_t &cnt, uint64_t &sum, uint64_t *pairs,
-                                          uint64_t &wpos, uint64_t cap) {
-    const uint32_t k = key_of(s);
-    const uint32_t h = bucket_of(k, lg);
-    const uint32_t e = bst[h + 1];
-    for (uint32_t x = bst[h]; x < e; ++x) {
-        const uint64_t r = tab[x];
-        if (key_of(r) == k) {
-            if (WRITE) {
-                if (wpos < cap) pairs[wpos] = pair_value(r, s);
-                ++wpos;
-            } else {
-                ++cnt;
-                sum += pair_value(r, s);
-            }
+// Probe S tuples held in registers against the round's table (bucket ends in
+// `bst`): element j of `sv` is tuple tid + (Q0 + j)*JB of the task's slice
+// (valid below ns).
+// PGROUP tuples of a lane advance together through their buckets: each step
+// reads one candidate per tuple (unconditionally, clamped inside the table)
+// and then compares, so a step costs one LDS round trip.
+template <int JB, int CAP, bool WRITE, int Q0, int N, int G0 = 0>
+__device__ __forceinline__ void probe_part(const uint64_t (&sv)[N], uint32_t ns,
+                                           const uint64_t *tab, const uint16_t *bst, int lg,
+                                           uint64_t &cnt, uint64_t &sum, uint64_t *pairs,
+                                           uint64_t &wpos, uint64_t cap, uint32_t tid) {
+    if constexpr (G0 < N) {
+        constexpr int G = N - G0 < PGROUP ? N - G0 : PGROUP;
+        uint32_t be[G];  // bucket cursor | end << 16
+        unroll<G>([&](auto j) {
+            const uint32_t h = bucket_of(key_of(sv[G0 + j]), lg);
+            const uint32_t lo = bst[h ? h - 1 : 0], hi = bst[h];
+            be[j] = tid + (uint32_t)(Q0 + G0 + j) * JB < ns ? (h ? lo : 0u) | (hi << 16) : 0u;
+        });
+        for (;;) {
+            uint64_t r[G];
+            unroll<G>([&](auto j) { r[j] = tab[umin32(be[j] & 0xffffu, CAP - 1)]; });
+            bool more = false;
+            unroll<G>([&](auto j) {
+                const uint32_t x = be[j] & 0xffffu, e = be[j] >> 16;
+                if (x < e) {
+                    if (key_of(r[j]) == key_of(sv[G0 + j])) {
+                        const uint64_t v = (r[j] << 32) | payload_of(sv[G0 + j]);
+                        if (WRITE) {
+                            if (wpos < cap) pairs[wpos] = v;
+                            ++wpos;
+                        } else {
+                            ++cnt;
+                            sum += v;
+                        }
+                    }
+                    be[j] += 1;
+                    more |= x + 1 < e;
+                }
+            });
+            if (!more) break;
         }
+        probe_part<JB, CAP, WRITE, Q0, N, G0 + PGROUP>(sv, ns, tab, bst, lg, cnt, sum, pairs,
+                                                       wpos, cap, tid);
     }
 }
 
-// Probe the task's S slice: the first SREG*JB tuples from registers, the rest
-// streamed from memory.
-template <int JB, bool WRITE>
-__device__ __forceinline__ void probe_task(const uint64_t (&sreg)[SREG], const uint64_t *S,
-                                           uint64_t ns, const uint64_t *tab, const uint32_t *bst,
-                                           int lg, uint64_t &cnt, uint64_t &sum, uint64_t *pairs,
-                                           uint64_t &wpos, uint64_t cap) {
-#pragma unroll
-    for (int q = 0; q < SREG; ++q)
-        if (threadIdx.x + (uint64_t)q * JB < ns)
-            probe_one<WRITE>(sreg[q], tab, bst, lg, cnt, sum, pairs, wpos, cap);
-    uint64_t i = (uint64_t)SREG * JB + threadIdx.x;
-    for (; i + (JUNROLL - 1) * JB < ns; i += JUNROLL * JB) {
-        uint64_t s[JUNROLL];
-#pragma unroll
-        for (int u = 0; u < JUNROLL; ++u) s[u] = S[i + u * JB];
-#pragma unroll
-        for (int u = 0; u < JUNROLL; ++u)
-            probe_one<WRITE>(s[u], tab, bst, lg, cnt, sum, pairs, wpos, cap);
-    }
-    for (; i < ns; i += JB) probe_one<WRITE>(S[i], tab, bst, lg, cnt, sum, pairs, wpos, cap);
+// Tuples tid + (Q0 + j)*JB, j < N, of `p` (n >= 1 tuples) into registers.
+// The loads are unconditional so that all of them are in flight at once: an
+// index at or past n reads tuple n-1 again, and callers ignore those elements.
+template <int JB, int Q0, int N>
+__device__ __forceinline__ void load_part(uint64_t (&v)[N], const uint64_t *p, uint32_t n,
+                                          uint32_t tid) {
+    unroll<N>([&](auto j) {
+        const uint32_t i = tid + (uint32_t)(Q0 + j) * JB;
+        v[j] = __builtin_nontemporal_load(p + umin32(i, n - 1));
+    });
 }
 
 // A task record is two uint4: {r0 lo, r0 hi, |R_p|, segment}, {s0 lo, s0 hi,
@@ -233,51 +306,57 @@ __device__ __forceinline__ TaskRec decode_task(uint4 a, uint4 b) {
     return r;
 }
 
-template <int JB>
-__device__ __forceinline__ void load_sreg(uint64_t (&sreg)[SREG], const uint64_t *S, uint64_t ns) {
-#pragma unroll
-    for (int q = 0; q < SREG; ++q) {
-        const uint64_t i = threadIdx.x + (uint64_t)q * JB;
-        sreg[q] = i < ns ? __builtin_nontemporal_load(S + i) : 0;
-    }
-}
-
 template <int JB, int CAP, bool PAIRS>
 __global__ __launch_bounds__(JB, 4) void k_join(JoinArgs a) {
+    constexpr int ITEMS = CAP / JB;
+    static_assert(ITEMS * JB == CAP, "CAP must be a multiple of the work-group size");
     extern __shared__ uint64_t jsm[];
-    uint64_t *tab = jsm;                                     // CAP tuples
-    uint32_t *bst = reinterpret_cast<uint32_t *>(tab + CAP);  // max_buckets(CAP) + 1 starts
+    uint64_t *tab = jsm;  // CAP tuples in bucket order + a spare slot
+    uint16_t *bst = reinterpret_cast<uint16_t *>(jsm + CAP + 1);  // max_buckets(CAP) + 1 starts
     __shared__ uint64_t red[2 * (JB / 64)];
     __shared__ uint32_t scan_tmp[JB / 64];
     __shared__ uint64_t s_base;
     constexpr int NW = JB / 64;
 
     // Static round-robin over tasks (a shared device-wide task counter costs one
-    // serialised atomic per task).  The next task's record and its first S
-    // tuples are fetched while the current task runs.
+    // serialised atomic per task).  The S slice (at most CAP tuples) is read
+    // in two halves: the first half of the next task's slice is loaded while
+    // this task finishes, the second half after the build, in flight while the
+    // first half probes.
+    constexpr int SA = ITEMS / 2, SB = ITEMS - SA;
     const uint32_t total = a.task_base[a.max_parts];
     uint32_t t = blockIdx.x;
     if (t >= total) return;
     TaskRec cur = decode_task(a.task_info[2 * t], a.task_info[2 * t + 1]);
-    uint64_t sreg[SREG];
-    load_sreg<JB>(sreg, a.S + cur.s0, cur.ns);
+    uint64_t sa[SA];
+    load_part<JB, 0, SA>(sa, a.S + cur.s0, cur.ns, opaque_tid());
     for (;;) {
         const uint32_t tn = t + gridDim.x;
         const bool has_next = tn < total;
-        uint4 n0 = make_uint4(0, 0, 0, 0), n1 = n0;
-        if (has_next) {
-            n0 = a.task_info[2 * tn];
-            n1 = a.task_info[2 * tn + 1];
-        }
-        const uint64_t *S = a.S + cur.s0;
-        const uint64_t ns = cur.ns;
-        const uint64_t r1 = cur.r0 + cur.nr;
+        // the last task reads its own record again, so the loads below need
+        // no condition
+        const uint32_t tl = has_next ? tn : t;
+        const uint4 n0 = a.task_info[2 * tl], n1 = a.task_info[2 * tl + 1];
+        const uint32_t ns = cur.ns;
         uint64_t cnt = 0, sum = 0;
-        for (uint64_t rb = cur.r0; rb < r1; rb += CAP) {
-            const uint32_t m = (uint32_t)min<uint64_t>(CAP, r1 - rb);
-            const int lg = build_round<JB, CAP>(a.R + rb, m, tab, bst, scan_tmp);
+        for (uint64_t rb = cur.r0; rb < cur.r0 + cur.nr; rb += CAP) {
+            // one LDS round of R: CAP tuples (more rounds only for partitions
+            // larger than CAP)
+            const uint32_t m = (uint32_t)min<uint64_t>(CAP, cur.r0 + cur.nr - rb);
+            int lg;
+            {
+                uint64_t rv[ITEMS];
+                load_part<JB, 0, ITEMS>(rv, a.R + rb, m, opaque_tid());
+                lg = build_regs<JB, CAP>(rv, m, tab, bst, scan_tmp, opaque_tid());
+            }
+            // second half of the S slice: in flight while the first half probes
+            uint64_t sb[SB];
+            load_part<JB, SA, SB>(sb, a.S + cur.s0, ns, opaque_tid());
             uint64_t c = 0, x = 0, wpos = 0;
-            probe_task<JB, false>(sreg, S, ns, tab, bst, lg, c, x, nullptr, wpos, 0);
+            probe_part<JB, CAP, false, 0, SA>(sa, ns, tab, bst, lg, c, x, nullptr, wpos, 0,
+                                             opaque_tid());
+            probe_part<JB, CAP, false, SA, SB>(sb, ns, tab, bst, lg, c, x, nullptr, wpos, 0,
+                                              opaque_tid());
             if (PAIRS) {
                 uint32_t total_c;
                 const uint32_t pre = block_exclusive_scan<JB>((uint32_t)c, scan_tmp, &total_c);
@@ -286,17 +365,18 @@ __global__ __launch_bounds__(JB, 4) void k_join(JoinArgs a) {
                 __syncthreads();
                 wpos = s_base + pre;
                 uint64_t dc = 0, dx = 0;
-                probe_task<JB, true>(sreg, S, ns, tab, bst, lg, dc, dx, a.pairs, wpos, a.pairs_cap);
+                probe_part<JB, CAP, true, 0, SA>(sa, ns, tab, bst, lg, dc, dx, a.pairs, wpos,
+                                                a.pairs_cap, opaque_tid());
+                probe_part<JB, CAP, true, SA, SB>(sb, ns, tab, bst, lg, dc, dx, a.pairs, wpos,
+                                                 a.pairs_cap, opaque_tid());
             }
             cnt += c;
             sum += x;
             __syncthreads();  // the next round rebuilds tab/bst
         }
-        TaskRec nxt{};
-        if (has_next) {
-            nxt = decode_task(n0, n1);
-            load_sreg<JB>(sreg, a.S + nxt.s0, nxt.ns);  // in flight during the reduction
-        }
+        const TaskRec nxt = decode_task(n0, n1);
+        // in flight during the reduction and the next task's build
+        load_part<JB, 0, SA>(sa, a.S + nxt.s0, nxt.ns, opaque_tid());
         // count and checksum reduced together: one barrier pair per task
         cnt = wave_sum(cnt);
         sum = wave_sum(sum);
@@ -331,13 +411,17 @@ size_t join_scan_tmp_bytes(uint64_t max_parts) {
     return bytes;
 }
 
+uint32_t join_task_split(uint32_t split, int cap) {
+    return split ? std::min<uint32_t>(split, (uint32_t)cap) : (uint32_t)cap;
+}
+
 uint64_t join_max_tasks(uint64_t max_parts, uint64_t max_S, uint32_t split) {
     return max_parts + (split ? max_S / split : max_S) + 1;
 }
 
 template <int JB, int CAP, bool PAIRS>
 static int launch_join(const JoinArgs &a, int num_cu, hipStream_t st) {
-    const size_t lds = sizeof(uint64_t) * CAP + sizeof(uint32_t) * (max_buckets(CAP) + 2);
+    const size_t lds = join_lds_bytes(CAP);
     static bool attr = false;
     if (!attr) {
         AJ_HIP(hipFuncSetAttribute((const void *)k_join<JB, CAP, PAIRS>,
@@ -359,8 +443,10 @@ static int launch_join2(const JoinArgs &a, bool pairs, int num_cu, hipStream_t s
 }
 
 int run_join(const JoinDesc &d, hipStream_t st, const Marks *mk, int m0) {
-    if (d.split == 0 || d.max_parts >= (1ull << 31) || !d.task_info) {
-        set_error(AJ_ERR_ARG, "bad join split/partition count");
+    if (d.split == 0 || d.split > (uint32_t)d.cap || d.max_parts >= (1ull << 31) ||
+        !d.task_info) {
+        set_error(AJ_ERR_ARG, "bad join split/partition count (split %u, cap %d)", d.split,
+                  d.cap);
         return AJ_ERR_ARG;
     }
     JoinArgs a;
@@ -403,9 +489,10 @@ int run_join(const JoinDesc &d, hipStream_t st, const Marks *mk, int m0) {
         case 2048: rc = launch_join2<256, 2048>(a, pairs, d.num_cu, st); break;
         case 4096: rc = launch_join2<256, 4096>(a, pairs, d.num_cu, st); break;
         case 8192: rc = launch_join2<512, 8192>(a, pairs, d.num_cu, st); break;
-        case 12288: rc = launch_join2<512, 12288>(a, pairs, d.num_cu, st); break;
+        case 8704: rc = launch_join2<512, 8704>(a, pairs, d.num_cu, st); break;
+        case 12288: rc = launch_join2<1024, 12288>(a, pairs, d.num_cu, st); break;
         default:
-            set_error(AJ_ERR_ARG, "cap must be 2048, 4096, 8192 or 12288 (got %d)", d.cap);
+            set_error(AJ_ERR_ARG, "cap must be 2048, 4096, 8192, 8704 or 12288 (got %d)", d.cap);
             return AJ_ERR_ARG;
     }
     if (rc) return rc;

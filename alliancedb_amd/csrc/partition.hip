@@ -29,9 +29,8 @@
 
 namespace aj {
 
-constexpr int PBLOCK = 512;
-constexpr int PITEMS = 16;
-constexpr int PTILE = PBLOCK * PITEMS;  // 8192 tuples per tile
+constexpr int PITEMS = 16;                // tuples per lane per tile
+constexpr int PTILE_MAX = 512 * PITEMS;   // largest tile (512-lane work-groups)
 constexpr int MAX_PASS_BITS = 13;
 
 struct PassArgs {
@@ -147,13 +146,14 @@ __device__ __forceinline__ bool locate_chunk(const PassArgs &a, uint64_t g, Chun
 // 8-byte tuples are read two per lane with 16-byte loads (consecutive lanes on
 // consecutive 16 bytes); a load never reaches past `hi`, so nothing outside
 // the segment's array is touched.  12-byte stream tuples are read one per lane.
+template <int PBLOCK>
 __device__ __forceinline__ void load_tile(const void *in, int stride, uint64_t tb, uint64_t lo,
                                           uint64_t hi, uint64_t (&t)[PITEMS],
                                           bool (&v)[PITEMS]) {
     if (stride == 8) {
         typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
         const uint64_t *p = static_cast<const uint64_t *>(in);
-        if (tb >= lo && tb + PTILE <= hi) {  // whole tile inside: unconditional 16-byte loads
+        if (tb >= lo && tb + PBLOCK * PITEMS <= hi) {  // whole tile inside: unconditional 16-byte loads
 #pragma unroll
             for (int j = 0; j < PITEMS / 2; ++j) {
                 const u64x2 q = __builtin_nontemporal_load(
@@ -198,8 +198,9 @@ __device__ __forceinline__ uint64_t tile_start(uint64_t begin, int stride) {
 }
 
 // ---------------------------------------------------------------- k_hist
-template <int STRIDE>
+template <int STRIDE, int PBLOCK>
 __global__ __launch_bounds__(PBLOCK) void k_hist(PassArgs a) {
+    constexpr int PTILE = PBLOCK * PITEMS;
     extern __shared__ uint32_t cnt[];
     ChunkLoc L;
     if (!locate_chunk(a, blockIdx.x, L)) return;
@@ -211,7 +212,7 @@ __global__ __launch_bounds__(PBLOCK) void k_hist(PassArgs a) {
     for (uint64_t base = tile_start(L.begin, STRIDE); base < L.end; base += PTILE) {
         uint64_t t[PITEMS];
         bool v[PITEMS];
-        load_tile(in, STRIDE, base, L.begin, L.end, t, v);
+        load_tile<PBLOCK>(in, STRIDE, base, L.begin, L.end, t, v);
 #pragma unroll
         for (int j = 0; j < PITEMS; ++j) {
             const uint32_t d = (key_of(t[j]) >> a.shift) & mask;
@@ -224,8 +225,9 @@ __global__ __launch_bounds__(PBLOCK) void k_hist(PassArgs a) {
 }
 
 // ---------------------------------------------------------------- k_scatter
-template <int STRIDE>
+template <int STRIDE, int PBLOCK>
 __global__ __launch_bounds__(PBLOCK) void k_scatter(PassArgs a) {
+    constexpr int PTILE = PBLOCK * PITEMS;
     extern __shared__ uint64_t smem[];
     __shared__ uint32_t scan_tmp[PBLOCK / 64];
     ChunkLoc L;
@@ -247,7 +249,7 @@ __global__ __launch_bounds__(PBLOCK) void k_scatter(PassArgs a) {
         for (uint32_t d = threadIdx.x; d <= F; d += PBLOCK) lofs[d] = 0;
         uint64_t t[PITEMS];
         bool v[PITEMS];
-        load_tile(in, STRIDE, base, L.begin, L.end, t, v);
+        load_tile<PBLOCK>(in, STRIDE, base, L.begin, L.end, t, v);
         __syncthreads();
         uint32_t rank[PITEMS];
 #pragma unroll
@@ -316,16 +318,47 @@ size_t pass_scan_tmp_bytes(uint64_t max_hist) {
 // Raises a kernel's dynamic-LDS limit once to the largest size requested so far
 // (callers hold the context lock).
 static int lds_limit_set(const void *fn, size_t bytes) {
-    static const void *fns[8] = {};
-    static size_t set[8] = {};
+    static const void *fns[16] = {};
+    static size_t set[16] = {};
     int i = 0;
-    while (i < 8 && fns[i] && fns[i] != fn) ++i;
-    if (i == 8) i = 7;
+    while (i < 16 && fns[i] && fns[i] != fn) ++i;
+    if (i == 16) i = 15;
     if (!fns[i]) fns[i] = fn;
     if (bytes > set[i]) {
         AJ_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
         set[i] = bytes;
     }
+    return AJ_OK;
+}
+
+template <int STRIDE, int PBLOCK>
+static int launch_pass_kernels(const PassDesc &d, const PassWork &w, const PassArgs &a,
+                               hipStream_t st, const Marks *mk, int m0) {
+    const uint32_t F = 1u << d.max_bits;
+    const size_t lds_hist = sizeof(uint32_t) * F;
+    const size_t lds_scatter =
+        sizeof(uint64_t) * PBLOCK * PITEMS + sizeof(uint32_t) * (2 * F + 1);
+    AJ_TRY(lds_limit_set((const void *)k_hist<STRIDE, PBLOCK>, lds_hist));
+    AJ_TRY(lds_limit_set((const void *)k_scatter<STRIDE, PBLOCK>, lds_scatter));
+    if (mk) mk->mark(m0);
+    if (d.max_chunks)
+        hipLaunchKernelGGL((k_hist<STRIDE, PBLOCK>), dim3((unsigned)d.max_chunks), dim3(PBLOCK),
+                           lds_hist, st, a);
+    if (mk) mk->mark(m0 + 1);
+    size_t tmp_bytes = w.scan_tmp_bytes;
+    if (d.max_hist) {
+        hipError_t e = hipcub::DeviceScan::ExclusiveSum(w.scan_tmp, tmp_bytes, a.hist,
+                                                        a.hist_scan, (int)d.max_hist, st);
+        if (e != hipSuccess) {
+            set_error(AJ_ERR_HIP, "histogram scan failed: %s", hipGetErrorString(e));
+            return AJ_ERR_HIP;
+        }
+    }
+    if (mk) mk->mark(m0 + 2);
+    if (d.max_chunks)
+        hipLaunchKernelGGL((k_scatter<STRIDE, PBLOCK>), dim3((unsigned)d.max_chunks),
+                           dim3(PBLOCK), lds_scatter, st, a);
+    if (mk) mk->mark(m0 + 3);
     return AJ_OK;
 }
 
@@ -336,8 +369,16 @@ int run_partition_pass(const PassDesc &d, const PassWork &w, hipStream_t st, con
                   d.max_bits);
         return AJ_ERR_ARG;
     }
-    if (d.chunk % PTILE) {
-        set_error(AJ_ERR_ARG, "chunk must be a multiple of %d", PTILE);
+    if (d.block != 256 && d.block != 512) {
+        set_error(AJ_ERR_ARG, "partition work-group size must be 256 or 512, got %d", d.block);
+        return AJ_ERR_ARG;
+    }
+    if (d.chunk % PTILE_MAX) {
+        set_error(AJ_ERR_ARG, "chunk must be a multiple of %d", PTILE_MAX);
+        return AJ_ERR_ARG;
+    }
+    if (d.in_stride != 8 && d.in_stride != 12) {
+        set_error(AJ_ERR_ARG, "tuple stride must be 8 or 12 bytes");
         return AJ_ERR_ARG;
     }
     if (d.n_total[0] + d.n_total[1] >= (1ull << 32)) {
@@ -378,48 +419,17 @@ int run_partition_pass(const PassDesc &d, const PassWork &w, hipStream_t st, con
     a.max_hist = d.max_hist;
     a.max_parts = d.max_parts;
 
-    const uint32_t F = 1u << d.max_bits;
-    const size_t lds_hist = sizeof(uint32_t) * F;
-    const size_t lds_scatter = sizeof(uint64_t) * PTILE + sizeof(uint32_t) * (2 * F + 1);
-    if (d.in_stride != 8 && d.in_stride != 12) {
-        set_error(AJ_ERR_ARG, "tuple stride must be 8 or 12 bytes");
-        return AJ_ERR_ARG;
-    }
-    const bool s8 = d.in_stride == 8;
-    const void *fh = s8 ? (const void *)k_hist<8> : (const void *)k_hist<12>;
-    const void *fs = s8 ? (const void *)k_scatter<8> : (const void *)k_scatter<12>;
-    AJ_TRY(lds_limit_set(fh, lds_hist));
-    AJ_TRY(lds_limit_set(fs, lds_scatter));
-
     hipLaunchKernelGGL(k_plan, dim3(1), dim3(PLAN_BLOCK), 0, st, a);
     hipLaunchKernelGGL(k_zero_tail, dim3(256), dim3(256), 0, st, a.hist, a.header, d.max_hist);
-    if (mk) mk->mark(m0);
-    if (d.max_chunks) {
-        if (s8)
-            hipLaunchKernelGGL(k_hist<8>, dim3((unsigned)d.max_chunks), dim3(PBLOCK), lds_hist, st, a);
-        else
-            hipLaunchKernelGGL(k_hist<12>, dim3((unsigned)d.max_chunks), dim3(PBLOCK), lds_hist, st, a);
-    }
-    if (mk) mk->mark(m0 + 1);
-    size_t tmp_bytes = w.scan_tmp_bytes;
-    if (d.max_hist) {
-        hipError_t e = hipcub::DeviceScan::ExclusiveSum(w.scan_tmp, tmp_bytes, a.hist,
-                                                        a.hist_scan, (int)d.max_hist, st);
-        if (e != hipSuccess) {
-            set_error(AJ_ERR_HIP, "histogram scan failed: %s", hipGetErrorString(e));
-            return AJ_ERR_HIP;
-        }
-    }
-    if (mk) mk->mark(m0 + 2);
-    if (d.max_chunks) {
-        if (s8)
-            hipLaunchKernelGGL(k_scatter<8>, dim3((unsigned)d.max_chunks), dim3(PBLOCK),
-                               lds_scatter, st, a);
-        else
-            hipLaunchKernelGGL(k_scatter<12>, dim3((unsigned)d.max_chunks), dim3(PBLOCK),
-                               lds_scatter, st, a);
-    }
-    if (mk) mk->mark(m0 + 3);
+    const bool s8 = d.in_stride == 8;
+    int rc;
+    if (d.block == 512)
+        rc = s8 ? launch_pass_kernels<8, 512>(d, w, a, st, mk, m0)
+                : launch_pass_kernels<12, 512>(d, w, a, st, mk, m0);
+    else
+        rc = s8 ? launch_pass_kernels<8, 256>(d, w, a, st, mk, m0)
+                : launch_pass_kernels<12, 256>(d, w, a, st, mk, m0);
+    AJ_TRY(rc);
     const unsigned pgrid = (unsigned)std::min<uint64_t>((d.max_parts + 256) / 256, 4096);
     hipLaunchKernelGGL(k_part_off, dim3(pgrid), dim3(256), 0, st, a);
     AJ_HIP(hipGetLastError());

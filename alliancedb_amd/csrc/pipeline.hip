@@ -65,15 +65,18 @@ static uint32_t pick_chunk(uint64_t n_total, int num_cu) {
 ProConfig choose_pro_config(uint64_t nR, uint64_t nS, const aj_join_opts_t *o) {
     ProConfig c;
     if (o && o->cap) c.cap = o->cap;
-    if (o && o->split) c.split = o->split;
+    c.split = (int)join_task_split(o ? o->split : 0u, c.cap);
     int bits;
     if (o && o->radix_bits > 0) {
         bits = o->radix_bits;
     } else {
-        // average R partition of about one LDS round (measured best at 2^27:
-        // larger partitions halve the join tasks; the overflow takes a 2nd round)
+        // average R partition of the largest power of two that fits one LDS
+        // round; a round's capacity above that power of two (8704 = 8192 +
+        // 512) absorbs the spread of partition sizes, so uniform keys take
+        // one round per partition
         bits = 0;
-        const uint64_t target = (uint64_t)c.cap;
+        uint64_t target = 1;
+        while (2 * target <= (uint64_t)c.cap) target *= 2;
         while (bits < 24 && (nR >> bits) > target) ++bits;
     }
     // one pass only while the scatter keeps runs of >= 16 tuples per 8K tile
@@ -113,7 +116,7 @@ static ProPlan plan_pro(uint64_t nR, uint64_t nS, const ProConfig &c, int num_cu
     p.b2 = c.bits2;
     const uint64_t F1 = 1ull << p.b1, F2 = 1ull << p.b2;
     const bool part = (p.b1 + p.b2) > 0;
-    p.chunk1 = pick_chunk(nR + nS, num_cu);
+    p.chunk1 = c.chunk ? (uint32_t)c.chunk : pick_chunk(nR + nS, num_cu);
     p.max_chunks1 = (nR + p.chunk1 - 1) / p.chunk1 + (nS + p.chunk1 - 1) / p.chunk1;
     p.max_hist1 = F1 * p.max_chunks1;
     p.max_parts1 = F1;
@@ -235,6 +238,7 @@ int pro_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
         d1.shift = 0;
         d1.max_bits = p.b1;
         d1.chunk = p.chunk1;
+        d1.block = cfg.block;
         d1.n_total[0] = nR;
         d1.n_total[1] = nS;
         d1.max_chunks = p.max_chunks1;

@@ -321,7 +321,7 @@ int aj_batch_plan_create(aj_ctx_t *ctx, uint64_t max_R, uint64_t max_S, int max_
     p.max_win = max_windows;
     p.max_window = max_window_tuples ? max_window_tuples : p.max_R;
     p.cap = (opts && opts->cap) ? opts->cap : 8192;
-    p.split = (opts && opts->split) ? opts->split : (1u << 16);
+    p.split = join_task_split(opts ? opts->split : 0u, p.cap);
     p.target = (opts && opts->target) ? opts->target : (uint32_t)p.cap / 2;
     uint32_t chunk = (opts && opts->chunk) ? opts->chunk : 0;
     if (!chunk) {
@@ -330,8 +330,8 @@ int aj_batch_plan_create(aj_ctx_t *ctx, uint64_t max_R, uint64_t max_S, int max_
     }
     p.chunk = (chunk + WTILE - 1) / WTILE * WTILE;
     p.use_graph = !(opts && opts->no_graph);
-    if (p.cap != 4096 && p.cap != 8192 && p.cap != 12288) {
-        set_error(AJ_ERR_ARG, "cap must be 4096, 8192 or 12288");
+    if (p.cap != 4096 && p.cap != 8192 && p.cap != 8704 && p.cap != 12288) {
+        set_error(AJ_ERR_ARG, "cap must be 4096, 8192, 8704 or 12288");
         return AJ_ERR_ARG;
     }
     AJ_TRY(plan_init(&p));

@@ -87,6 +87,13 @@ const aj_result_t *aj_result_ext(const result_t *r);
  *   "passes"       0..2  partition passes (0 = tuned default)
  *   "device"       >=0   HIP device of the default context
  *   "kernel_timing" 0/1  record per-kernel device times (aj_join_out_t.phase_ms)
+ *   "cap"          0, 2048, 4096, 8192, 8704, 12288  R tuples per LDS build
+ *                        round (0 = default 8704)
+ *   "split"        >=0   S tuples per join task (0 = cap; larger values are clamped
+ *                        to cap so that a task's S slice fits the registers)
+ *   "part_block"   0, 256, 512  lanes per partition work-group (0 = 512)
+ *   "part_chunk"   multiple of 8192  tuples per partition work-group (0 = derived)
+ *   "device_budget" bytes of device memory the library may hold (0 = unlimited)
  */
 int aj_set_option(const char *name, int64_t value);
 int64_t aj_get_option(const char *name);
@@ -117,7 +124,7 @@ typedef struct aj_join_opts_t {
     int radix_bits;    /* PRO: total radix bits, 0 = tuned */
     int passes;        /* PRO: 1 or 2, 0 = tuned */
     int cap;           /* PRO: R tuples per LDS build round, 0 = default */
-    uint32_t split;    /* PRO: S tuples per join task, 0 = default */
+    uint32_t split;    /* PRO: S tuples per join task, 0 = cap, at most cap */
     uint64_t *pairs;   /* device buffer for pairs, NULL = count/checksum only */
     uint64_t pairs_capacity;
 } aj_join_opts_t;
@@ -170,8 +177,8 @@ int aj_join_dev(aj_ctx_t *ctx, const tuple_t *dR, uint64_t nR, const tuple_t *dS
  * sizes; the kernel sequence is captured once into a hipGraph and replayed.
  */
 typedef struct aj_batch_opts_t {
-    int cap;          /* R tuples per LDS round (4096/8192/12288), 0 = 8192 */
-    uint32_t split;   /* S tuples per join task, 0 = 65536 */
+    int cap;          /* R tuples per LDS round (4096/8192/8704/12288), 0 = 8192 */
+    uint32_t split;   /* S tuples per join task, 0 = cap, at most cap */
     uint32_t target;  /* target R tuples per partition, 0 = cap/2 */
     uint32_t chunk;   /* tuples per partition work-group, 0 = derived */
     int no_graph;     /* 1 = enqueue kernels directly instead of a graph */
