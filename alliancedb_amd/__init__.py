@@ -6,6 +6,7 @@ include/alliancedb/aj.h; this package is its Python binding.
 from ._lib import (  # noqa: F401
     AJ_ALGO_NPO, AJ_ALGO_PRO, AJ_ERR_ARG, AJ_ERR_CAPACITY, AJ_ERR_HIP, AJ_ERR_NODEV,
     AJ_ERR_NOMEM, AJ_ERR_STATE, AJ_OK, LIB_PATH, NPO, PRO, STREAM_DTYPE, TUPLE_DTYPE,
-    AjError, BatchPlan, Context, DeviceArray, gen_window_streams, device_sync, JoinResult, as_tuples, gen_uniform, gen_unique, gen_zipf, get_option,
-    check, last_error, lib, set_option,
+    AjError, BatchPlan, Context, DeviceArray, JoinResult, Server, as_tuples, check,
+    device_sync, gen_stream_tuples, gen_uniform, gen_unique, gen_window_streams, gen_zipf,
+    get_option, last_error, lib, set_option,
 )

@@ -73,6 +73,30 @@ class aj_batch_opts_t(ctypes.Structure):
                 ("chunk", ctypes.c_uint32), ("no_graph", ctypes.c_int)]
 
 
+class aj_server_opts_t(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_uint64), ("slots", ctypes.c_int),
+                ("max_batch_windows", ctypes.c_int), ("max_batch_tuples", ctypes.c_uint64),
+                ("max_window_tuples", ctypes.c_uint64), ("cap", ctypes.c_int)]
+
+
+class aj_window_t(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_uint64), ("dR", ctypes.c_void_p), ("dS", ctypes.c_void_p),
+                ("nR", ctypes.c_uint64), ("nS", ctypes.c_uint64)]
+
+
+class aj_server_result_t(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_uint64), ("tag", ctypes.c_uint64), ("count", ctypes.c_uint64),
+                ("checksum", ctypes.c_uint64), ("status", ctypes.c_int),
+                ("batch_windows", ctypes.c_int), ("close_us", ctypes.c_double),
+                ("start_us", ctypes.c_double), ("done_us", ctypes.c_double)]
+
+
+class aj_server_stats_t(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in (
+        "windows_done", "batches", "tuples_done", "window_bytes", "window_bytes_used",
+        "window_bytes_peak", "workspace_bytes", "alloc_waits", "pending", "in_flight")]
+
+
 class AjError(RuntimeError):
     def __init__(self, status: int, msg: str):
         super().__init__(f"aj status {status}: {msg}")
@@ -110,6 +134,22 @@ def _load() -> ctypes.CDLL:
         "aj_batch_run": (c.c_int, [P, P, c.c_uint64, P, c.c_uint64, P, c.c_int, P, P, P]),
         "aj_gen_window_streams": (c.c_int, [P, c.c_uint64, c.c_int, P, c.c_double, c.c_uint32,
                                             P, c.c_uint64, P, c.POINTER(c.c_uint64)]),
+        "aj_gen_stream_tuples": (c.c_int, [P, P, c.c_uint64, c.c_uint64, c.c_uint32, c.c_double,
+                                           c.c_uint32, P]),
+        "aj_server_create": (c.c_int, [P, c.POINTER(aj_server_opts_t), c.POINTER(P)]),
+        "aj_server_destroy": (c.c_int, [P]),
+        "aj_server_window_alloc": (c.c_int, [P, c.c_uint64, c.c_uint64, c.c_int64,
+                                             c.POINTER(aj_window_t)]),
+        "aj_server_window_release": (c.c_int, [P, c.POINTER(aj_window_t)]),
+        "aj_server_window_fill": (c.c_int, [P, c.POINTER(aj_window_t), c.c_uint64, c.c_uint32,
+                                            c.c_double]),
+        "aj_server_window_upload": (c.c_int, [P, c.POINTER(aj_window_t), P, P]),
+        "aj_server_submit": (c.c_int, [P, c.POINTER(aj_window_t), c.c_uint64, c.c_double]),
+        "aj_server_poll": (c.c_int, [P, c.POINTER(aj_server_result_t), c.c_int, c.c_int64,
+                                     c.POINTER(c.c_int)]),
+        "aj_server_drain": (c.c_int, [P]),
+        "aj_server_now_us": (c.c_double, [P]),
+        "aj_server_stats": (c.c_int, [P, c.POINTER(aj_server_stats_t)]),
         "aj_gen_unique": (c.c_int, [P, c.c_uint64, c.c_uint64]),
         "aj_gen_uniform": (c.c_int, [P, c.c_uint64, c.c_uint32, c.c_uint32, c.c_uint64]),
         "aj_gen_zipf": (c.c_int, [P, c.c_uint64, c.c_uint32, c.c_double, c.c_uint64]),
@@ -263,6 +303,76 @@ def gen_window_streams(ctx: "Context", seed: int, sizes: np.ndarray, rate_per_us
                                     float(rate_per_us), dup, d_out, capacity, d_bounds,
                                     ctypes.byref(n)))
     return int(n.value)
+
+
+def gen_stream_tuples(ctx: "Context", d_out: int, n: int, seed: int, key_domain: int,
+                      theta: float = 0.0, payload0: int = 0) -> None:
+    """Device generator of n 12-byte window tuples (blocks until done)."""
+    check(lib.aj_gen_stream_tuples(ctx.handle, d_out, n, seed, key_domain, float(theta),
+                                   payload0, None))
+
+
+class Server:
+    """aj_server_t: admission by HBM budget, continuous batching over HIP streams."""
+
+    def __init__(self, ctx: "Context", hbm_budget: int = 0, slots: int = 0,
+                 max_batch_windows: int = 0, max_batch_tuples: int = 0,
+                 max_window_tuples: int = 0, cap: int = 0):
+        o = aj_server_opts_t(hbm_budget, slots, max_batch_windows, max_batch_tuples,
+                             max_window_tuples, cap)
+        h = ctypes.c_void_p()
+        check(lib.aj_server_create(ctx.handle, ctypes.byref(o), ctypes.byref(h)))
+        self.handle = h
+
+    def alloc(self, nR: int, nS: int, timeout_us: int = -1) -> aj_window_t:
+        w = aj_window_t()
+        check(lib.aj_server_window_alloc(self.handle, nR, nS, timeout_us, ctypes.byref(w)))
+        return w
+
+    def release(self, w: aj_window_t) -> None:
+        check(lib.aj_server_window_release(self.handle, ctypes.byref(w)))
+
+    def fill(self, w: aj_window_t, seed: int, key_domain: int, theta_s: float = 0.0) -> None:
+        check(lib.aj_server_window_fill(self.handle, ctypes.byref(w), seed, key_domain,
+                                        float(theta_s)))
+
+    def upload(self, w: aj_window_t, R: np.ndarray, S: np.ndarray) -> None:
+        R = np.ascontiguousarray(R, dtype=STREAM_DTYPE)
+        S = np.ascontiguousarray(S, dtype=STREAM_DTYPE)
+        assert len(R) == w.nR and len(S) == w.nS
+        check(lib.aj_server_window_upload(self.handle, ctypes.byref(w), R.ctypes.data,
+                                          S.ctypes.data))
+
+    def submit(self, w: aj_window_t, tag: int = 0, close_us: float = -1.0) -> None:
+        check(lib.aj_server_submit(self.handle, ctypes.byref(w), tag, float(close_us)))
+
+    def poll(self, max_results: int = 256, timeout_us: int = 0) -> list:
+        buf = (aj_server_result_t * max_results)()
+        n = ctypes.c_int()
+        check(lib.aj_server_poll(self.handle, buf, max_results, timeout_us, ctypes.byref(n)))
+        return [buf[i] for i in range(n.value)]
+
+    def drain(self) -> None:
+        check(lib.aj_server_drain(self.handle))
+
+    def now_us(self) -> float:
+        return lib.aj_server_now_us(self.handle)
+
+    def stats(self) -> dict:
+        st = aj_server_stats_t()
+        check(lib.aj_server_stats(self.handle, ctypes.byref(st)))
+        return {n: int(getattr(st, n)) for n, _ in aj_server_stats_t._fields_}
+
+    def close(self) -> None:
+        if self.handle:
+            lib.aj_server_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 class DeviceArray:

@@ -173,6 +173,55 @@ struct NpoDesc {
 int run_npo(const NpoDesc &d, hipStream_t st, const Marks *marks = nullptr, int m0 = 0);
 size_t npo_acc_words(int num_cu);  // u64 words of NpoDesc::acc
 
+// ---------------------------------------------------------------- batches
+// Workspace and launch geometry of batched window joins (windows.hip).
+struct BatchLayout {
+    size_t rb, re, sb, se, bits, cpre[2], hb[2], ob[2], pb, hdr, poff[2], tcnt, tbase, tinfo, tres, scal;
+    size_t bytes;
+};
+
+struct BatchPlan {  // sized for at most max_R/max_S tuples in windows, max_win windows
+    Ctx *ctx = nullptr;
+    uint64_t max_R = 0, max_S = 0, max_window = 0;
+    int max_win = 0;
+    int max_bits = 0;
+    uint32_t chunk = 0;
+    int cap = 8192;
+    uint32_t split = 1 << 16;
+    uint32_t target = 4096;
+    uint64_t max_chunks = 0, max_hist = 0, max_parts = 0;
+    BatchLayout L{};
+    DevBuf work, hist, scan_tmp, outR, outS;
+    size_t scan_bytes = 0;
+    // graph
+    bool use_graph = true;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t cap_stream = nullptr;
+    struct Key {
+        const void *R, *S, *bounds;
+        uint64_t nR, nS;
+        int nwin;
+        uint64_t *cnt, *sum;
+    } key{};
+    bool have_key = false;
+};
+
+
+// Sizes the plan (cap, split, target, chunk from opts, 0 = defaults) and
+// allocates its workspace.
+int batch_plan_setup(BatchPlan *p, Ctx *c, uint64_t max_R, uint64_t max_S, int max_windows,
+                     uint64_t max_window_tuples, const aj_batch_opts_t *opts);
+int batch_plan_init(BatchPlan *p);
+void batch_plan_release(BatchPlan *p);
+// Device block of the plan's window ranges: rb | re | sb | se, max_win u64
+// entries each (tuple indices into the R and S arrays, end exclusive).
+uint64_t *batch_ranges(BatchPlan *p);
+// Enqueues the joins of nwin windows whose ranges are in batch_ranges(p);
+// nR/nS = tuples in those windows.  Per-window count/checksum into cnt/sum.
+int enqueue_ranges(BatchPlan *p, const aj_stream_tuple_t *dR, uint64_t nR,
+                   const aj_stream_tuple_t *dS, uint64_t nS, int nwin, uint64_t *cnt,
+                   uint64_t *sum, hipStream_t st);
+
 // fill kernel (graph-safe replacement of hipMemsetAsync)
 int fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t st);
 int fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st);

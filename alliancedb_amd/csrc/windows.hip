@@ -78,44 +78,13 @@ __global__ void k_window_bits(const uint64_t *rb, const uint64_t *re, int nwin, 
     bits[w] = (uint32_t)b;
 }
 
-struct BatchLayout {
-    size_t rb, re, sb, se, bits, cpre[2], hb[2], ob[2], pb, hdr, poff[2], tcnt, tbase, tinfo, tres, scal;
-    size_t bytes;
-};
-
-struct BatchPlan {
-    Ctx *ctx = nullptr;
-    uint64_t max_R = 0, max_S = 0, max_window = 0;
-    int max_win = 0;
-    int max_bits = 0;
-    uint32_t chunk = 0;
-    int cap = 8192;
-    uint32_t split = 1 << 16;
-    uint32_t target = 4096;
-    uint64_t max_chunks = 0, max_hist = 0, max_parts = 0;
-    BatchLayout L{};
-    DevBuf work, hist, scan_tmp, outR, outS;
-    size_t scan_bytes = 0;
-    // graph
-    bool use_graph = true;
-    hipGraphExec_t exec = nullptr;
-    hipStream_t cap_stream = nullptr;
-    struct Key {
-        const void *R, *S, *bounds;
-        uint64_t nR, nS;
-        int nwin;
-        uint64_t *cnt, *sum;
-    } key{};
-    bool have_key = false;
-};
-
 static size_t lay(size_t &off, size_t bytes) {
     size_t o = off;
     off += (bytes + 255) / 256 * 256;
     return o;
 }
 
-static int plan_init(BatchPlan *p) {
+int batch_plan_init(BatchPlan *p) {
     const uint64_t W = (uint64_t)p->max_win;
     int b = 0;
     while (b < 13 && (p->max_window >> b) > p->target) ++b;
@@ -128,10 +97,11 @@ static int plan_init(BatchPlan *p) {
     p->max_parts = std::min<uint64_t>(W << b, 2 * p->max_R / p->target + W);
     BatchLayout &L = p->L;
     size_t off = 0;
-    L.rb = lay(off, 8 * W);
-    L.re = lay(off, 8 * W);
-    L.sb = lay(off, 8 * W);
-    L.se = lay(off, 8 * W);
+    // window ranges: rb | re | sb | se, W entries each, in one block
+    L.rb = lay(off, 32 * W);
+    L.re = L.rb + 8 * W;
+    L.sb = L.rb + 16 * W;
+    L.se = L.rb + 24 * W;
     L.bits = lay(off, 4 * W);
     for (int r = 0; r < 2; ++r) {
         L.cpre[r] = lay(off, 8 * (W + 1));
@@ -166,13 +136,25 @@ static int enqueue_batch(BatchPlan *p, const aj_stream_tuple_t *dR, uint64_t nR,
                          const aj_stream_tuple_t *dS, uint64_t nS, const uint32_t *bounds, int nwin,
                          uint64_t *cnt, uint64_t *sum, hipStream_t st) {
     const BatchLayout &L = p->L;
-    AJ_TRY(fill_u64(cnt, nwin, 0, st));
-    AJ_TRY(fill_u64(sum, nwin, 0, st));
-    AJ_TRY(fill_u64(W_<uint64_t>(p, L.scal), 8, 0, st));
     const unsigned g = (unsigned)((nwin + 1 + 255) / 256);
     hipLaunchKernelGGL(k_window_bounds, dim3(g), dim3(256), 0, st, (const uint32_t *)dR, nR,
                        (const uint32_t *)dS, nS, bounds, nwin, W_<uint64_t>(p, L.rb),
                        W_<uint64_t>(p, L.re), W_<uint64_t>(p, L.sb), W_<uint64_t>(p, L.se));
+    return enqueue_ranges(p, dR, nR, dS, nS, nwin, cnt, sum, st);
+}
+
+uint64_t *batch_ranges(BatchPlan *p) { return W_<uint64_t>(p, p->L.rb); }
+
+// Enqueues the joins of nwin windows whose tuple ranges (in tuples of dR and
+// dS) are already in the plan's range block.
+int enqueue_ranges(BatchPlan *p, const aj_stream_tuple_t *dR, uint64_t nR,
+                   const aj_stream_tuple_t *dS, uint64_t nS, int nwin, uint64_t *cnt,
+                   uint64_t *sum, hipStream_t st) {
+    const BatchLayout &L = p->L;
+    AJ_TRY(fill_u64(cnt, nwin, 0, st));
+    AJ_TRY(fill_u64(sum, nwin, 0, st));
+    AJ_TRY(fill_u64(W_<uint64_t>(p, L.scal), 8, 0, st));
+    const unsigned g = (unsigned)((nwin + 255) / 256);
     hipLaunchKernelGGL(k_window_bits, dim3(g), dim3(256), 0, st, W_<uint64_t>(p, L.rb),
                        W_<uint64_t>(p, L.re), nwin, W_<uint32_t>(p, L.bits), p->target,
                        p->max_bits);
@@ -248,6 +230,43 @@ static int check_batch_args(const BatchPlan *p, uint64_t nR, uint64_t nS, int nw
     return AJ_OK;
 }
 
+int batch_plan_setup(BatchPlan *p, Ctx *c, uint64_t max_R, uint64_t max_S, int max_windows,
+                     uint64_t max_window_tuples, const aj_batch_opts_t *opts) {
+    p->ctx = c;
+    p->max_R = std::max<uint64_t>(max_R, 1);
+    p->max_S = std::max<uint64_t>(max_S, 1);
+    p->max_win = max_windows;
+    p->max_window = max_window_tuples ? max_window_tuples : p->max_R;
+    p->cap = (opts && opts->cap) ? opts->cap : 8192;
+    p->split = join_task_split(opts ? opts->split : 0u, p->cap);
+    p->target = (opts && opts->target) ? opts->target : (uint32_t)p->cap / 2;
+    uint32_t chunk = (opts && opts->chunk) ? opts->chunk : 0;
+    if (!chunk) {
+        const uint64_t want = (max_R + max_S) / ((uint64_t)c->num_cu * 4);
+        chunk = (uint32_t)std::min<uint64_t>(
+            std::max<uint64_t>((want + WTILE - 1) / WTILE * WTILE, WTILE), 1u << 20);
+    }
+    p->chunk = (chunk + WTILE - 1) / WTILE * WTILE;
+    p->use_graph = !(opts && opts->no_graph);
+    if (p->cap != 4096 && p->cap != 8192 && p->cap != 8704 && p->cap != 12288) {
+        set_error(AJ_ERR_ARG, "cap must be 4096, 8192, 8704 or 12288");
+        return AJ_ERR_ARG;
+    }
+    return batch_plan_init(p);
+}
+
+void batch_plan_release(BatchPlan *p) {
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
+    p->exec = nullptr;
+    p->cap_stream = nullptr;
+    p->work.release();
+    p->hist.release();
+    p->scan_tmp.release();
+    p->outR.release();
+    p->outS.release();
+}
+
 // ---------------------------------------------------------------- generator
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -315,26 +334,7 @@ int aj_batch_plan_create(aj_ctx_t *ctx, uint64_t max_R, uint64_t max_S, int max_
     AJ_HIP(hipSetDevice(c->device));
     std::unique_ptr<aj_batch_plan> P(new aj_batch_plan());
     BatchPlan &p = P->p;
-    p.ctx = c;
-    p.max_R = std::max<uint64_t>(max_R, 1);
-    p.max_S = std::max<uint64_t>(max_S, 1);
-    p.max_win = max_windows;
-    p.max_window = max_window_tuples ? max_window_tuples : p.max_R;
-    p.cap = (opts && opts->cap) ? opts->cap : 8192;
-    p.split = join_task_split(opts ? opts->split : 0u, p.cap);
-    p.target = (opts && opts->target) ? opts->target : (uint32_t)p.cap / 2;
-    uint32_t chunk = (opts && opts->chunk) ? opts->chunk : 0;
-    if (!chunk) {
-        const uint64_t want = (max_R + max_S) / ((uint64_t)c->num_cu * 4);
-        chunk = (uint32_t)std::min<uint64_t>(std::max<uint64_t>((want + WTILE - 1) / WTILE * WTILE, WTILE), 1u << 20);
-    }
-    p.chunk = (chunk + WTILE - 1) / WTILE * WTILE;
-    p.use_graph = !(opts && opts->no_graph);
-    if (p.cap != 4096 && p.cap != 8192 && p.cap != 8704 && p.cap != 12288) {
-        set_error(AJ_ERR_ARG, "cap must be 4096, 8192, 8704 or 12288");
-        return AJ_ERR_ARG;
-    }
-    AJ_TRY(plan_init(&p));
+    AJ_TRY(batch_plan_setup(&p, c, max_R, max_S, max_windows, max_window_tuples, opts));
     AJ_HIP(hipStreamCreateWithFlags(&p.cap_stream, hipStreamNonBlocking));
     *out = P.release();
     return AJ_OK;
@@ -346,13 +346,7 @@ int aj_batch_plan_destroy(aj_batch_plan_t *plan) {
     {
         std::lock_guard<std::mutex> lk(p.ctx->mu);
         (void)hipSetDevice(p.ctx->device);
-        if (p.exec) (void)hipGraphExecDestroy(p.exec);
-        if (p.cap_stream) (void)hipStreamDestroy(p.cap_stream);
-        p.work.release();
-        p.hist.release();
-        p.scan_tmp.release();
-        p.outR.release();
-        p.outS.release();
+        batch_plan_release(&p);
     }
     delete plan;
     return AJ_OK;

@@ -207,6 +207,109 @@ int aj_gen_window_streams(aj_ctx_t *ctx, uint64_t seed, int nwin, const uint32_t
                           double rate_per_us, uint32_t dup, aj_stream_tuple_t *d_out,
                           uint64_t capacity, uint32_t *d_bounds, uint64_t *n_out);
 
+/*
+ * Device generator of n window tuples (12-byte aj_stream_tuple_t, ts = 0):
+ * keys uniform on [1, key_domain] (theta = 0) or Zipf(theta) ranks over
+ * key_domain drawn by rejection-inversion and mapped through a seeded
+ * permutation of 1..key_domain; payload = payload0 + row.  Value i depends
+ * only on (seed, i), so any slice can be regenerated.  Runs on `stream`
+ * (a hipStream_t); NULL = the context's stream, and then blocks until done.
+ * No counterpart in the reference (its windows come from createRelation,
+ * benchmark.cpp:38-126, on the host).
+ */
+int aj_gen_stream_tuples(aj_ctx_t *ctx, aj_stream_tuple_t *d_out, uint64_t n, uint64_t seed,
+                         uint32_t key_domain, double theta, uint32_t payload0, void *stream);
+
+/* ------------------------------------------------------------ serving loop */
+/*
+ * A request loop for window joins arriving from many streams (no counterpart
+ * in the reference, which joins one window per process run).  Each window's
+ * tuples live in device memory owned by the server ("window memory"); the
+ * window memory plus the workspaces of the batches in flight are bounded by
+ * an HBM budget.  Life of a request:
+ *   aj_server_window_alloc   admission: reserves window memory for |R|, |S|
+ *                            12-byte tuples; waits while the budget is taken
+ *   fill / upload            the window's tuples arrive (device generator or
+ *                            host copy, on the server's ingest stream)
+ *   aj_server_submit         the window closed: queue it for joining
+ *   (server)                 a dispatcher thread packs queued windows into a
+ *                            batch whenever one of `slots` HIP streams is free
+ *                            and launches the batched join on it, so batches
+ *                            overlap on the device; a completer thread waits
+ *                            for each batch, publishes per-window results and
+ *                            returns their window memory
+ *   aj_server_poll           results with close/start/done times
+ * Results are exactly those of a plain join of each window alone.
+ */
+typedef struct aj_server aj_server_t;
+
+typedef struct aj_server_opts_t {
+    uint64_t hbm_budget;        /* device bytes for windows + batch workspaces, 0 = 200e9 */
+    int slots;                  /* batches in flight (one HIP stream each), 0 = 3 */
+    int max_batch_windows;      /* windows per batch, 0 = 256 */
+    uint64_t max_batch_tuples;  /* tuples per side per batch, 0 = 2^27 */
+    uint64_t max_window_tuples; /* tuples per side per window, 0 = 2^24 */
+    int cap;                    /* LDS round as in aj_batch_opts_t, 0 = 8192 */
+} aj_server_opts_t;
+
+typedef struct aj_window_t {
+    uint64_t id;
+    aj_stream_tuple_t *dR; /* device: nR tuples inside the server's window memory */
+    aj_stream_tuple_t *dS;
+    uint64_t nR, nS;
+} aj_window_t;
+
+typedef struct aj_server_result_t {
+    uint64_t id, tag;
+    uint64_t count, checksum;
+    int status;            /* AJ_OK, or the error of the batch that carried it */
+    int batch_windows;     /* windows in that batch */
+    double close_us;       /* window close (aj_server_submit), server clock */
+    double start_us;       /* its batch was launched */
+    double done_us;        /* its result was on the host */
+} aj_server_result_t;
+
+typedef struct aj_server_stats_t {
+    uint64_t windows_done, batches, tuples_done;
+    uint64_t window_bytes;      /* window memory (budget minus workspaces) */
+    uint64_t window_bytes_used; /* currently reserved */
+    uint64_t window_bytes_peak;
+    uint64_t workspace_bytes;   /* all slots */
+    uint64_t alloc_waits;       /* window_alloc calls that waited for memory */
+    uint64_t pending;           /* submitted, not yet in a batch */
+    uint64_t in_flight;         /* batches on the device */
+} aj_server_stats_t;
+
+int aj_server_create(aj_ctx_t *ctx, const aj_server_opts_t *opts, aj_server_t **out);
+/* Waits for the batches in flight; windows still queued are dropped. */
+int aj_server_destroy(aj_server_t *srv);
+/* Reserves window memory; waits up to timeout_us (< 0: no limit, 0: no
+ * wait) for other windows to finish.  AJ_ERR_CAPACITY on timeout or if the
+ * window can never fit; AJ_ERR_ARG above max_window_tuples. */
+int aj_server_window_alloc(aj_server_t *srv, uint64_t nR, uint64_t nS, int64_t timeout_us,
+                           aj_window_t *win);
+/* Cancels a window that was allocated and not submitted. */
+int aj_server_window_release(aj_server_t *srv, const aj_window_t *win);
+/* R keys uniform on [1, key_domain]; S keys uniform (theta_s = 0) or
+ * Zipf(theta_s) over key_domain (aj_gen_stream_tuples with seeds
+ * seed and seed ^ 0x5DEECE66D).  Asynchronous on the ingest stream; batches
+ * wait for it on the device. */
+int aj_server_window_fill(aj_server_t *srv, const aj_window_t *win, uint64_t seed,
+                          uint32_t key_domain, double theta_s);
+/* Copies host tuples into the window (blocking). */
+int aj_server_window_upload(aj_server_t *srv, const aj_window_t *win,
+                            const aj_stream_tuple_t *hostR, const aj_stream_tuple_t *hostS);
+/* Queues the window for joining; close_us < 0 means now. */
+int aj_server_submit(aj_server_t *srv, const aj_window_t *win, uint64_t tag, double close_us);
+/* Up to max finished results into out[0..*n); waits up to timeout_us (< 0:
+ * no limit) for the first one. */
+int aj_server_poll(aj_server_t *srv, aj_server_result_t *out, int max, int64_t timeout_us,
+                   int *n);
+/* Waits until every submitted window has a result. */
+int aj_server_drain(aj_server_t *srv);
+double aj_server_now_us(aj_server_t *srv);
+int aj_server_stats(aj_server_t *srv, aj_server_stats_t *out);
+
 /* ------------------------------------------------------------ generators */
 /*
  * Seeded host generators (counter-based splitmix64 stream: value i is output
