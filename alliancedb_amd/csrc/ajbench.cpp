@@ -5,6 +5,10 @@
 //   ajbench -a PRO|NPO [-r |R|] [-s |S|] [-x r_seed] [-y s_seed]
 //           [-d key distribution: 0 unique/uniform, 2 zipf] [-z zipf theta]
 //           [-w window -e step -l interval] [-n nthreads] [-m] [-R repeats]
+//   ajbench -a SERVE [--load f] [--peak Mtps] [--seconds s] [--streams n]
+//           [--budget bytes] [--slots n] [--batch-windows n] [--clients n]
+//           [--verify n] [--median n] [--sigma s] [--cap-tuples n]
+//           [--domain n] [-z theta]      (served windows: ajbench_serve.cpp)
 //
 // Sizes: with -w/-e/-l, |R| = |S| = (window / interval) * step as in the
 // reference (benchmark.cpp:179-180); otherwise -r/-s (default 128M as in the
@@ -21,6 +25,25 @@
 #include <vector>
 
 #include "../../include/alliancedb/aj.h"
+
+struct ServeParams {
+    int streams = 64;
+    double load = 0.5;
+    double peak_mtps = 55000.0;
+    double seconds = 2.0;
+    double median = 262144, sigma = 1.0;
+    uint64_t cap_tuples = 1ull << 24;
+    uint32_t domain = 1u << 24;
+    double theta = 1.0;
+    double budget = 200e9;
+    int slots = 3;
+    int batch_windows = 256;
+    uint64_t batch_tuples = 1ull << 27;
+    int clients = 4;
+    int verify = 64;
+    uint64_t seed = 2024;
+};
+int run_serve(const ServeParams &P);
 
 namespace {
 
@@ -50,15 +73,36 @@ struct params {
 
 void usage(const char *prog) {
     printf("Usage: %s -a <PRO|NPO> [-r R] [-s S] [-x seed] [-y seed] [-d 0|2] [-z theta]\n"
-           "          [-w window -e step -l interval] [-n nthreads] [-m] [-R repeats]\n",
-           prog);
+           "          [-w window -e step -l interval] [-n nthreads] [-m] [-R repeats]\n"
+           "       %s -a SERVE [--load f] [--peak Mtps] [--seconds s] [--streams n]\n"
+           "          [--budget bytes] [--slots n] [--batch-windows n] [--clients n]\n"
+           "          [--verify n] [--median n] [--sigma s] [--cap-tuples n] [--domain n]\n",
+           prog, prog);
 }
+
+enum { OPT_LOAD = 1000, OPT_PEAK, OPT_SECONDS, OPT_STREAMS, OPT_BUDGET, OPT_SLOTS, OPT_BWIN,
+       OPT_CLIENTS, OPT_VERIFY, OPT_MEDIAN, OPT_SIGMA, OPT_CAPT, OPT_DOMAIN };
 
 }  // namespace
 
 int main(int argc, char **argv) {
     params p;
-    static option longopts[] = {{"algo", required_argument, 0, 'a'},
+    ServeParams sp;
+    bool serve = false, skew_given = false;
+    static option longopts[] = {
+                                {"load", required_argument, 0, OPT_LOAD},
+                                {"peak", required_argument, 0, OPT_PEAK},
+                                {"seconds", required_argument, 0, OPT_SECONDS},
+                                {"streams", required_argument, 0, OPT_STREAMS},
+                                {"budget", required_argument, 0, OPT_BUDGET},
+                                {"slots", required_argument, 0, OPT_SLOTS},
+                                {"batch-windows", required_argument, 0, OPT_BWIN},
+                                {"clients", required_argument, 0, OPT_CLIENTS},
+                                {"verify", required_argument, 0, OPT_VERIFY},
+                                {"median", required_argument, 0, OPT_MEDIAN},
+                                {"sigma", required_argument, 0, OPT_SIGMA},
+                                {"cap-tuples", required_argument, 0, OPT_CAPT},
+                                {"domain", required_argument, 0, OPT_DOMAIN},{"algo", required_argument, 0, 'a'},
                                 {"r-size", required_argument, 0, 'r'},
                                 {"s-size", required_argument, 0, 's'},
                                 {"r-seed", required_argument, 0, 'x'},
@@ -77,6 +121,10 @@ int main(int argc, char **argv) {
     while ((c = getopt_long(argc, argv, "a:r:s:x:y:d:z:w:e:l:n:mR:h", longopts, nullptr)) != -1) {
         switch (c) {
             case 'a': {
+                if (strcmp(optarg, "SERVE") == 0) {
+                    serve = true;
+                    break;
+                }
                 const algo_t *a = algos;
                 while (a->name && strcmp(a->name, optarg) != 0) ++a;
                 if (!a->name) {
@@ -92,7 +140,20 @@ int main(int argc, char **argv) {
             case 'x': p.r_seed = strtoull(optarg, nullptr, 10); break;
             case 'y': p.s_seed = strtoull(optarg, nullptr, 10); break;
             case 'd': p.key_distribution = atoi(optarg); break;
-            case 'z': p.skew = atof(optarg); break;
+            case 'z': p.skew = atof(optarg); skew_given = true; break;
+            case OPT_LOAD: sp.load = atof(optarg); break;
+            case OPT_PEAK: sp.peak_mtps = atof(optarg); break;
+            case OPT_SECONDS: sp.seconds = atof(optarg); break;
+            case OPT_STREAMS: sp.streams = atoi(optarg); break;
+            case OPT_BUDGET: sp.budget = atof(optarg); break;
+            case OPT_SLOTS: sp.slots = atoi(optarg); break;
+            case OPT_BWIN: sp.batch_windows = atoi(optarg); break;
+            case OPT_CLIENTS: sp.clients = atoi(optarg); break;
+            case OPT_VERIFY: sp.verify = atoi(optarg); break;
+            case OPT_MEDIAN: sp.median = atof(optarg); break;
+            case OPT_SIGMA: sp.sigma = atof(optarg); break;
+            case OPT_CAPT: sp.cap_tuples = strtoull(optarg, nullptr, 10); break;
+            case OPT_DOMAIN: sp.domain = (uint32_t)strtoul(optarg, nullptr, 10); break;
             case 'w': p.window = atol(optarg); break;
             case 'e': p.step = atol(optarg); break;
             case 'l': p.interval = atol(optarg); break;
@@ -102,6 +163,15 @@ int main(int argc, char **argv) {
             case 'h': usage(argv[0]); return EXIT_SUCCESS;
             default: usage(argv[0]); return EXIT_FAILURE;
         }
+    }
+    if (serve) {
+        if (skew_given) sp.theta = p.skew;
+        if (sp.streams < 1 || sp.clients < 1 || !(sp.load > 0) || !(sp.seconds > 0) ||
+            sp.cap_tuples < 1 || sp.domain < 1) {
+            fprintf(stderr, "[ERROR] bad SERVE parameters\n");
+            return EXIT_FAILURE;
+        }
+        return run_serve(sp);
     }
     if (p.window > 0 && p.step > 0 && p.interval > 0) {
         p.r_size = p.s_size = (uint64_t)(p.window / p.interval) * (uint64_t)p.step;

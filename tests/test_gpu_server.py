@@ -92,7 +92,7 @@ def test_server_admission_queues_under_budget(aj, oracle):
     specs = [(60000, 60000, 500 + i, 1 << 12, 0.5 if i % 3 == 0 else 0.0) for i in range(24)]
     with aj.Context(0) as ctx:
         # workspaces for 2 slots of 2^18 tuples, plus room for about 4 windows
-        probe = aj.Server(ctx, hbm_budget=64 << 30, slots=2, max_batch_windows=8,
+        probe = aj.Server(ctx, hbm_budget=8 << 30, slots=2, max_batch_windows=8,
                           max_batch_tuples=1 << 18, max_window_tuples=1 << 17)
         ws = probe.stats()["workspace_bytes"]
         probe.close()
