@@ -57,6 +57,7 @@ struct ProConfig {
     int split = 0;        // S tuples per join task (0 = cap; at most cap)
     int chunk = 0;        // tuples per partition work-group (0 = derived from sizes)
     int block = 512;      // lanes per partition work-group (256 or 512)
+    bool p2d = true;      // two passes: pass 2 from the 2-D pass-1 histogram when it fits
 };
 ProConfig choose_pro_config(uint64_t nR, uint64_t nS, const aj_join_opts_t *opts);
 
@@ -70,6 +71,7 @@ struct Ctx {
     DevBuf inR, inS;            // uploaded inputs (drop-in path)
     DevBuf p1R, p1S, p2R, p2S;  // partitioned copies
     DevBuf hist, scan_tmp, plan, seg, tasks, acc, pairs, table, misc;
+    DevBuf hist2d;              // per-chunk (d1, d2) counts of a two-pass PRO
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t evk[AJ_NPHASES + 1] = {};  // per-kernel timing (option kernel_timing)
 };
@@ -106,6 +108,10 @@ struct PassDesc {
     uint64_t max_chunks;      // upper bound of chunks over R+S (grid size)
     uint64_t max_hist;        // upper bound of histogram entries
     uint64_t max_parts;       // upper bound of partitions per relation
+    // one-segment 8-byte pass 1 of a two-pass PRO: also count (d1, d2) pairs
+    // per chunk into hist2d (chunks x 2^(max_bits + bits2)), see run_pass2_2d
+    uint32_t *hist2d = nullptr;
+    int bits2 = 0;
 };
 
 // Workspace carved out of ctx buffers by the pass (device pointers).
@@ -127,6 +133,26 @@ size_t pass_scan_tmp_bytes(uint64_t max_hist);
 int run_partition_pass(const PassDesc &d, const PassWork &w, hipStream_t st,
                        const Marks *marks = nullptr, int m0 = 0);
 
+// Pass 2 of a two-pass one-segment PRO driven by the 2-D histogram of pass 1
+// (no pass-2 histogram kernel).  Reads the pass-1 output and workspace.
+struct P2Desc {
+    const uint64_t *in[2];       // pass-1 output
+    uint64_t *out[2];            // pass-2 output
+    int b1, b2;
+    uint32_t *hist2d;            // as filled by pass 1
+    uint32_t *unit_counts;       // workspace: max_units * 2^b2
+    uint32_t *totals;            // workspace: 2 * 2^(b1+b2)
+    uint64_t *poff2[2];          // out: 2^(b1+b2)+1 final partition offsets per relation
+    uint32_t *ubase;             // workspace: 2 * 2^b1 + 1
+    const PassWork *w1;          // pass-1 workspace (offsets, header)
+    const uint32_t *hist_scan;   // pass-1 scanned histogram (PassWork::hist + max_hist)
+    uint64_t unit;               // pass-2 tuples per work-group (target)
+    uint64_t max_units;          // grid bound: (nR + nS) / unit + 2 * 2^b1
+    int block;
+};
+constexpr int P2D_MAX_BITS = 14;  // b1 + b2 for the 2-D histogram path (64 KB of LDS)
+int run_pass2_2d(const P2Desc &d, hipStream_t st, const Marks *marks = nullptr, int m0 = 0);
+
 // Join of partition pairs (R part_off, S part_off) into per-segment results.
 struct JoinDesc {
     const uint64_t *R, *S;          // partitioned tuples
@@ -147,13 +173,14 @@ struct JoinDesc {
     uint32_t *task_next;            // device scalar (zeroed by the caller)
     uint4 *task_info;               // workspace: 2 * join_max_tasks() entries
     ulonglong2 *task_res;           // workspace: join_max_tasks() partial results
+    uint64_t max_tasks;             // join_max_tasks() of this workspace
     void *scan_tmp;
     size_t scan_tmp_bytes;
     int num_cu;
 };
 size_t join_scan_tmp_bytes(uint64_t max_parts);
-// S tuples per join task actually used: at most one LDS round (`cap`), so a
-// task's S slice fits the work-group's registers; 0 = cap.
+// S tuples per join task (0 = 8 LDS rounds of `cap`).  A task probes its
+// slice in register-resident sub-slices of `cap` tuples.
 uint32_t join_task_split(uint32_t split, int cap);
 uint64_t join_max_tasks(uint64_t max_parts, uint64_t max_S, uint32_t split);
 // marks (nullable): m0 after task planning, m0+1 after the join kernel

@@ -91,7 +91,8 @@ void DevBuf::release() {
 
 // ---------------------------------------------------------------- options
 static std::atomic<int64_t> opt_materialize{0}, opt_radix_bits{0}, opt_passes{0}, opt_device{0},
-    opt_cap{0}, opt_split{0}, opt_kernel_timing{0}, opt_part_block{0}, opt_part_chunk{0};
+    opt_cap{0}, opt_split{0}, opt_kernel_timing{0}, opt_part_block{0}, opt_part_chunk{0},
+    opt_part_hist2d{1};
 
 bool kernel_timing_enabled() { return opt_kernel_timing.load() != 0; }
 
@@ -122,7 +123,7 @@ static void ctx_fini(Ctx *c) {
     (void)hipSetDevice(c->device);
     DevBuf *bufs[] = {&c->inR, &c->inS, &c->p1R, &c->p1S, &c->p2R, &c->p2S, &c->hist,
                       &c->scan_tmp, &c->plan, &c->seg, &c->tasks, &c->acc, &c->pairs,
-                      &c->table, &c->misc};
+                      &c->table, &c->misc, &c->hist2d};
     for (DevBuf *b : bufs) b->release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -161,6 +162,7 @@ static ProConfig config_from(uint64_t nR, uint64_t nS, const aj_join_opts_t *o) 
     ProConfig c = choose_pro_config(nR, nS, &eff);
     if (opt_part_block.load()) c.block = (int)opt_part_block.load();
     if (opt_part_chunk.load()) c.chunk = (int)opt_part_chunk.load();
+    c.p2d = opt_part_hist2d.load() != 0;
     return c;
 }
 
@@ -321,7 +323,8 @@ int aj_set_option(const char *name, int64_t value) {
                {"cap", &opt_cap, 0, 12288},             {"split", &opt_split, 0, 1ll << 30},
                {"kernel_timing", &opt_kernel_timing, 0, 1},
                {"part_block", &opt_part_block, 0, 512},
-               {"part_chunk", &opt_part_chunk, 0, 1ll << 24}};
+               {"part_chunk", &opt_part_chunk, 0, 1ll << 24},
+               {"part_hist2d", &opt_part_hist2d, 0, 1}};
     for (auto &e : tab) {
         if (strcmp(e.n, name) == 0) {
             if (value < e.lo || value > e.hi) {
@@ -364,6 +367,7 @@ int64_t aj_get_option(const char *name) {
     if (!strcmp(name, "kernel_timing")) return opt_kernel_timing.load();
     if (!strcmp(name, "part_block")) return opt_part_block.load();
     if (!strcmp(name, "part_chunk")) return opt_part_chunk.load();
+    if (!strcmp(name, "part_hist2d")) return opt_part_hist2d.load();
     if (!strcmp(name, "device_budget")) return (int64_t)g_dev_budget.load();
     if (!strcmp(name, "device_used")) return (int64_t)g_dev_used.load();
     return -1;

@@ -7,11 +7,11 @@
 // one partition to several threads (parallel_radix_join.cpp:1299-1321).
 //
 // CDNA4 form:
-//   k_task_count  tasks per partition = ceil(|S_p| / split) if |R_p|,|S_p| > 0;
-//                 split <= CAP, so a task's S slice fits the registers
+//   k_task_count  tasks per partition = ceil(|S_p| / split) if |R_p|,|S_p| > 0
+//                 (split: 8 LDS rounds of S by default)
 //   scan          task bases (flat exclusive scan)
-//   k_task_fill   task -> (R range, S slice, segment) records, so a worker
-//                 finds its task with one load
+//   k_task_fill   task -> (R range, S slice, segment) records, one thread per
+//                 task, so a worker finds its task with one load
 //   k_join        persistent work-groups take tasks round-robin (t = wg,
 //                 wg + grid, ...).  A task's first CAP tuples of R_p and its
 //                 whole S slice arrive in registers before the task starts
@@ -122,26 +122,22 @@ __global__ void k_task_count(JoinArgs a) {
     }
 }
 
+// One thread per task: its partition is the last p with task_base[p] <= t.
 __global__ void k_task_fill(JoinArgs a) {
-    for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < a.max_parts;
-         p += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t n = a.task_cnt[p];
-        if (!n) continue;
-        const uint32_t b = a.task_base[p];
+    const uint32_t total = a.task_base[a.max_parts];
+    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < total;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t p = last_le<uint32_t>(a.task_base, a.max_parts + 1, (uint32_t)t);
+        const uint32_t j = (uint32_t)t - a.task_base[p];
         const uint32_t seg =
             a.nseg == 1 ? 0u
                         : (uint32_t)last_le<uint32_t>(a.part_seg_base, (uint64_t)a.nseg + 1,
                                                       (uint32_t)p);
         const uint64_t r0 = a.roff[p], nr = a.roff[p + 1] - r0;
-        const uint64_t sb = a.soff[p], se = a.soff[p + 1];
-        for (uint32_t j = 0; j < n; ++j) {
-            const uint64_t s0 = sb + (uint64_t)j * a.split;
-            const uint64_t ns = min<uint64_t>(a.split, se - s0);
-            a.task_info[2 * (uint64_t)(b + j)] =
-                make_uint4((uint32_t)r0, (uint32_t)(r0 >> 32), (uint32_t)nr, seg);
-            a.task_info[2 * (uint64_t)(b + j) + 1] =
-                make_uint4((uint32_t)s0, (uint32_t)(s0 >> 32), (uint32_t)ns, 0u);
-        }
+        const uint64_t s0 = a.soff[p] + (uint64_t)j * a.split;
+        const uint64_t ns = min<uint64_t>(a.split, a.soff[p + 1] - s0);
+        a.task_info[2 * t] = make_uint4((uint32_t)r0, (uint32_t)(r0 >> 32), (uint32_t)nr, seg);
+        a.task_info[2 * t + 1] = make_uint4((uint32_t)s0, (uint32_t)(s0 >> 32), (uint32_t)ns, 0u);
     }
 }
 
@@ -212,20 +208,28 @@ __device__ __forceinline__ int build_regs(const uint64_t (&rv)[CAP / JB], uint32
     for (uint32_t i = threadIdx.x; i < nb + 2; i += JB) cnt[i] = 0;
     if (threadIdx.x == 0) bst[nb + 1] = 0;
     __syncthreads();
+    // items no lane of this wave holds are skipped (a wave-uniform branch);
+    // in a partly filled item, lanes without a tuple use the spare counter,
+    // cursor and slot
+    const uint32_t wave0 = __builtin_amdgcn_readfirstlane(tid);
     unroll<ITEMS>([&](auto q) {
-        const uint32_t h = bucket_of(key_of(rv[q]), lg);
-        atomicAdd(&cnt[tid + q * JB < m ? h : nb + 1], 1u);
+        if (wave0 + q * JB < m) {
+            const uint32_t h = bucket_of(key_of(rv[q]), lg);
+            atomicAdd(&cnt[tid + q * JB < m ? h : nb + 1], 1u);
+        }
     });
     __syncthreads();
     bucket_scan<JB>(cnt, bst, nb, scan_tmp, tid);
     uint32_t *cur = reinterpret_cast<uint32_t *>(bst);
     unroll<ITEMS>([&](auto q) {
-        const bool ok = tid + q * JB < m;
-        const uint32_t hb = bucket_of(key_of(rv[q]), lg);
-        const uint32_t h = ok ? hb : nb + 1;
-        const uint32_t sh = (h & 1u) * 16;
-        const uint32_t old = atomicAdd(&cur[h >> 1], 1u << sh);
-        tab[ok ? (old >> sh) & 0xffffu : (uint32_t)CAP] = rv[q];
+        if (wave0 + q * JB < m) {
+            const bool ok = tid + q * JB < m;
+            const uint32_t hb = bucket_of(key_of(rv[q]), lg);
+            const uint32_t h = ok ? hb : nb + 1;
+            const uint32_t sh = (h & 1u) * 16;
+            const uint32_t old = atomicAdd(&cur[h >> 1], 1u << sh);
+            tab[ok ? (old >> sh) & 0xffffu : (uint32_t)CAP] = rv[q];
+        }
     });
     __syncthreads();
     return lg;
@@ -244,6 +248,8 @@ __device__ __forceinline__ void probe_part(const uint64_t (&sv)[N], uint32_t ns,
                                            uint64_t &wpos, uint64_t cap, uint32_t tid) {
     if constexpr (G0 < N) {
         constexpr int G = N - G0 < PGROUP ? N - G0 : PGROUP;
+        // groups that no lane of this wave holds are skipped (wave-uniform)
+        if (__builtin_amdgcn_readfirstlane(tid) + (uint32_t)(Q0 + G0) * JB >= ns) return;
         uint32_t be[G];  // bucket cursor | end << 16
         unroll<G>([&](auto j) {
             const uint32_t h = bucket_of(key_of(sv[G0 + j]), lg);
@@ -329,7 +335,7 @@ __global__ __launch_bounds__(JB, 4) void k_join(JoinArgs a) {
     if (t >= total) return;
     TaskRec cur = decode_task(a.task_info[2 * t], a.task_info[2 * t + 1]);
     uint64_t sa[SA];
-    load_part<JB, 0, SA>(sa, a.S + cur.s0, cur.ns, opaque_tid());
+    load_part<JB, 0, SA>(sa, a.S + cur.s0, umin32(cur.ns, CAP), opaque_tid());
     for (;;) {
         const uint32_t tn = t + gridDim.x;
         const bool has_next = tn < total;
@@ -349,34 +355,44 @@ __global__ __launch_bounds__(JB, 4) void k_join(JoinArgs a) {
                 load_part<JB, 0, ITEMS>(rv, a.R + rb, m, opaque_tid());
                 lg = build_regs<JB, CAP>(rv, m, tab, bst, scan_tmp, opaque_tid());
             }
-            // second half of the S slice: in flight while the first half probes
-            uint64_t sb[SB];
-            load_part<JB, SA, SB>(sb, a.S + cur.s0, ns, opaque_tid());
-            uint64_t c = 0, x = 0, wpos = 0;
-            probe_part<JB, CAP, false, 0, SA>(sa, ns, tab, bst, lg, c, x, nullptr, wpos, 0,
-                                             opaque_tid());
-            probe_part<JB, CAP, false, SA, SB>(sb, ns, tab, bst, lg, c, x, nullptr, wpos, 0,
-                                              opaque_tid());
-            if (PAIRS) {
-                uint32_t total_c;
-                const uint32_t pre = block_exclusive_scan<JB>((uint32_t)c, scan_tmp, &total_c);
-                if (threadIdx.x == 0)
-                    s_base = atomicAdd((unsigned long long *)a.pair_cursor, (unsigned long long)total_c);
-                __syncthreads();
-                wpos = s_base + pre;
-                uint64_t dc = 0, dx = 0;
-                probe_part<JB, CAP, true, 0, SA>(sa, ns, tab, bst, lg, dc, dx, a.pairs, wpos,
-                                                a.pairs_cap, opaque_tid());
-                probe_part<JB, CAP, true, SA, SB>(sb, ns, tab, bst, lg, dc, dx, a.pairs, wpos,
-                                                 a.pairs_cap, opaque_tid());
+            // the S slice in sub-slices of CAP (one, unless a heavy partition
+            // gives the task more): the first half of the first sub-slice was
+            // loaded ahead; it is read again only after a later sub-slice
+            // replaced it
+            for (uint32_t so = 0; so < ns; so += CAP) {
+                const uint32_t sub = umin32(CAP, ns - so);
+                if (ns > (uint32_t)CAP && (so > 0 || rb > cur.r0))
+                    load_part<JB, 0, SA>(sa, a.S + cur.s0 + so, sub, opaque_tid());
+                // second half: in flight while the first half probes
+                uint64_t sb[SB];
+                load_part<JB, SA, SB>(sb, a.S + cur.s0 + so, sub, opaque_tid());
+                uint64_t c = 0, x = 0, wpos = 0;
+                probe_part<JB, CAP, false, 0, SA>(sa, sub, tab, bst, lg, c, x, nullptr, wpos, 0,
+                                                 opaque_tid());
+                probe_part<JB, CAP, false, SA, SB>(sb, sub, tab, bst, lg, c, x, nullptr, wpos, 0,
+                                                  opaque_tid());
+                if (PAIRS) {
+                    uint32_t total_c;
+                    const uint32_t pre = block_exclusive_scan<JB>((uint32_t)c, scan_tmp, &total_c);
+                    if (threadIdx.x == 0)
+                        s_base = atomicAdd((unsigned long long *)a.pair_cursor,
+                                           (unsigned long long)total_c);
+                    __syncthreads();
+                    wpos = s_base + pre;
+                    uint64_t dc = 0, dx = 0;
+                    probe_part<JB, CAP, true, 0, SA>(sa, sub, tab, bst, lg, dc, dx, a.pairs, wpos,
+                                                    a.pairs_cap, opaque_tid());
+                    probe_part<JB, CAP, true, SA, SB>(sb, sub, tab, bst, lg, dc, dx, a.pairs,
+                                                     wpos, a.pairs_cap, opaque_tid());
+                }
+                cnt += c;
+                sum += x;
             }
-            cnt += c;
-            sum += x;
             __syncthreads();  // the next round rebuilds tab/bst
         }
         const TaskRec nxt = decode_task(n0, n1);
         // in flight during the reduction and the next task's build
-        load_part<JB, 0, SA>(sa, a.S + nxt.s0, nxt.ns, opaque_tid());
+        load_part<JB, 0, SA>(sa, a.S + nxt.s0, umin32(nxt.ns, CAP), opaque_tid());
         // count and checksum reduced together: one barrier pair per task
         cnt = wave_sum(cnt);
         sum = wave_sum(sum);
@@ -412,7 +428,7 @@ size_t join_scan_tmp_bytes(uint64_t max_parts) {
 }
 
 uint32_t join_task_split(uint32_t split, int cap) {
-    return split ? std::min<uint32_t>(split, (uint32_t)cap) : (uint32_t)cap;
+    return split ? split : 8u * (uint32_t)cap;
 }
 
 uint64_t join_max_tasks(uint64_t max_parts, uint64_t max_S, uint32_t split) {
@@ -443,8 +459,7 @@ static int launch_join2(const JoinArgs &a, bool pairs, int num_cu, hipStream_t s
 }
 
 int run_join(const JoinDesc &d, hipStream_t st, const Marks *mk, int m0) {
-    if (d.split == 0 || d.split > (uint32_t)d.cap || d.max_parts >= (1ull << 31) ||
-        !d.task_info) {
+    if (d.split == 0 || d.max_parts >= (1ull << 31) || !d.task_info) {
         set_error(AJ_ERR_ARG, "bad join split/partition count (split %u, cap %d)", d.split,
                   d.cap);
         return AJ_ERR_ARG;
@@ -481,7 +496,8 @@ int run_join(const JoinDesc &d, hipStream_t st, const Marks *mk, int m0) {
         set_error(AJ_ERR_HIP, "task scan failed: %s", hipGetErrorString(e));
         return AJ_ERR_HIP;
     }
-    hipLaunchKernelGGL(k_task_fill, dim3(g), dim3(256), 0, st, a);
+    const unsigned gt = (unsigned)std::min<uint64_t>((d.max_tasks + 255) / 256, 16384);
+    hipLaunchKernelGGL(k_task_fill, dim3(gt), dim3(256), 0, st, a);
     if (mk) mk->mark(m0);
     const bool pairs = d.pairs != nullptr;
     int rc;

@@ -19,6 +19,9 @@
 //   k_scatter   one WG per chunk, tiles of 8192 tuples: rank by LDS atomic,
 //               LDS counting sort of the tile, then each digit's run is
 //               written with consecutive lanes on consecutive addresses
+//               (k_scatter_pipe: the same with fewer barriers and the next
+//               tile's loads in flight during the write, for passes of at
+//               most 10 bits)
 //   k_part_off  partition start offsets for the join
 //
 // Bytes per tuple: k_hist reads 8, k_scatter reads 8 and writes 8.
@@ -278,6 +281,334 @@ __global__ __launch_bounds__(PBLOCK) void k_scatter(PassArgs a) {
     }
 }
 
+// ---------------------------------------------------------------- k_scatter_pipe
+// The scatter for passes of at most PIPE_MAX_BITS radix bits, with four
+// barriers per tile:
+//   ranks     LDS atomic per tuple on its digit's counter
+//   scan      wave 0 alone scans the F counters (F/64 per lane) into the tile
+//             starts, the per-digit write bases (global cursor - tile start)
+//             and the advanced cursors
+//   place     tuples into LDS at start + rank (digit order)
+//   write     sorted tuple i goes to out[base[digit] + i]: consecutive lanes on
+//             consecutive addresses within a digit's run; the next tile's loads
+//             are issued before this step, so they are in flight meanwhile
+constexpr int PIPE_MAX_BITS = 10;
+
+// Tile loop of the pipelined scatter over input tuples [begin, end): digit
+// (key >> shift) & (F-1); `cur` holds the F output cursors and `cnt` is zero
+// (both visible to the whole work-group) on entry.
+template <int STRIDE, int PBLOCK>
+__device__ __forceinline__ void scatter_tiles(const void *in, uint64_t begin, uint64_t end,
+                                              uint64_t *out, int shift, uint32_t F,
+                                              uint64_t *buf, uint32_t *cnt, uint32_t *dst,
+                                              uint32_t *cur, uint32_t *s_len) {
+    constexpr int PTILE = PBLOCK * PITEMS;
+    constexpr int MAXPER = (1 << PIPE_MAX_BITS) / 64;
+    const uint32_t mask = F - 1;
+    uint64_t base = tile_start(begin, STRIDE);
+    if (base >= end) return;
+    uint64_t t[PITEMS];
+    bool v[PITEMS];
+    load_tile<PBLOCK>(in, STRIDE, base, begin, end, t, v);
+    for (;;) {
+        uint32_t rank[PITEMS];
+#pragma unroll
+        for (int j = 0; j < PITEMS; ++j) {
+            const uint32_t d = (key_of(t[j]) >> shift) & mask;
+            rank[j] = v[j] ? atomicAdd(&cnt[d], 1u) : 0u;
+        }
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            const uint32_t lane = threadIdx.x;
+            const uint32_t per = (F + 63) >> 6;
+            const uint32_t b = lane * per;
+            uint32_t c[MAXPER];
+            uint32_t sum = 0;
+#pragma unroll
+            for (int k = 0; k < MAXPER; ++k) {
+                c[k] = ((uint32_t)k < per && b + k < F) ? cnt[b + k] : 0u;
+                sum += c[k];
+            }
+            const uint32_t inc = wave_inclusive_scan(sum);
+            uint32_t pre = inc - sum;
+#pragma unroll
+            for (int k = 0; k < MAXPER; ++k) {
+                if ((uint32_t)k < per && b + k < F) {
+                    cnt[b + k] = pre;
+                    dst[b + k] = cur[b + k] - pre;
+                    cur[b + k] += c[k];
+                    pre += c[k];
+                }
+            }
+            if (lane == 63) *s_len = inc;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < PITEMS; ++j) {
+            if (v[j]) {
+                const uint32_t d = (key_of(t[j]) >> shift) & mask;
+                buf[cnt[d] + rank[j]] = t[j];
+            }
+        }
+        __syncthreads();
+        const uint32_t len = *s_len;
+        const uint64_t nbase = base + PTILE;
+        const bool more = nbase < end;
+        if (more) load_tile<PBLOCK>(in, STRIDE, nbase, begin, end, t, v);
+        for (uint32_t i = threadIdx.x; i < len; i += PBLOCK) {
+            const uint64_t u = buf[i];
+            const uint32_t d = (key_of(u) >> shift) & mask;
+            out[(uint64_t)(uint32_t)(dst[d] + i)] = u;
+        }
+        for (uint32_t d = threadIdx.x; d < F; d += PBLOCK) cnt[d] = 0;
+        __syncthreads();
+        if (!more) break;
+        base = nbase;
+    }
+}
+
+template <int STRIDE, int PBLOCK>
+__global__ __launch_bounds__(PBLOCK, 4) void k_scatter_pipe(PassArgs a) {
+    constexpr int PTILE = PBLOCK * PITEMS;
+    extern __shared__ uint64_t smem[];
+    __shared__ uint32_t s_len;
+    ChunkLoc L;
+    if (!locate_chunk(a, blockIdx.x, L)) return;
+    const uint32_t F = 1u << a.bits[L.seg];
+    uint64_t *buf = smem;                                        // PTILE tuples
+    uint32_t *cnt = reinterpret_cast<uint32_t *>(buf + PTILE);  // F: counts, then tile starts
+    uint32_t *dst = cnt + F;                                     // F: cursor - tile start
+    uint32_t *cur = dst + F;                                     // F: global write cursors
+    // S offsets follow every R tuple that lies in a segment
+    const uint32_t rel_base = L.rel ? (uint32_t)a.obase0[a.nseg] : 0u;
+    const uint64_t hb = (L.rel ? a.hbase1 : a.hbase0)[L.seg];
+    for (uint32_t d = threadIdx.x; d < F; d += PBLOCK) {
+        cur[d] = a.hist_scan[hb + d * L.nchunk + L.c] - rel_base;
+        cnt[d] = 0;
+    }
+    __syncthreads();
+    scatter_tiles<STRIDE, PBLOCK>(L.rel ? a.in1 : a.in0, L.begin, L.end, L.rel ? a.out1 : a.out0,
+                                  a.shift, F, buf, cnt, dst, cur, &s_len);
+}
+
+// ---------------------------------------------------------------- pass 2 from a 2-D histogram
+// For the one-segment PRO with two passes of b1 and b2 bits (F1*F2 <= 2^14):
+//   k_hist2d     pass-1 histogram kernel that also counts (d1, d2) pairs of
+//                its chunk in LDS and stores them in H, laid out
+//                [relation][d1][chunk][d2]; the 1-D pass-1 histogram is the
+//                row sums
+//   k_p2_units   pass-2 work units: pass-1 partition d1 of a relation is
+//                split at pass-1 chunk boundaries into about size/unit units
+//   k_h2_usum    per unit and d2: the count over the unit's chunks (G), added
+//                to the (d1, d2) totals
+//   k_h2_base    final partition offsets: exclusive scan of the totals
+//   k_scatter_p2 one work-group per unit: its pass-1 chunks' runs of
+//                partition d1 are contiguous in the pass-1 output; the
+//                cursors start at the final offsets plus the counts of the
+//                earlier units of the partition, so no pass-2 histogram is read
+// This removes the pass-2 histogram read (8 bytes per tuple).
+struct P2Args {
+    const uint64_t *in0, *in1;   // pass-1 output
+    uint64_t *out0, *out1;       // pass-2 output
+    int b1, b2;
+    uint32_t *H;                 // [rel][d1][chunk][d2] counts
+    uint32_t *G;                 // [unit][d2] counts over the unit's chunks
+    uint32_t *T;                 // [rel][d1][d2] totals (zeroed, then summed from G)
+    uint64_t *poff2_0, *poff2_1; // F1*F2+1 final offsets per relation
+    uint32_t *ubase;             // 2*F1+1 unit bases
+    const uint64_t *poff1_0, *poff1_1;  // pass-1 partition offsets
+    const uint32_t *hist_scan;   // pass-1 scanned histogram (column-major d1*C + c)
+    const uint64_t *hbase0, *hbase1;
+    const uint64_t *header;      // pass-1 header: [0] chunks of R, [1] chunks of S
+    const uint64_t *obase0;      // obase0[1] = R tuples (S offsets start there)
+    uint64_t unit;               // tuples per unit (target)
+};
+
+// offset of H[rel][d1][c][0]
+__device__ __forceinline__ uint64_t h_row(const P2Args &p, int rel, uint32_t d1, uint64_t c) {
+    const uint64_t C0 = p.header[0], C = rel ? p.header[1] : C0;
+    const uint64_t base = rel ? (C0 << p.b1) << p.b2 : 0;
+    return base + (((uint64_t)d1 * C + c) << p.b2);
+}
+
+// unit u -> relation, partition d1, its index j among the partition's U units
+// and its chunk range [c_lo, c_hi)
+struct Unit {
+    int rel;
+    uint32_t d1, j, U, idx;
+    uint64_t C, c_lo, c_hi;
+};
+__device__ __forceinline__ Unit unit_of(const P2Args &p, uint32_t u) {
+    const uint32_t F1 = 1u << p.b1;
+    Unit x;
+    x.idx = (uint32_t)last_le<uint32_t>(p.ubase, 2ull * F1 + 1, u);
+    x.rel = x.idx >= F1;
+    x.d1 = x.idx - (x.rel ? F1 : 0);
+    x.U = p.ubase[x.idx + 1] - p.ubase[x.idx];
+    x.j = u - p.ubase[x.idx];
+    x.C = x.rel ? p.header[1] : p.header[0];
+    x.c_lo = x.j * x.C / x.U;
+    x.c_hi = (x.j + 1) * x.C / x.U;
+    return x;
+}
+
+template <int PBLOCK>
+__global__ __launch_bounds__(PBLOCK) void k_hist2d(PassArgs a, P2Args p) {
+    constexpr int PTILE = PBLOCK * PITEMS;
+    extern __shared__ uint32_t h[];
+    ChunkLoc L;
+    if (!locate_chunk(a, blockIdx.x, L)) return;
+    const uint32_t F1 = 1u << p.b1, F2 = 1u << p.b2, FF = F1 * F2;
+    const uint32_t m1 = F1 - 1, m2 = F2 - 1;
+    for (uint32_t i = threadIdx.x; i < FF; i += PBLOCK) h[i] = 0;
+    __syncthreads();
+    const void *in = L.rel ? a.in1 : a.in0;
+    for (uint64_t base = tile_start(L.begin, 8); base < L.end; base += PTILE) {
+        uint64_t t[PITEMS];
+        bool v[PITEMS];
+        load_tile<PBLOCK>(in, 8, base, L.begin, L.end, t, v);
+#pragma unroll
+        for (int j = 0; j < PITEMS; ++j) {
+            const uint32_t k = key_of(t[j]);
+            if (v[j]) atomicAdd(&h[((k & m1) << p.b2) | ((k >> p.b1) & m2)], 1u);
+        }
+    }
+    __syncthreads();
+    // H[rel][d1][c][d2]: F1 runs of F2 counts
+    const uint64_t C0 = a.header[0];
+    const uint64_t hbase = L.rel ? (C0 << p.b1) << p.b2 : 0;
+    for (uint32_t i = threadIdx.x; i < FF; i += PBLOCK) {
+        const uint32_t d1 = i >> p.b2, d2 = i & m2;
+        p.H[hbase + (((uint64_t)d1 * L.nchunk + L.c) << p.b2) + d2] = h[i];
+    }
+    const uint64_t hb = (L.rel ? a.hbase1 : a.hbase0)[L.seg];
+    for (uint32_t d1 = threadIdx.x; d1 < F1; d1 += PBLOCK) {
+        uint32_t sum = 0;
+        for (uint32_t k = 0; k < F2; ++k) sum += h[(d1 << p.b2) | ((k + d1) & m2)];  // rotated: no bank conflicts
+        a.hist[hb + d1 * L.nchunk + L.c] = sum;
+    }
+}
+
+constexpr int H2B_BLOCK = 1024;
+__global__ __launch_bounds__(H2B_BLOCK) void k_p2_units(P2Args p) {
+    __shared__ uint32_t tmp[H2B_BLOCK / 64];
+    const uint32_t F1 = 1u << p.b1;
+    const uint64_t C[2] = {p.header[0], p.header[1]};
+    uint32_t run = 0;
+    for (uint32_t base = 0; base < 2 * F1; base += H2B_BLOCK) {
+        const uint32_t i = base + threadIdx.x;
+        uint32_t u = 0;
+        if (i < 2 * F1) {
+            const int rel = i >= F1;
+            const uint32_t d1 = i - (rel ? F1 : 0);
+            const uint64_t *po = rel ? p.poff1_1 : p.poff1_0;
+            const uint64_t size = po[d1 + 1] - po[d1];
+            if (size) {
+                const uint64_t want = (size + p.unit - 1) / p.unit;
+                u = (uint32_t)(want < C[rel] ? want : C[rel]);
+                if (u == 0) u = 1;
+            }
+        }
+        uint32_t total;
+        const uint32_t pre = block_exclusive_scan<H2B_BLOCK>(u, tmp, &total);
+        if (i < 2 * F1) p.ubase[i] = run + pre;
+        run += total;
+    }
+    if (threadIdx.x == 0) p.ubase[2 * F1] = run;
+}
+
+// G[u][d2] = sum of H[rel][d1][c][d2] over the unit's chunks: lanes split the
+// chunks (stride lanes = USUM_BLOCK / F2 rows per step), then an LDS reduce.
+constexpr int USUM_BLOCK = 512;
+__global__ __launch_bounds__(USUM_BLOCK) void k_h2_usum(P2Args p) {
+    __shared__ uint32_t red[USUM_BLOCK];
+    const uint32_t F1 = 1u << p.b1, F2 = 1u << p.b2;
+    const uint32_t u = blockIdx.x;
+    if (u >= p.ubase[2 * F1]) return;
+    const Unit x = unit_of(p, u);
+    const uint32_t *row0 = p.H + h_row(p, x.rel, x.d1, 0);
+    uint32_t *tot = p.T + ((uint64_t)x.idx << p.b2);  // [rel][d1] row
+    if (F2 >= USUM_BLOCK) {
+        for (uint32_t d2 = threadIdx.x; d2 < F2; d2 += USUM_BLOCK) {
+            uint32_t sum = 0;
+            for (uint64_t c = x.c_lo; c < x.c_hi; ++c) sum += row0[(c << p.b2) + d2];
+            p.G[((uint64_t)u << p.b2) + d2] = sum;
+            if (sum) atomicAdd(&tot[d2], sum);
+        }
+        return;
+    }
+    const uint32_t lanes = USUM_BLOCK >> p.b2;  // chunk rows per step
+    const uint32_t d2 = threadIdx.x & (F2 - 1), cl = threadIdx.x >> p.b2;
+    uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    uint64_t c = x.c_lo + cl;
+    for (; c + 3 * lanes < x.c_hi; c += 4 * lanes) {  // four independent loads in flight
+        s0 += row0[(c << p.b2) + d2];
+        s1 += row0[((c + lanes) << p.b2) + d2];
+        s2 += row0[((c + 2 * lanes) << p.b2) + d2];
+        s3 += row0[((c + 3 * lanes) << p.b2) + d2];
+    }
+    for (; c < x.c_hi; c += lanes) s0 += row0[(c << p.b2) + d2];
+    red[threadIdx.x] = s0 + s1 + s2 + s3;
+    __syncthreads();
+    if (threadIdx.x < F2) {
+        uint32_t sum = 0;
+        for (uint32_t k = 0; k < lanes; ++k) sum += red[(k << p.b2) + threadIdx.x];
+        p.G[((uint64_t)u << p.b2) + threadIdx.x] = sum;
+        if (sum) atomicAdd(&tot[threadIdx.x], sum);
+    }
+}
+
+// Final offsets per relation: exclusive scan of the (d1, d2) totals.
+__global__ __launch_bounds__(H2B_BLOCK) void k_h2_base(P2Args p) {
+    __shared__ uint64_t tmp[H2B_BLOCK / 64];
+    const uint32_t F1 = 1u << p.b1, F2 = 1u << p.b2, FF = F1 * F2;
+    const int rel = blockIdx.x;
+    uint64_t *po = rel ? p.poff2_1 : p.poff2_0;
+    uint64_t run = 0;
+    for (uint32_t base = 0; base < FF; base += H2B_BLOCK) {
+        const uint32_t i = base + threadIdx.x;
+        const uint64_t v = i < FF ? p.T[(rel ? (uint64_t)FF : 0) + i] : 0;
+        uint64_t total;
+        const uint64_t pre = block_exclusive_scan<H2B_BLOCK>(v, tmp, &total);
+        if (i < FF) po[i] = run + pre;
+        run += total;
+    }
+    if (threadIdx.x == 0) po[FF] = run;
+}
+
+template <int PBLOCK>
+__global__ __launch_bounds__(PBLOCK, 4) void k_scatter_p2(P2Args p) {
+    constexpr int PTILE = PBLOCK * PITEMS;
+    extern __shared__ uint64_t smem[];
+    __shared__ uint32_t s_len;
+    const uint32_t F1 = 1u << p.b1, F2 = 1u << p.b2;
+    const uint32_t u = blockIdx.x;
+    if (u >= p.ubase[2 * F1]) return;
+    const Unit x = unit_of(p, u);
+    // input: the runs of partition d1 written by pass-1 chunks [c_lo, c_hi)
+    const uint64_t hb = (x.rel ? p.hbase1 : p.hbase0)[0];
+    const uint64_t rel_base = x.rel ? p.obase0[1] : 0;
+    const uint64_t *po1 = x.rel ? p.poff1_1 : p.poff1_0;
+    const uint64_t begin = p.hist_scan[hb + x.d1 * x.C + x.c_lo] - rel_base;
+    const uint64_t end =
+        x.c_hi < x.C ? p.hist_scan[hb + x.d1 * x.C + x.c_hi] - rel_base : po1[x.d1 + 1];
+    uint64_t *buf = smem;
+    uint32_t *cnt = reinterpret_cast<uint32_t *>(buf + PTILE);
+    uint32_t *dst = cnt + F2;
+    uint32_t *cur = dst + F2;
+    const uint64_t *po2 = x.rel ? p.poff2_1 : p.poff2_0;
+    const uint32_t u0 = p.ubase[x.idx];  // first unit of this partition
+    for (uint32_t d = threadIdx.x; d < F2; d += PBLOCK) {
+        uint64_t c = po2[((uint64_t)x.d1 << p.b2) + d];
+        for (uint32_t k = u0; k < u; ++k) c += p.G[((uint64_t)k << p.b2) + d];
+        cur[d] = (uint32_t)c;
+        cnt[d] = 0;
+    }
+    __syncthreads();
+    scatter_tiles<8, PBLOCK>(x.rel ? (const void *)p.in1 : (const void *)p.in0, begin, end,
+                             x.rel ? p.out1 : p.out0, p.b1, F2, buf, cnt, dst, cur, &s_len);
+}
+
 // ---------------------------------------------------------------- k_part_off
 __global__ void k_part_off(PassArgs a) {
     const uint64_t parts = a.header[4];
@@ -318,11 +649,11 @@ size_t pass_scan_tmp_bytes(uint64_t max_hist) {
 // Raises a kernel's dynamic-LDS limit once to the largest size requested so far
 // (callers hold the context lock).
 static int lds_limit_set(const void *fn, size_t bytes) {
-    static const void *fns[16] = {};
-    static size_t set[16] = {};
+    static const void *fns[24] = {};
+    static size_t set[24] = {};
     int i = 0;
-    while (i < 16 && fns[i] && fns[i] != fn) ++i;
-    if (i == 16) i = 15;
+    while (i < 24 && fns[i] && fns[i] != fn) ++i;
+    if (i == 24) i = 23;
     if (!fns[i]) fns[i] = fn;
     if (bytes > set[i]) {
         AJ_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
@@ -340,10 +671,23 @@ static int launch_pass_kernels(const PassDesc &d, const PassWork &w, const PassA
         sizeof(uint64_t) * PBLOCK * PITEMS + sizeof(uint32_t) * (2 * F + 1);
     AJ_TRY(lds_limit_set((const void *)k_hist<STRIDE, PBLOCK>, lds_hist));
     AJ_TRY(lds_limit_set((const void *)k_scatter<STRIDE, PBLOCK>, lds_scatter));
+    const size_t lds_pipe = sizeof(uint64_t) * PBLOCK * PITEMS + sizeof(uint32_t) * 3 * F;
+    if (d.max_bits <= PIPE_MAX_BITS)
+        AJ_TRY(lds_limit_set((const void *)k_scatter_pipe<STRIDE, PBLOCK>, lds_pipe));
     if (mk) mk->mark(m0);
-    if (d.max_chunks)
+    if (d.max_chunks && d.hist2d) {
+        P2Args p{};
+        p.b1 = d.max_bits;
+        p.b2 = d.bits2;
+        p.H = d.hist2d;
+        const size_t lds2 = sizeof(uint32_t) << (d.max_bits + d.bits2);
+        AJ_TRY(lds_limit_set((const void *)k_hist2d<PBLOCK>, lds2));
+        hipLaunchKernelGGL((k_hist2d<PBLOCK>), dim3((unsigned)d.max_chunks), dim3(PBLOCK), lds2,
+                           st, a, p);
+    } else if (d.max_chunks) {
         hipLaunchKernelGGL((k_hist<STRIDE, PBLOCK>), dim3((unsigned)d.max_chunks), dim3(PBLOCK),
                            lds_hist, st, a);
+    }
     if (mk) mk->mark(m0 + 1);
     size_t tmp_bytes = w.scan_tmp_bytes;
     if (d.max_hist) {
@@ -355,9 +699,14 @@ static int launch_pass_kernels(const PassDesc &d, const PassWork &w, const PassA
         }
     }
     if (mk) mk->mark(m0 + 2);
-    if (d.max_chunks)
-        hipLaunchKernelGGL((k_scatter<STRIDE, PBLOCK>), dim3((unsigned)d.max_chunks),
-                           dim3(PBLOCK), lds_scatter, st, a);
+    if (d.max_chunks) {
+        if (d.max_bits <= PIPE_MAX_BITS)
+            hipLaunchKernelGGL((k_scatter_pipe<STRIDE, PBLOCK>), dim3((unsigned)d.max_chunks),
+                               dim3(PBLOCK), lds_pipe, st, a);
+        else
+            hipLaunchKernelGGL((k_scatter<STRIDE, PBLOCK>), dim3((unsigned)d.max_chunks),
+                               dim3(PBLOCK), lds_scatter, st, a);
+    }
     if (mk) mk->mark(m0 + 3);
     return AJ_OK;
 }
@@ -379,6 +728,13 @@ int run_partition_pass(const PassDesc &d, const PassWork &w, hipStream_t st, con
     }
     if (d.in_stride != 8 && d.in_stride != 12) {
         set_error(AJ_ERR_ARG, "tuple stride must be 8 or 12 bytes");
+        return AJ_ERR_ARG;
+    }
+    if (d.hist2d && (d.in_stride != 8 || d.nseg != 1 || d.bits2 < 1 ||
+                     d.max_bits + d.bits2 > P2D_MAX_BITS || d.max_bits > PIPE_MAX_BITS ||
+                     d.bits2 > PIPE_MAX_BITS)) {
+        set_error(AJ_ERR_ARG, "2-D pass-1 histogram needs one 8-byte segment and at most %d bits",
+                  P2D_MAX_BITS);
         return AJ_ERR_ARG;
     }
     if (d.n_total[0] + d.n_total[1] >= (1ull << 32)) {
@@ -432,6 +788,61 @@ int run_partition_pass(const PassDesc &d, const PassWork &w, hipStream_t st, con
     AJ_TRY(rc);
     const unsigned pgrid = (unsigned)std::min<uint64_t>((d.max_parts + 256) / 256, 4096);
     hipLaunchKernelGGL(k_part_off, dim3(pgrid), dim3(256), 0, st, a);
+    AJ_HIP(hipGetLastError());
+    return AJ_OK;
+}
+
+template <int PBLOCK>
+static int launch_scatter_p2(const P2Args &p, uint64_t max_units, hipStream_t st) {
+    const size_t lds = sizeof(uint64_t) * PBLOCK * PITEMS + sizeof(uint32_t) * 3 * (1u << p.b2);
+    AJ_TRY(lds_limit_set((const void *)k_scatter_p2<PBLOCK>, lds));
+    hipLaunchKernelGGL((k_scatter_p2<PBLOCK>), dim3((unsigned)max_units), dim3(PBLOCK), lds, st, p);
+    return AJ_OK;
+}
+
+int run_pass2_2d(const P2Desc &d, hipStream_t st, const Marks *mk, int m0) {
+    if (d.b1 < 1 || d.b2 < 1 || d.b1 + d.b2 > P2D_MAX_BITS || d.b2 > PIPE_MAX_BITS || !d.w1 ||
+        d.unit == 0 || (d.block != 256 && d.block != 512)) {
+        set_error(AJ_ERR_ARG, "run_pass2_2d: bad geometry");
+        return AJ_ERR_ARG;
+    }
+    P2Args p{};
+    p.in0 = d.in[0];
+    p.in1 = d.in[1];
+    p.out0 = d.out[0];
+    p.out1 = d.out[1];
+    p.b1 = d.b1;
+    p.b2 = d.b2;
+    p.H = d.hist2d;
+    p.G = d.unit_counts;
+    p.T = d.totals;
+    p.poff2_0 = d.poff2[0];
+    p.poff2_1 = d.poff2[1];
+    p.ubase = d.ubase;
+    p.poff1_0 = d.w1->part_off[0];
+    p.poff1_1 = d.w1->part_off[1];
+    p.hbase0 = d.w1->seg_hist_base[0];
+    p.hbase1 = d.w1->seg_hist_base[1];
+    p.header = d.w1->header;
+    p.obase0 = d.w1->seg_out_base[0];
+    p.unit = d.unit;
+    p.hist_scan = d.hist_scan;
+    if (!p.hist_scan || !p.H || !p.G || !p.T || !p.ubase) {
+        set_error(AJ_ERR_ARG, "run_pass2_2d: missing workspace");
+        return AJ_ERR_ARG;
+    }
+    if (mk) mk->mark(m0);
+    AJ_TRY(fill_u32(p.T, 2ull << (d.b1 + d.b2), 0u, st));
+    hipLaunchKernelGGL(k_p2_units, dim3(1), dim3(H2B_BLOCK), 0, st, p);
+    hipLaunchKernelGGL(k_h2_usum, dim3((unsigned)d.max_units), dim3(USUM_BLOCK), 0, st, p);
+    hipLaunchKernelGGL(k_h2_base, dim3(2), dim3(H2B_BLOCK), 0, st, p);
+    if (mk) {
+        mk->mark(m0 + 1);
+        mk->mark(m0 + 2);
+    }
+    AJ_TRY(d.block == 512 ? launch_scatter_p2<512>(p, d.max_units, st)
+                          : launch_scatter_p2<256>(p, d.max_units, st));
+    if (mk) mk->mark(m0 + 3);
     AJ_HIP(hipGetLastError());
     return AJ_OK;
 }

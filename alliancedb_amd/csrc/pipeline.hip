@@ -54,8 +54,10 @@ __global__ void k_setup_nopart(uint64_t *roff, uint64_t *soff, uint64_t nR, uint
     for (int i = threadIdx.x; i < nscalars; i += blockDim.x) scalars[i] = 0;
 }
 
+// Tuples per partition work-group: about two work-groups per CU (one wave of
+// work-groups; measured best at 2^27 + 2^27 tuples).
 static uint32_t pick_chunk(uint64_t n_total, int num_cu) {
-    uint64_t want = n_total / ((uint64_t)num_cu * 4);
+    uint64_t want = n_total / ((uint64_t)num_cu * 2);
     want = (want + PART_TILE - 1) / PART_TILE * PART_TILE;
     want = std::max<uint64_t>(want, PART_TILE);
     want = std::min<uint64_t>(want, 1ull << 20);
@@ -105,6 +107,10 @@ struct ProPlan {
     size_t o_cpre[2], o_hb[2], o_ob[2], o_pb, o_hdr, o_poff1[2];
     size_t o_cpre2[2], o_hb2[2], o_ob2[2], o_pb2, o_hdr2, o_poff2[2];
     size_t o_tcnt, o_tbase, o_tinfo, o_tres, o_psb, o_scal;
+    bool p2d;                 // pass 2 from the 2-D pass-1 histogram
+    size_t o_tot, o_ub;
+    uint64_t max_units, unit, max_tasks;
+    size_t o_h2tot;
     size_t bytes;
     size_t scan_bytes;
 };
@@ -153,10 +159,20 @@ static ProPlan plan_pro(uint64_t nR, uint64_t nS, const ProConfig &c, int num_cu
     const uint64_t jparts = p.parts_final;
     p.o_tcnt = L.add<uint32_t>(jparts + 1);
     p.o_tbase = L.add<uint32_t>(jparts + 1);
-    p.o_tinfo = L.add<uint4>(2 * join_max_tasks(jparts, nS, (uint32_t)c.split));
-    p.o_tres = L.add<ulonglong2>(join_max_tasks(jparts, nS, (uint32_t)c.split));
+    p.max_tasks = join_max_tasks(jparts, nS, (uint32_t)c.split);
+    p.o_tinfo = L.add<uint4>(2 * p.max_tasks);
+    p.o_tres = L.add<ulonglong2>(p.max_tasks);
     p.o_psb = L.add<uint32_t>(2);
     p.o_scal = L.add<uint64_t>(8);
+    p.p2d = p.b2 > 0 && p.b1 + p.b2 <= P2D_MAX_BITS && c.p2d;
+    if (p.p2d) {
+        // pass-2 units of about 16 per CU (at least one tile, at most a pass-1 chunk)
+        p.unit = std::max<uint64_t>(PART_TILE, std::min<uint64_t>(p.chunk1, (nR + nS) / ((uint64_t)num_cu * 16)));
+        p.max_units = (nR + nS) / p.unit + 2 * F1 + 1;
+        p.o_tot = L.add<uint32_t>(p.max_units * F2);
+        p.o_ub = L.add<uint32_t>(2 * F1 + 2);
+        p.o_h2tot = L.add<uint32_t>(2 * F1 * F2);
+    }
     p.bytes = L.off;
     p.scan_bytes = std::max({pass_scan_tmp_bytes(std::max<uint64_t>(p.max_hist1, 1)),
                              pass_scan_tmp_bytes(std::max<uint64_t>(p.max_hist2, 1)),
@@ -197,6 +213,7 @@ int pro_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
             AJ_TRY(ctx->p2R.ensure(nR * 8 + 256));
             AJ_TRY(ctx->p2S.ensure(nS * 8 + 256));
         }
+        if (p.p2d) AJ_TRY(ctx->hist2d.ensure(sizeof(uint32_t) * p.max_chunks1 << (p.b1 + p.b2)));
     }
     const DevBuf &B = ctx->plan;
     uint64_t *scal = at<uint64_t>(B, p.o_scal);
@@ -239,6 +256,10 @@ int pro_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
         d1.max_bits = p.b1;
         d1.chunk = p.chunk1;
         d1.block = cfg.block;
+        if (p.p2d) {
+            d1.hist2d = ctx->hist2d.as<uint32_t>();
+            d1.bits2 = p.b2;
+        }
         d1.n_total[0] = nR;
         d1.n_total[1] = nS;
         d1.max_chunks = p.max_chunks1;
@@ -261,7 +282,30 @@ int pro_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
         jS = ctx->p1S.as<uint64_t>();
         roff = w1.part_off[0];
         soff = w1.part_off[1];
-        if (p.b2) {
+        if (p.p2d) {
+            P2Desc q;
+            q.in[0] = jR;
+            q.in[1] = jS;
+            q.out[0] = ctx->p2R.as<uint64_t>();
+            q.out[1] = ctx->p2S.as<uint64_t>();
+            q.b1 = p.b1;
+            q.b2 = p.b2;
+            q.hist2d = ctx->hist2d.as<uint32_t>();
+            q.unit_counts = at<uint32_t>(B, p.o_tot);
+            q.totals = at<uint32_t>(B, p.o_h2tot);
+            for (int r = 0; r < 2; ++r) q.poff2[r] = at<uint64_t>(B, p.o_poff2[r]);
+            q.ubase = at<uint32_t>(B, p.o_ub);
+            q.w1 = &w1;
+            q.hist_scan = w1.hist + p.max_hist1;
+            q.unit = p.unit;
+            q.max_units = p.max_units;
+            q.block = cfg.block;
+            AJ_TRY(run_pass2_2d(q, st, mkp, 5));
+            jR = ctx->p2R.as<uint64_t>();
+            jS = ctx->p2S.as<uint64_t>();
+            roff = q.poff2[0];
+            soff = q.poff2[1];
+        } else if (p.b2) {
             AJ_TRY(fill_u32(at<uint32_t>(B, p.o_bits2), 1ull << p.b1, (uint32_t)p.b2, st));
             PassDesc d2 = d1;
             d2.in[0] = jR;
@@ -321,6 +365,7 @@ int pro_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
     j.task_next = reinterpret_cast<uint32_t *>(scal + 3);
     j.task_info = at<uint4>(B, p.o_tinfo);
     j.task_res = at<ulonglong2>(B, p.o_tres);
+    j.max_tasks = p.max_tasks;
     j.scan_tmp = ctx->scan_tmp.p;
     j.scan_tmp_bytes = p.scan_bytes;
     j.num_cu = ctx->num_cu;

@@ -464,7 +464,7 @@ int aj_server_create(aj_ctx_t *ctx, const aj_server_opts_t *opts, aj_server_t **
     if (!s.o.max_batch_windows) s.o.max_batch_windows = 256;
     if (!s.o.max_batch_tuples) s.o.max_batch_tuples = 1ull << 27;
     if (!s.o.max_window_tuples) s.o.max_window_tuples = 1ull << 24;
-    if (!s.o.cap) s.o.cap = 8192;
+    if (!s.o.cap) s.o.cap = 8704;
     if (s.o.slots < 1 || s.o.slots > 16 || s.o.max_batch_windows < 1 ||
         s.o.max_batch_windows > 65536 || s.o.max_window_tuples > s.o.max_batch_tuples ||
         2 * s.o.max_batch_tuples >= (1ull << 32)) {

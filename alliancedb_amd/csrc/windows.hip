@@ -213,6 +213,7 @@ int enqueue_ranges(BatchPlan *p, const aj_stream_tuple_t *dR, uint64_t nR,
     j.task_next = reinterpret_cast<uint32_t *>(W_<uint64_t>(p, L.scal) + 3);
     j.task_info = W_<uint4>(p, L.tinfo);
     j.task_res = W_<ulonglong2>(p, L.tres);
+    j.max_tasks = join_max_tasks(p->max_parts, p->max_S, p->split);
     j.scan_tmp = p->scan_tmp.p;
     j.scan_tmp_bytes = p->scan_bytes;
     j.num_cu = p->ctx->num_cu;
@@ -237,9 +238,13 @@ int batch_plan_setup(BatchPlan *p, Ctx *c, uint64_t max_R, uint64_t max_S, int m
     p->max_S = std::max<uint64_t>(max_S, 1);
     p->max_win = max_windows;
     p->max_window = max_window_tuples ? max_window_tuples : p->max_R;
-    p->cap = (opts && opts->cap) ? opts->cap : 8192;
+    p->cap = (opts && opts->cap) ? opts->cap : 8704;
     p->split = join_task_split(opts ? opts->split : 0u, p->cap);
-    p->target = (opts && opts->target) ? opts->target : (uint32_t)p->cap / 2;
+    // default target: the largest power of two that fits one LDS round (the
+    // rest of the round absorbs the spread of partition sizes)
+    uint32_t t = 1;
+    while (2 * t <= (uint32_t)p->cap) t *= 2;
+    p->target = (opts && opts->target) ? opts->target : t;
     uint32_t chunk = (opts && opts->chunk) ? opts->chunk : 0;
     if (!chunk) {
         const uint64_t want = (max_R + max_S) / ((uint64_t)c->num_cu * 4);

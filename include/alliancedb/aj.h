@@ -93,6 +93,8 @@ const aj_result_t *aj_result_ext(const result_t *r);
  *                        to cap so that a task's S slice fits the registers)
  *   "part_block"   0, 256, 512  lanes per partition work-group (0 = 512)
  *   "part_chunk"   multiple of 8192  tuples per partition work-group (0 = derived)
+ *   "part_hist2d"  0/1   two-pass PRO: pass 2 from a (d1, d2) histogram taken in
+ *                        pass 1 instead of its own histogram pass (default 1)
  *   "device_budget" bytes of device memory the library may hold (0 = unlimited)
  */
 int aj_set_option(const char *name, int64_t value);
@@ -177,9 +179,10 @@ int aj_join_dev(aj_ctx_t *ctx, const tuple_t *dR, uint64_t nR, const tuple_t *dS
  * sizes; the kernel sequence is captured once into a hipGraph and replayed.
  */
 typedef struct aj_batch_opts_t {
-    int cap;          /* R tuples per LDS round (4096/8192/8704/12288), 0 = 8192 */
+    int cap;          /* R tuples per LDS round (4096/8192/8704/12288), 0 = 8704 */
     uint32_t split;   /* S tuples per join task, 0 = cap, at most cap */
-    uint32_t target;  /* target R tuples per partition, 0 = cap/2 */
+    uint32_t target;  /* largest R tuples per partition aimed at, 0 = the largest
+                         power of two <= cap */
     uint32_t chunk;   /* tuples per partition work-group, 0 = derived */
     int no_graph;     /* 1 = enqueue kernels directly instead of a graph */
 } aj_batch_opts_t;
@@ -249,7 +252,7 @@ typedef struct aj_server_opts_t {
     int max_batch_windows;      /* windows per batch, 0 = 256 */
     uint64_t max_batch_tuples;  /* tuples per side per batch, 0 = 2^27 */
     uint64_t max_window_tuples; /* tuples per side per window, 0 = 2^24 */
-    int cap;                    /* LDS round as in aj_batch_opts_t, 0 = 8192 */
+    int cap;                    /* LDS round as in aj_batch_opts_t, 0 = 8704 */
 } aj_server_opts_t;
 
 typedef struct aj_window_t {
