@@ -39,11 +39,14 @@ HBM_PEAK_GBS = 8000.0
 PRO_BYTES_PER_TUPLE = 32  # histogram read 8 + scatter read 8 + write 8 + build/probe read 8
 NPO_BYTES_PER_TUPLE = 16  # R read 8 + slot write 8 / S read 8 + slot read 8
 WINDOW_BYTES_PER_TUPLE = 40  # 12 hist read + 12 scatter read + 8 write + 8 build/probe read
-# algorithmic bytes per input tuple of each kernel (8-byte tuples)
-KERNEL_BYTES = {"hist1": 8, "scatter1": 16, "hist2": 8, "scatter2": 16, "join": 8,
+# algorithmic bytes per input tuple of each kernel (8-byte tuples).  With the
+# default two-pass geometry pass 1's histogram kernel (k_hist2d) also counts the
+# pass-2 digits, and the "hist2" phase is only the pass-2 unit planning.
+KERNEL_BYTES = {"hist1": 8, "scatter1": 16, "hist2": 0, "scatter2": 16, "join": 8,
                 "npo_build": 16, "npo_probe": 16}
-KERNEL_NAMES = {"hist1": "k_hist (pass 1)", "scatter1": "k_scatter (pass 1)",
-                "hist2": "k_hist (pass 2)", "scatter2": "k_scatter (pass 2)",
+KERNEL_NAMES = {"hist1": "k_hist2d (pass 1)", "scatter1": "k_scatter_pipe (pass 1)",
+                "hist2": "pass-2 planning (k_p2_units, k_h2_usum, k_h2_base)",
+                "scatter2": "k_scatter_p2 (pass 2)",
                 "join": "k_join", "npo_build": "k_npo_build", "npo_probe": "k_npo_probe"}
 
 
