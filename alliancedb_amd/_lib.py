@@ -97,6 +97,15 @@ class aj_server_stats_t(ctypes.Structure):
         "window_bytes_peak", "workspace_bytes", "alloc_waits", "pending", "in_flight")]
 
 
+class aj_tune_entry_t(ctypes.Structure):
+    _fields_ = [("log2_r", ctypes.c_int), ("log2_s", ctypes.c_int), ("radix_bits", ctypes.c_int),
+                ("passes", ctypes.c_int), ("cap", ctypes.c_int), ("part_block", ctypes.c_int),
+                ("part_chunk", ctypes.c_uint32), ("device_ms", ctypes.c_float)]
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 class AjError(RuntimeError):
     def __init__(self, status: int, msg: str):
         super().__init__(f"aj status {status}: {msg}")
@@ -150,6 +159,13 @@ def _load() -> ctypes.CDLL:
         "aj_server_drain": (c.c_int, [P]),
         "aj_server_now_us": (c.c_double, [P]),
         "aj_server_stats": (c.c_int, [P, c.POINTER(aj_server_stats_t)]),
+        "aj_tune_pro": (c.c_int, [P, P, c.c_uint64, P, c.c_uint64, c.c_int,
+                                  c.POINTER(aj_tune_entry_t)]),
+        "aj_tune_put": (c.c_int, [c.POINTER(aj_tune_entry_t)]),
+        "aj_tune_get": (c.c_int, [c.c_uint64, c.c_uint64, c.POINTER(aj_tune_entry_t)]),
+        "aj_tune_clear": (c.c_int, []),
+        "aj_tune_save": (c.c_int, [c.c_char_p]),
+        "aj_tune_load": (c.c_int, [c.c_char_p]),
         "aj_gen_unique": (c.c_int, [P, c.c_uint64, c.c_uint64]),
         "aj_gen_uniform": (c.c_int, [P, c.c_uint64, c.c_uint32, c.c_uint32, c.c_uint64]),
         "aj_gen_zipf": (c.c_int, [P, c.c_uint64, c.c_uint32, c.c_double, c.c_uint64]),
@@ -303,6 +319,37 @@ def gen_window_streams(ctx: "Context", seed: int, sizes: np.ndarray, rate_per_us
                                     float(rate_per_us), dup, d_out, capacity, d_bounds,
                                     ctypes.byref(n)))
     return int(n.value)
+
+
+def tune_pro(ctx: "Context", dR: int, nR: int, dS: int, nS: int, reps: int = 3) -> dict:
+    """Autotunes the PRO geometry for the size bucket of (nR, nS) on device data."""
+    e = aj_tune_entry_t()
+    check(lib.aj_tune_pro(ctx.handle, dR, nR, dS, nS, reps, ctypes.byref(e)))
+    return e.as_dict()
+
+
+def tune_get(nR: int, nS: int):
+    e = aj_tune_entry_t()
+    if lib.aj_tune_get(nR, nS, ctypes.byref(e)) != 0:
+        return None
+    return e.as_dict()
+
+
+def tune_put(**fields) -> None:
+    e = aj_tune_entry_t(**{k: v for k, v in fields.items()})
+    check(lib.aj_tune_put(ctypes.byref(e)))
+
+
+def tune_save(path: str) -> None:
+    check(lib.aj_tune_save(path.encode()))
+
+
+def tune_load(path: str) -> None:
+    check(lib.aj_tune_load(path.encode()))
+
+
+def tune_clear() -> None:
+    check(lib.aj_tune_clear())
 
 
 def gen_stream_tuples(ctx: "Context", d_out: int, n: int, seed: int, key_domain: int,

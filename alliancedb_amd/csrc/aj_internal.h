@@ -60,6 +60,8 @@ struct ProConfig {
     bool p2d = true;      // two passes: pass 2 from the 2-D pass-1 histogram when it fits
 };
 ProConfig choose_pro_config(uint64_t nR, uint64_t nS, const aj_join_opts_t *opts);
+// the autotuner's entry for the size bucket of (nR, nS), if any (tune.cpp)
+bool tune_lookup(uint64_t nR, uint64_t nS, aj_tune_entry_t *out);
 
 // ---------------------------------------------------------------- context
 struct Ctx {

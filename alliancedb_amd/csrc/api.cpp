@@ -92,7 +92,7 @@ void DevBuf::release() {
 // ---------------------------------------------------------------- options
 static std::atomic<int64_t> opt_materialize{0}, opt_radix_bits{0}, opt_passes{0}, opt_device{0},
     opt_cap{0}, opt_split{0}, opt_kernel_timing{0}, opt_part_block{0}, opt_part_chunk{0},
-    opt_part_hist2d{1};
+    opt_part_hist2d{1}, opt_tuned{1};
 
 bool kernel_timing_enabled() { return opt_kernel_timing.load() != 0; }
 
@@ -159,7 +159,19 @@ static ProConfig config_from(uint64_t nR, uint64_t nS, const aj_join_opts_t *o) 
     if (!eff.passes) eff.passes = (int)opt_passes.load();
     if (!eff.cap) eff.cap = (int)opt_cap.load();
     if (!eff.split) eff.split = (uint32_t)opt_split.load();
+    aj_tune_entry_t te;
+    const bool use_table = opt_tuned.load() && !eff.radix_bits && !eff.passes && !eff.cap &&
+                           tune_lookup(nR, nS, &te);
+    if (use_table) {
+        eff.radix_bits = te.radix_bits;
+        eff.passes = te.passes;
+        eff.cap = te.cap;
+    }
     ProConfig c = choose_pro_config(nR, nS, &eff);
+    if (use_table) {
+        c.block = te.part_block;
+        if (te.part_chunk) c.chunk = (int)te.part_chunk;
+    }
     if (opt_part_block.load()) c.block = (int)opt_part_block.load();
     if (opt_part_chunk.load()) c.chunk = (int)opt_part_chunk.load();
     c.p2d = opt_part_hist2d.load() != 0;
@@ -324,7 +336,8 @@ int aj_set_option(const char *name, int64_t value) {
                {"kernel_timing", &opt_kernel_timing, 0, 1},
                {"part_block", &opt_part_block, 0, 512},
                {"part_chunk", &opt_part_chunk, 0, 1ll << 24},
-               {"part_hist2d", &opt_part_hist2d, 0, 1}};
+               {"part_hist2d", &opt_part_hist2d, 0, 1},
+               {"tuned", &opt_tuned, 0, 1}};
     for (auto &e : tab) {
         if (strcmp(e.n, name) == 0) {
             if (value < e.lo || value > e.hi) {
@@ -368,6 +381,7 @@ int64_t aj_get_option(const char *name) {
     if (!strcmp(name, "part_block")) return opt_part_block.load();
     if (!strcmp(name, "part_chunk")) return opt_part_chunk.load();
     if (!strcmp(name, "part_hist2d")) return opt_part_hist2d.load();
+    if (!strcmp(name, "tuned")) return opt_tuned.load();
     if (!strcmp(name, "device_budget")) return (int64_t)g_dev_budget.load();
     if (!strcmp(name, "device_used")) return (int64_t)g_dev_used.load();
     return -1;

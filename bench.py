@@ -68,6 +68,8 @@ def parse():
     ap.add_argument("--passes", type=int, default=0)
     ap.add_argument("--cap", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="pro/skew: use the size heuristic instead of autotuning the geometry")
     ap.add_argument("--cpu-log2n", type=int, default=24)
     ap.add_argument("--calib", action="store_true",
                     help="also run aj_copy_kernel over 1 GiB (traffic-counter calibration)")
@@ -117,6 +119,17 @@ def closed_form_fk(R: np.ndarray, S: np.ndarray):
     x = (inv[S["key"]] << np.uint64(32)).sum(dtype=np.uint64)
     x = (int(x) + int(S["payloadID"].astype(np.uint64).sum(dtype=np.uint64))) % (1 << 64)
     return len(S), x
+
+
+def autotune(aj, ctx, dR, nR, dS, nS, algo, kw, args):
+    """Autotuned geometry for this size bucket (untimed, before the warm-up); the
+    timed joins then run with the table's entry (their options leave it open)."""
+    if args.no_tune or algo != aj.AJ_ALGO_PRO or any(kw.values()):
+        return None
+    t0 = time.time()
+    best = aj.tune_pro(ctx, dR.ptr, nR, dS.ptr, nS, reps=3)
+    print(f"[bench] tuned in {time.time() - t0:.1f}s: {best}", file=sys.stderr, flush=True)
+    return best
 
 
 def timed_joins(aj, ctx, dR, nR, dS, nS, algo, kw, steps, warmup, want):
@@ -190,13 +203,20 @@ def run_pro(aj, args):
     algo = aj.AJ_ALGO_PRO if args.algo == "PRO" else aj.AJ_ALGO_NPO
     kw = dict(radix_bits=args.radix_bits, passes=args.passes, cap=args.cap)
     with aj.Context(0) as ctx:
+        tuned = autotune(aj, ctx, dR, n, dS, n, algo, kw, args)
         out, dev, ms_step, phases = timed_joins(aj, ctx, dR, n, dS, n, algo, kw, args.steps,
                                                 args.warmup, want)
     name = (f"{args.algo} |R|=|S|=2^{args.log2n} 8-byte tuples, "
             f"unique R keys x uniform S keys")
     line = join_line(aj, args, name, n, n, algo, out, dev, ms_step, phases,
                      "join throughput M tuples/s (|R|+|S|)")
-    if not (args.radix_bits or args.passes or args.cap):
+    if tuned:
+        line["config"]["geometry"] = {k: tuned[k] for k in ("radix_bits", "passes", "cap",
+                                                            "part_block")}
+        line["config"]["geometry"]["chosen_by"] = "autotuner (aj_tune_pro)"
+    default_geom = not (args.radix_bits or args.passes or args.cap) and (
+        not tuned or (tuned["radix_bits"], tuned["passes"], tuned["cap"]) == (14, 2, 8704))
+    if default_geom:
         line["roofline"]["traffic"] = measured_traffic(line["roofline"]["kernel"], name)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(aj, args.cpu_log2n)
@@ -216,12 +236,17 @@ def run_skew(aj, args):
     algo = aj.AJ_ALGO_PRO if args.algo == "PRO" else aj.AJ_ALGO_NPO
     kw = dict(radix_bits=args.radix_bits, passes=args.passes, cap=args.cap)
     with aj.Context(0) as ctx:
+        tuned = autotune(aj, ctx, dR, nR, dS, nS, algo, kw, args)
         out, dev, ms_step, phases = timed_joins(aj, ctx, dR, nR, dS, nS, algo, kw, args.steps,
                                                 args.warmup, want)
     name = (f"{args.algo} skewed foreign-key join |R|=2^24 unique x |S|=2^28 "
             f"Zipf(theta={args.theta})")
-    return join_line(aj, args, name, nR, nS, algo, out, dev, ms_step, phases,
+    line = join_line(aj, args, name, nR, nS, algo, out, dev, ms_step, phases,
                      "join throughput M tuples/s (|R|+|S|)")
+    if tuned:
+        line["config"]["geometry"] = {k: tuned[k] for k in ("radix_bits", "passes", "cap",
+                                                            "part_block")}
+    return line
 
 
 def run_windows(aj, args):

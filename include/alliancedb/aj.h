@@ -95,6 +95,7 @@ const aj_result_t *aj_result_ext(const result_t *r);
  *   "part_chunk"   multiple of 8192  tuples per partition work-group (0 = derived)
  *   "part_hist2d"  0/1   two-pass PRO: pass 2 from a (d1, d2) histogram taken in
  *                        pass 1 instead of its own histogram pass (default 1)
+ *   "tuned"        0/1   use the autotuner's table for open geometries (default 1)
  *   "device_budget" bytes of device memory the library may hold (0 = unlimited)
  */
 int aj_set_option(const char *name, int64_t value);
@@ -312,6 +313,39 @@ int aj_server_poll(aj_server_t *srv, aj_server_result_t *out, int max, int64_t t
 int aj_server_drain(aj_server_t *srv);
 double aj_server_now_us(aj_server_t *srv);
 int aj_server_stats(aj_server_t *srv, aj_server_stats_t *out);
+
+/* ------------------------------------------------------------ autotuner */
+/*
+ * PRO geometry per size bucket (floor log2 |R|, floor log2 |S|).  No
+ * counterpart in the reference, which fixes NUM_RADIX_BITS / NUM_PASSES at
+ * compile time (prj_params.h:16-21).  A PRO call whose options leave radix
+ * bits, passes and cap all at 0 uses the bucket's entry when one exists and
+ * the option "tuned" is 1 (the default).
+ */
+typedef struct aj_tune_entry_t {
+    int log2_r, log2_s;   /* size bucket */
+    int radix_bits;       /* total radix bits */
+    int passes;           /* 1 or 2 (0 with 0 bits) */
+    int cap;              /* LDS round */
+    int part_block;       /* 256 or 512 */
+    uint32_t part_chunk;  /* 0 = derived */
+    float device_ms;      /* measured at tuning time */
+} aj_tune_entry_t;
+
+/* Times candidate geometries (radix bits around the size heuristic, 1 or 2
+ * passes, LDS rounds 8192/8704/12288, then the partition work-group size) on
+ * the given device relations, `reps` timed runs each after a warm-up; every
+ * candidate must return the same count and checksum (else AJ_ERR_STATE).  The
+ * fastest becomes the bucket's entry and is returned in *best. */
+int aj_tune_pro(aj_ctx_t *ctx, const tuple_t *dR, uint64_t nR, const tuple_t *dS, uint64_t nS,
+                int reps, aj_tune_entry_t *best);
+int aj_tune_put(const aj_tune_entry_t *entry);
+/* AJ_ERR_STATE when the bucket has no entry. */
+int aj_tune_get(uint64_t nR, uint64_t nS, aj_tune_entry_t *out);
+int aj_tune_clear(void);
+/* JSON, one entry per line; load merges into the current table. */
+int aj_tune_save(const char *path);
+int aj_tune_load(const char *path);
 
 /* ------------------------------------------------------------ generators */
 /*

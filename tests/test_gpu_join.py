@@ -190,3 +190,28 @@ def test_skewed_foreign_key_join(aj, oracle, theta, algo):
     with aj.Context(0) as ctx:
         out = ctx.join(dR.ptr, nR, dS.ptr, nS, algo)
     assert (out.count, out.checksum) == want
+
+
+def test_autotuner_picks_a_geometry_with_identical_results(aj, oracle):
+    """aj_tune_pro times candidates on device data; the kept geometry joins with
+    the same count and checksum as the oracle, and open-geometry calls use it."""
+    n = 1 << 20
+    R = aj.gen_unique(n, 5)
+    S = aj.gen_uniform(n, 1, n, 6)
+    want = oracle.aggregate(R, S)
+    dR, dS = aj.DeviceArray.from_host(R), aj.DeviceArray.from_host(S)
+    aj.tune_clear()
+    try:
+        with aj.Context(0) as ctx:
+            print("case autotune 2^20", flush=True)
+            best = aj.tune_pro(ctx, dR.ptr, n, dS.ptr, n, reps=2)
+            assert best["log2_r"] == 20 and best["log2_s"] == 20
+            assert best["cap"] in (8192, 8704, 12288) and best["device_ms"] > 0
+            assert aj.tune_get(n, n) == best
+            out = ctx.join(dR.ptr, n, dS.ptr, n)  # geometry from the table
+            assert (out.count, out.checksum) == want
+            out = ctx.join(dR.ptr, n, dS.ptr, n, radix_bits=best["radix_bits"],
+                           passes=best["passes"], cap=best["cap"])
+            assert (out.count, out.checksum) == want
+    finally:
+        aj.tune_clear()
