@@ -42,6 +42,30 @@ class result_t(ctypes.Structure):
                 ("nthreads", ctypes.c_int)]
 
 
+class threadresult_t(ctypes.Structure):
+    _fields_ = [("nresults", ctypes.c_int64), ("results", ctypes.c_void_p),
+                ("threadid", ctypes.c_uint32)]
+
+
+class aj_tuplebuffer_t(ctypes.Structure):
+    pass
+
+
+aj_tuplebuffer_t._fields_ = [("tuples", ctypes.POINTER(ctypes.c_uint64)),
+                             ("next", ctypes.POINTER(aj_tuplebuffer_t))]
+
+
+class aj_chainedtuplebuffer_t(ctypes.Structure):
+    _fields_ = [("buf", ctypes.POINTER(aj_tuplebuffer_t)),
+                ("readcursor", ctypes.POINTER(aj_tuplebuffer_t)),
+                ("writecursor", ctypes.POINTER(aj_tuplebuffer_t)),
+                ("writepos", ctypes.c_uint32), ("readpos", ctypes.c_uint32),
+                ("readlen", ctypes.c_uint32), ("numbufs", ctypes.c_uint32)]
+
+
+CHAINEDBUFF_NUMTUPLESPERBUF = 1024 * 1024
+
+
 class aj_result_t(ctypes.Structure):
     _fields_ = [("base", result_t), ("checksum", ctypes.c_uint64), ("npairs", ctypes.c_uint64),
                 ("pairs", ctypes.POINTER(ctypes.c_uint64)), ("status", ctypes.c_int),
@@ -51,7 +75,8 @@ class aj_result_t(ctypes.Structure):
 class aj_join_opts_t(ctypes.Structure):
     _fields_ = [("algo", ctypes.c_int), ("radix_bits", ctypes.c_int), ("passes", ctypes.c_int),
                 ("cap", ctypes.c_int), ("split", ctypes.c_uint32),
-                ("pairs", ctypes.c_void_p), ("pairs_capacity", ctypes.c_uint64)]
+                ("pairs", ctypes.c_void_p), ("pairs_capacity", ctypes.c_uint64),
+                ("records", ctypes.c_int)]
 
 
 AJ_NPHASES = 12
@@ -125,6 +150,8 @@ def _load() -> ctypes.CDLL:
         "PRO": (c.POINTER(result_t), [c.POINTER(relation_t), c.POINTER(relation_t), c.c_int]),
         "NPO": (c.POINTER(result_t), [c.POINTER(relation_t), c.POINTER(relation_t), c.c_int]),
         "aj_result_ext": (c.POINTER(aj_result_t), [c.POINTER(result_t)]),
+        "aj_free_result": (None, [c.POINTER(result_t)]),
+        "aj_write_result_file": (c.c_int, [c.POINTER(result_t), c.c_char_p]),
         "aj_set_option": (c.c_int, [c.c_char_p, c.c_int64]),
         "aj_get_option": (c.c_int64, [c.c_char_p]),
         "aj_ctx_create": (c.c_int, [c.c_int, c.POINTER(P)]),
@@ -218,9 +245,36 @@ class JoinResult:
     pairs: Optional[np.ndarray]
     device_ms: float
     total_ms: float = 0.0
+    # option materialize = 2: per resultlist entry, its <S.key, S.payloadID>
+    # records as uint64 (key << 32 | payloadID) in append order (oldest buffer
+    # first); the reference's backward read gives them in reverse
+    records: Optional[list] = None
+    # per resultlist entry: (threadid, numbufs, writepos)
+    chains: Optional[list] = None
 
 
-def _dropin(fn, R: np.ndarray, S: np.ndarray, nthreads: int) -> JoinResult:
+def _read_chain(cb: aj_chainedtuplebuffer_t, n: int) -> np.ndarray:
+    """All n records of a chain in append order: the head buffer (newest, cb.buf)
+    holds writepos records, each older buffer a full CHAINEDBUFF_NUMTUPLESPERBUF
+    (hashing/utils/tuple_buffer.h:93-109)."""
+    out = []
+    b = cb.buf
+    take = cb.writepos
+    left = n
+    while left > 0:
+        if not b:
+            raise AjError(AJ_ERR_STATE, "record chain shorter than its count")
+        k = min(take, left)
+        if k:
+            out.append(np.ctypeslib.as_array(b.contents.tuples, shape=(k,)).copy())
+        left -= k
+        b = b.contents.next
+        take = CHAINEDBUFF_NUMTUPLESPERBUF
+    return np.concatenate(out[::-1]) if out else np.zeros(0, np.uint64)
+
+
+def _dropin(fn, R: np.ndarray, S: np.ndarray, nthreads: int,
+            result_file: Optional[str] = None) -> JoinResult:
     R = np.ascontiguousarray(R, dtype=TUPLE_DTYPE)
     S = np.ascontiguousarray(S, dtype=TUPLE_DTYPE)
     rr = relation_t(R.ctypes.data if len(R) else None, len(R), None)
@@ -233,20 +287,33 @@ def _dropin(fn, R: np.ndarray, S: np.ndarray, nthreads: int) -> JoinResult:
         pairs = None
         if ext.npairs:
             pairs = np.ctypeslib.as_array(ext.pairs, shape=(ext.npairs,)).copy()
+        records = chains = None
+        if ext.base.resultlist:
+            tl = ctypes.cast(ext.base.resultlist, ctypes.POINTER(threadresult_t))
+            records, chains = [], []
+            for i in range(ext.base.nthreads):
+                cb = ctypes.cast(tl[i].results, ctypes.POINTER(aj_chainedtuplebuffer_t)).contents
+                records.append(_read_chain(cb, int(tl[i].nresults)))
+                chains.append((int(tl[i].threadid), int(cb.numbufs), int(cb.writepos)))
+        if result_file is not None:
+            check(lib.aj_write_result_file(res, os.fsencode(result_file)))
         return JoinResult(int(ext.base.totalresults), int(ext.checksum), pairs,
-                          float(ext.device_ms), float(ext.total_ms))
+                          float(ext.device_ms), float(ext.total_ms), records, chains)
     finally:
-        _libc.free(ctypes.cast(res, ctypes.c_void_p))
+        lib.aj_free_result(res)
 
 
-def PRO(R: np.ndarray, S: np.ndarray, nthreads: int = 1) -> JoinResult:
-    """Drop-in PRO (radix join) on host relations; see aj.h."""
-    return _dropin(lib.PRO, R, S, nthreads)
+def PRO(R: np.ndarray, S: np.ndarray, nthreads: int = 1,
+        result_file: Optional[str] = None) -> JoinResult:
+    """Drop-in PRO (radix join) on host relations; see aj.h.  With result_file
+    the result's records are written there as the reference's Out.tbl."""
+    return _dropin(lib.PRO, R, S, nthreads, result_file)
 
 
-def NPO(R: np.ndarray, S: np.ndarray, nthreads: int = 1) -> JoinResult:
+def NPO(R: np.ndarray, S: np.ndarray, nthreads: int = 1,
+        result_file: Optional[str] = None) -> JoinResult:
     """Drop-in NPO (no-partitioning join) on host relations; see aj.h."""
-    return _dropin(lib.NPO, R, S, nthreads)
+    return _dropin(lib.NPO, R, S, nthreads, result_file)
 
 
 class Context:

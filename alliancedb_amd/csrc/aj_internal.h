@@ -15,7 +15,7 @@
 namespace aj {
 
 // ---------------------------------------------------------------- errors
-void set_error(int code, const char *fmt, ...);
+int set_error(int code, const char *fmt, ...);  // returns code
 int last_status();
 
 #define AJ_HIP(call)                                                              \
@@ -169,6 +169,7 @@ struct JoinDesc {
     uint64_t *seg_count, *seg_sum;  // nseg accumulators (zeroed by the caller)
     uint64_t *pairs;                // nullable: materialise pairs
     uint64_t pairs_cap;
+    bool records = false;           // materialise <S.key, S.payloadID> records instead
     uint64_t *pair_cursor;          // device scalar (zeroed by the caller)
     uint32_t *task_cnt;             // workspace: max_parts+1
     uint32_t *task_base;            // workspace: max_parts+1
@@ -197,6 +198,7 @@ struct NpoDesc {
     uint64_t *acc;       // [0]=count [1]=sum [2]=sentinel R tuples [3]=pair cursor
     uint64_t *pairs;
     uint64_t pairs_cap;
+    bool records = false;  // materialise <S.key, S.payloadID> records instead of pairs
     int num_cu;
 };
 int run_npo(const NpoDesc &d, hipStream_t st, const Marks *marks = nullptr, int m0 = 0);

@@ -5,6 +5,7 @@
 //   ajbench -a PRO|NPO [-r |R|] [-s |S|] [-x r_seed] [-y s_seed]
 //           [-d key distribution: 0 unique/uniform, 2 zipf] [-z zipf theta]
 //           [-w window -e step -l interval] [-n nthreads] [-m] [-R repeats]
+//           [--pairs] [--out FILE]
 //   ajbench -a SERVE [--load f] [--peak Mtps] [--seconds s] [--streams n]
 //           [--budget bytes] [--slots n] [--batch-windows n] [--clients n]
 //           [--verify n] [--median n] [--sigma s] [--cap-tuples n]
@@ -15,7 +16,11 @@
 // reference's help text, main.cpp:403-404).  R keys are a seeded permutation of
 // 1..|R|; S keys are uniform on 1..|R| (-d 0) or Zipf(theta) over R's keys
 // (-d 2).  Prints "Results = N" like benchmark.cpp:249, plus the checksum and
-// the device and end-to-end times.
+// the device and end-to-end times.  -m materialises the reference's result
+// records (<S.key, S.payloadID> per match in chained per-thread buffers, as a
+// JOIN_RESULT_MATERIALIZE build does); --out FILE also writes them as the
+// reference's Out.tbl (benchmark.cpp:262-265); --pairs returns the
+// (R.payloadID, S.payloadID) pair list instead.
 #include <getopt.h>
 
 #include <chrono>
@@ -69,19 +74,23 @@ struct params {
     int nthreads = 1;
     int materialize = 0;
     int repeats = 1;
+    const char *out_file = nullptr;
 };
 
 void usage(const char *prog) {
     printf("Usage: %s -a <PRO|NPO> [-r R] [-s S] [-x seed] [-y seed] [-d 0|2] [-z theta]\n"
            "          [-w window -e step -l interval] [-n nthreads] [-m] [-R repeats]\n"
+           "          [--pairs] [--out FILE]\n"
            "       %s -a SERVE [--load f] [--peak Mtps] [--seconds s] [--streams n]\n"
            "          [--budget bytes] [--slots n] [--batch-windows n] [--clients n]\n"
-           "          [--verify n] [--median n] [--sigma s] [--cap-tuples n] [--domain n]\n",
+           "          [--verify n] [--median n] [--sigma s] [--cap-tuples n] [--domain n]\n"
+           "          [--batch-tuples n]\n",
            prog, prog);
 }
 
 enum { OPT_LOAD = 1000, OPT_PEAK, OPT_SECONDS, OPT_STREAMS, OPT_BUDGET, OPT_SLOTS, OPT_BWIN,
-       OPT_CLIENTS, OPT_VERIFY, OPT_MEDIAN, OPT_SIGMA, OPT_CAPT, OPT_DOMAIN };
+       OPT_CLIENTS, OPT_VERIFY, OPT_MEDIAN, OPT_SIGMA, OPT_CAPT, OPT_DOMAIN, OPT_BTUP,
+       OPT_PAIRS, OPT_OUT };
 
 }  // namespace
 
@@ -102,7 +111,11 @@ int main(int argc, char **argv) {
                                 {"median", required_argument, 0, OPT_MEDIAN},
                                 {"sigma", required_argument, 0, OPT_SIGMA},
                                 {"cap-tuples", required_argument, 0, OPT_CAPT},
-                                {"domain", required_argument, 0, OPT_DOMAIN},{"algo", required_argument, 0, 'a'},
+                                {"domain", required_argument, 0, OPT_DOMAIN},
+                                {"batch-tuples", required_argument, 0, OPT_BTUP},
+                                {"pairs", no_argument, 0, OPT_PAIRS},
+                                {"out", required_argument, 0, OPT_OUT},
+                                {"algo", required_argument, 0, 'a'},
                                 {"r-size", required_argument, 0, 'r'},
                                 {"s-size", required_argument, 0, 's'},
                                 {"r-seed", required_argument, 0, 'x'},
@@ -154,11 +167,14 @@ int main(int argc, char **argv) {
             case OPT_SIGMA: sp.sigma = atof(optarg); break;
             case OPT_CAPT: sp.cap_tuples = strtoull(optarg, nullptr, 10); break;
             case OPT_DOMAIN: sp.domain = (uint32_t)strtoul(optarg, nullptr, 10); break;
+            case OPT_BTUP: sp.batch_tuples = strtoull(optarg, nullptr, 10); break;
             case 'w': p.window = atol(optarg); break;
             case 'e': p.step = atol(optarg); break;
             case 'l': p.interval = atol(optarg); break;
             case 'n': p.nthreads = atoi(optarg); break;
-            case 'm': p.materialize = 1; break;
+            case 'm': p.materialize = 2; break;
+            case OPT_PAIRS: p.materialize = 1; break;
+            case OPT_OUT: p.out_file = optarg; p.materialize = 2; break;
             case 'R': p.repeats = atoi(optarg); break;
             case 'h': usage(argv[0]); return EXIT_SUCCESS;
             default: usage(argv[0]); return EXIT_FAILURE;
@@ -212,7 +228,15 @@ int main(int argc, char **argv) {
                "total_ms = %.3f wall_ms = %.3f\n",
                (long long)res->totalresults, (unsigned long long)x->checksum,
                (unsigned long long)x->npairs, x->device_ms, x->total_ms, ms);
-        free(res);
+        if (p.out_file && rep + 1 == p.repeats) {
+            printf("[INFO ] Persisting the join result to \"%s\" ...\n", p.out_file);
+            if (aj_write_result_file(res, p.out_file) != AJ_OK) {
+                fprintf(stderr, "[ERROR] %s\n", aj_last_error());
+                aj_free_result(res);
+                return EXIT_FAILURE;
+            }
+        }
+        aj_free_result(res);
     }
     printf("[INFO ] DONE.\n");
     return EXIT_SUCCESS;

@@ -1,4 +1,5 @@
 // api.cpp -- the C ABI: errors, options, contexts, drop-in PRO/NPO, device API.
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <cstdio>
@@ -11,21 +12,23 @@
 namespace aj {
 
 int pro_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, uint64_t nS,
-               const ProConfig &cfg, uint64_t *pairs, uint64_t pairs_cap, aj_join_out_t *out,
-               hipStream_t st);
+               const ProConfig &cfg, uint64_t *pairs, uint64_t pairs_cap, bool records,
+               aj_join_out_t *out, hipStream_t st);
 int npo_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, uint64_t nS,
-               uint64_t *pairs, uint64_t pairs_cap, aj_join_out_t *out, hipStream_t st);
+               uint64_t *pairs, uint64_t pairs_cap, bool records, aj_join_out_t *out,
+               hipStream_t st);
 
 // ---------------------------------------------------------------- errors
 static thread_local char g_err[512] = "";
 static thread_local int g_status = AJ_OK;
 
-void set_error(int code, const char *fmt, ...) {
+int set_error(int code, const char *fmt, ...) {
     va_list ap;
     va_start(ap, fmt);
     vsnprintf(g_err, sizeof(g_err), fmt, ap);
     va_end(ap);
     g_status = code;
+    return code;
 }
 int last_status() { return g_status; }
 static void clear_error() {
@@ -186,11 +189,13 @@ static int join_dev_locked(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint
     const int algo = opts ? opts->algo : AJ_ALGO_PRO;
     uint64_t *pairs = opts ? opts->pairs : nullptr;
     const uint64_t pcap = opts ? opts->pairs_capacity : 0;
+    const bool records = opts && opts->records;
     int rc;
     if (algo == AJ_ALGO_NPO) {
-        rc = npo_device(ctx, dR, nR, dS, nS, pairs, pcap, out, st);
+        rc = npo_device(ctx, dR, nR, dS, nS, pairs, pcap, records, out, st);
     } else if (algo == AJ_ALGO_PRO) {
-        rc = pro_device(ctx, dR, nR, dS, nS, config_from(nR, nS, opts), pairs, pcap, out, st);
+        rc = pro_device(ctx, dR, nR, dS, nS, config_from(nR, nS, opts), pairs, pcap, records,
+                        out, st);
     } else {
         set_error(AJ_ERR_ARG, "unknown algorithm %d", algo);
         return AJ_ERR_ARG;
@@ -222,6 +227,88 @@ static int check_relation(const relation_t *r, const char *name) {
     return AJ_OK;
 }
 
+// ------------------------------------------- reference-format result lists
+// The reference materialises, per joining thread, one record per match: the
+// matching S tuple <S.key, S.payloadID> (parallel_radix_join.cpp:333-338,
+// common_functions.cpp:252-257), appended to a chainedtuplebuffer_t whose
+// newest 1M-tuple buffer is at the head of the chain (tuple_buffer.h:31-47,
+// cb_next_writepos :93-109).  materialize = 2 returns exactly that structure:
+// entry 0 of result_t.resultlist holds every record, the other entries are
+// empty chains (one buffer, writepos 0) as chainedtuplebuffer_init leaves them.
+
+static void chain_free(aj_chainedtuplebuffer_t *cb) {
+    if (!cb) return;
+    aj_tuplebuffer_t *b = cb->buf;
+    while (b) {
+        aj_tuplebuffer_t *nx = b->next;
+        free(b->tuples);
+        free(b);
+        b = nx;
+    }
+    free(cb);
+}
+
+// A chain holding the n records of dsrc (device memory; may be NULL when n = 0),
+// record i in buffer i / AJ_CHAINEDBUFF_NUMTUPLESPERBUF counted from the tail.
+static aj_chainedtuplebuffer_t *chain_from_device(const uint64_t *dsrc, uint64_t n, int *rc) {
+    const uint64_t per = AJ_CHAINEDBUFF_NUMTUPLESPERBUF;
+    const uint64_t nbuf = n ? (n + per - 1) / per : 1;
+    auto *cb = (aj_chainedtuplebuffer_t *)calloc(1, sizeof(aj_chainedtuplebuffer_t));
+    if (!cb) {
+        *rc = set_error(AJ_ERR_NOMEM, "host allocation of a result chain failed");
+        return nullptr;
+    }
+    for (uint64_t k = 0; k < nbuf; ++k) {  // oldest first; each new buffer becomes the head
+        auto *b = (aj_tuplebuffer_t *)calloc(1, sizeof(aj_tuplebuffer_t));
+        void *t = nullptr;
+        if (!b || posix_memalign(&t, 64, per * sizeof(tuple_t))) {
+            free(b);
+            chain_free(cb);
+            *rc = set_error(AJ_ERR_NOMEM, "host allocation of a result buffer failed");
+            return nullptr;
+        }
+        b->tuples = (tuple_t *)t;
+        b->next = cb->buf;
+        cb->buf = b;
+        const uint64_t len = std::min(per, n - std::min(n, k * per));
+        if (len && hipMemcpy(b->tuples, dsrc + k * per, len * 8, hipMemcpyDeviceToHost)) {
+            chain_free(cb);
+            *rc = set_error(AJ_ERR_HIP, "device-to-host copy of result records failed");
+            return nullptr;
+        }
+    }
+    cb->readcursor = cb->writecursor = cb->buf;
+    cb->writepos = (uint32_t)(n - (nbuf - 1) * per);
+    cb->numbufs = (uint32_t)nbuf;
+    return cb;
+}
+
+static void free_resultlist(threadresult_t *l, int n) {
+    if (!l) return;
+    for (int i = 0; i < n; ++i) chain_free((aj_chainedtuplebuffer_t *)l[i].results);
+    free(l);
+}
+
+// resultlist of nt entries, entry 0 holding the n device records of dsrc.
+static threadresult_t *build_resultlist(const uint64_t *dsrc, uint64_t n, int nt, int *rc) {
+    auto *l = (threadresult_t *)calloc((size_t)nt, sizeof(threadresult_t));
+    if (!l) {
+        *rc = set_error(AJ_ERR_NOMEM, "host allocation of the result list failed");
+        return nullptr;
+    }
+    for (int i = 0; i < nt; ++i) {
+        const uint64_t m = i == 0 ? n : 0;
+        l[i].threadid = (uint32_t)i;
+        l[i].nresults = (int64_t)m;
+        l[i].results = chain_from_device(dsrc, m, rc);
+        if (!l[i].results) {
+            free_resultlist(l, i);
+            return nullptr;
+        }
+    }
+    return l;
+}
+
 static result_t *dropin(int algo, relation_t *relR, relation_t *relS, int nthreads) {
     clear_error();
     if (check_relation(relR, "R") || check_relation(relS, "S")) return nullptr;
@@ -237,9 +324,11 @@ static result_t *dropin(int algo, relation_t *relR, relation_t *relS, int nthrea
     auto t0 = std::chrono::steady_clock::now();
     aj_join_out_t out;
     memset(&out, 0, sizeof(out));
-    const bool mat = opt_materialize.load() != 0;
+    const int64_t mat = opt_materialize.load();
+    const int nt = std::max(nthreads, 1);
     uint64_t npairs = 0;
     std::vector<uint64_t> host_pairs;
+    threadresult_t *rlist = nullptr;
     int rc = AJ_OK;
     do {
         if (nR == 0 || nS == 0) break;
@@ -260,10 +349,23 @@ static result_t *dropin(int algo, relation_t *relR, relation_t *relS, int nthrea
         if ((rc = ctx->pairs.ensure(out.count * 8))) break;
         o.pairs = ctx->pairs.as<uint64_t>();
         o.pairs_capacity = out.count;
+        o.records = mat == 2;
         aj_join_out_t out2;
         rc = join_dev_locked(ctx, ctx->inR.as<uint64_t>(), nR, ctx->inS.as<uint64_t>(), nS, &o,
                              &out2, ctx->stream);
         if (rc) break;
+        if (out2.count != out.count || out2.npairs != out.count) {
+            rc = set_error(AJ_ERR_STATE, "materialising join disagrees with the counting join");
+            break;
+        }
+        if (mat == 2) {
+            if (hipStreamSynchronize(ctx->stream)) {
+                rc = set_error(AJ_ERR_HIP, "join stream failed");
+                break;
+            }
+            rlist = build_resultlist(ctx->pairs.as<uint64_t>(), out2.npairs, nt, &rc);
+            break;
+        }
         npairs = out2.npairs;
         host_pairs.resize(npairs);
         if (hipMemcpy(host_pairs.data(), ctx->pairs.p, npairs * 8, hipMemcpyDeviceToHost)) {
@@ -272,14 +374,19 @@ static result_t *dropin(int algo, relation_t *relR, relation_t *relS, int nthrea
         }
     } while (0);
     if (rc) return nullptr;
+    if (mat == 2 && !rlist) {  // no matches (or an empty input): nt empty chains
+        rlist = build_resultlist(nullptr, 0, nt, &rc);
+        if (!rlist) return nullptr;
+    }
     aj_result_t *res = (aj_result_t *)malloc(sizeof(aj_result_t) + npairs * sizeof(uint64_t));
     if (!res) {
+        free_resultlist(rlist, nt);
         set_error(AJ_ERR_NOMEM, "host allocation of the result failed");
         return nullptr;
     }
     res->base.totalresults = (int64_t)out.count;
-    res->base.resultlist = nullptr;
-    res->base.nthreads = nthreads;
+    res->base.resultlist = rlist;
+    res->base.nthreads = rlist ? nt : nthreads;
     res->checksum = out.checksum;
     res->npairs = npairs;
     res->pairs = npairs ? reinterpret_cast<uint64_t *>(res + 1) : nullptr;
@@ -320,6 +427,71 @@ const aj_result_t *aj_result_ext(const result_t *r) {
     return reinterpret_cast<const aj_result_t *>(r);
 }
 
+void aj_free_result(result_t *r) {
+    if (!r) return;
+    free_resultlist(r->resultlist, r->nthreads);
+    free(r);
+}
+
+int aj_write_result_file(const result_t *r, const char *path) {
+    clear_error();
+    if (!r || !path) return set_error(AJ_ERR_ARG, "aj_write_result_file: NULL argument");
+    if (r->totalresults > 0 && !r->resultlist)
+        return set_error(AJ_ERR_STATE, "result holds no record lists (option materialize 2)");
+    FILE *f = fopen(path, "w");
+    if (!f) return set_error(AJ_ERR_ARG, "cannot write %s", path);
+    std::vector<char> buf(1 << 20);
+    size_t used = 0;
+    bool bad = false;
+    auto put_int = [&](int32_t v, char end) {
+        char tmp[12];
+        int k = 0;
+        uint32_t u = v < 0 ? 0u - (uint32_t)v : (uint32_t)v;
+        do {
+            tmp[k++] = (char)('0' + u % 10);
+            u /= 10;
+        } while (u);
+        if (v < 0) buf[used++] = '-';
+        while (k) buf[used++] = tmp[--k];
+        buf[used++] = end;
+    };
+    const int nt = r->resultlist ? r->nthreads : 0;
+    for (int i = 0; i < nt && !bad; ++i) {
+        const threadresult_t &tr = r->resultlist[i];
+        const auto *cb = (const aj_chainedtuplebuffer_t *)tr.results;
+        if (tr.nresults <= 0) continue;
+        // newest record first, as cb_begin_backwards / cb_read_backwards read
+        // (tuple_buffer.h:57-90), one "%d %d" line of key and payloadID each
+        const aj_tuplebuffer_t *b = cb ? cb->buf : nullptr;
+        uint64_t pos = cb && cb->writepos ? cb->writepos - 1 : 0;
+        for (int64_t j = 0; j < tr.nresults; ++j) {
+            if (!b || (cb->writepos == 0 && j == 0)) {
+                bad = true;
+                set_error(AJ_ERR_STATE, "result list %d holds fewer records than it counts", i);
+                break;
+            }
+            const tuple_t t = b->tuples[pos];
+            if (used + 32 > buf.size()) {
+                bad |= fwrite(buf.data(), 1, used, f) != used;
+                used = 0;
+            }
+            put_int(t.key, ' ');
+            put_int(t.payloadID, '\n');
+            if (pos == 0) {
+                b = b->next;
+                pos = AJ_CHAINEDBUFF_NUMTUPLESPERBUF - 1;
+            } else {
+                --pos;
+            }
+        }
+    }
+    if (!bad && used) bad = fwrite(buf.data(), 1, used, f) != used;
+    bad |= ferror(f) != 0;
+    fclose(f);
+    if (bad) return g_status != AJ_OK ? g_status : set_error(AJ_ERR_ARG, "error writing %s", path);
+    return AJ_OK;
+}
+
 int aj_set_option(const char *name, int64_t value) {
     clear_error();
     if (!name) {
@@ -330,7 +502,7 @@ int aj_set_option(const char *name, int64_t value) {
         const char *n;
         std::atomic<int64_t> *v;
         int64_t lo, hi;
-    } tab[] = {{"materialize", &opt_materialize, 0, 1}, {"radix_bits", &opt_radix_bits, 0, 24},
+    } tab[] = {{"materialize", &opt_materialize, 0, 2}, {"radix_bits", &opt_radix_bits, 0, 24},
                {"passes", &opt_passes, 0, 2},           {"device", &opt_device, 0, 1024},
                {"cap", &opt_cap, 0, 12288},             {"split", &opt_split, 0, 1ll << 30},
                {"kernel_timing", &opt_kernel_timing, 0, 1},

@@ -69,6 +69,7 @@ struct JoinArgs {
     uint64_t *pairs;
     uint64_t pairs_cap;
     uint64_t *pair_cursor;
+    int records;  // write the S tuple <S.key, S.payloadID> per match instead of the pair
     uint32_t *task_cnt, *task_base, *task_next;
     uint4 *task_info;
     ulonglong2 *task_res;  // per task {count, checksum}
@@ -245,7 +246,8 @@ template <int JB, int CAP, bool WRITE, int Q0, int N, int G0 = 0>
 __device__ __forceinline__ void probe_part(const uint64_t (&sv)[N], uint32_t ns,
                                            const uint64_t *tab, const uint16_t *bst, int lg,
                                            uint64_t &cnt, uint64_t &sum, uint64_t *pairs,
-                                           uint64_t &wpos, uint64_t cap, uint32_t tid) {
+                                           uint64_t &wpos, uint64_t cap, uint32_t tid,
+                                           int records = 0) {
     if constexpr (G0 < N) {
         constexpr int G = N - G0 < PGROUP ? N - G0 : PGROUP;
         // groups that no lane of this wave holds are skipped (wave-uniform)
@@ -266,7 +268,7 @@ __device__ __forceinline__ void probe_part(const uint64_t (&sv)[N], uint32_t ns,
                     if (key_of(r[j]) == key_of(sv[G0 + j])) {
                         const uint64_t v = (r[j] << 32) | payload_of(sv[G0 + j]);
                         if (WRITE) {
-                            if (wpos < cap) pairs[wpos] = v;
+                            if (wpos < cap) pairs[wpos] = records ? sv[G0 + j] : v;
                             ++wpos;
                         } else {
                             ++cnt;
@@ -280,7 +282,7 @@ __device__ __forceinline__ void probe_part(const uint64_t (&sv)[N], uint32_t ns,
             if (!more) break;
         }
         probe_part<JB, CAP, WRITE, Q0, N, G0 + PGROUP>(sv, ns, tab, bst, lg, cnt, sum, pairs,
-                                                       wpos, cap, tid);
+                                                       wpos, cap, tid, records);
     }
 }
 
@@ -381,9 +383,9 @@ __global__ __launch_bounds__(JB, 4) void k_join(JoinArgs a) {
                     wpos = s_base + pre;
                     uint64_t dc = 0, dx = 0;
                     probe_part<JB, CAP, true, 0, SA>(sa, sub, tab, bst, lg, dc, dx, a.pairs, wpos,
-                                                    a.pairs_cap, opaque_tid());
+                                                    a.pairs_cap, opaque_tid(), a.records);
                     probe_part<JB, CAP, true, SA, SB>(sb, sub, tab, bst, lg, dc, dx, a.pairs,
-                                                     wpos, a.pairs_cap, opaque_tid());
+                                                     wpos, a.pairs_cap, opaque_tid(), a.records);
                 }
                 cnt += c;
                 sum += x;
@@ -477,6 +479,7 @@ int run_join(const JoinDesc &d, hipStream_t st, const Marks *mk, int m0) {
     a.seg_sum = d.seg_sum;
     a.pairs = d.pairs;
     a.pairs_cap = d.pairs_cap;
+    a.records = d.records ? 1 : 0;
     a.pair_cursor = d.pair_cursor;
     a.task_cnt = d.task_cnt;
     a.task_base = d.task_base;

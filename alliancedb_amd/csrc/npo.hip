@@ -102,7 +102,8 @@ __global__ __launch_bounds__(NBLOCK) void k_npo_build(const uint64_t *R, uint64_
 template <bool WRITE>
 __device__ __forceinline__ void npo_probe_one(uint64_t s, const uint64_t *tab, uint64_t mask,
                                               uint64_t nsent, uint64_t &cnt, uint64_t &sum,
-                                              uint64_t *pairs, uint64_t &wpos, uint64_t cap) {
+                                              uint64_t *pairs, uint64_t &wpos, uint64_t cap,
+                                              int records = 0) {
     const uint32_t k = key_of(s);
     uint64_t h = fmix32(k) & mask;
     for (;;) {
@@ -110,7 +111,7 @@ __device__ __forceinline__ void npo_probe_one(uint64_t s, const uint64_t *tab, u
         if (v == EMPTY_SLOT) break;
         if (key_of(v) == k) {
             if (WRITE) {
-                if (wpos < cap) pairs[wpos] = pair_value(v, s);
+                if (wpos < cap) pairs[wpos] = records ? s : pair_value(v, s);
                 ++wpos;
             } else {
                 ++cnt;
@@ -123,7 +124,7 @@ __device__ __forceinline__ void npo_probe_one(uint64_t s, const uint64_t *tab, u
         const uint64_t pv = pair_value(EMPTY_SLOT, s);
         if (WRITE) {
             for (uint64_t q = 0; q < nsent; ++q) {
-                if (wpos < cap) pairs[wpos] = pv;
+                if (wpos < cap) pairs[wpos] = records ? s : pv;
                 ++wpos;
             }
         } else {
@@ -136,7 +137,8 @@ __device__ __forceinline__ void npo_probe_one(uint64_t s, const uint64_t *tab, u
 template <bool PAIRS>
 __global__ __launch_bounds__(NBLOCK) void k_npo_probe(const uint64_t *S, uint64_t n,
                                                     const uint64_t *tab, uint64_t mask,
-                                                    uint64_t *acc, uint64_t *pairs, uint64_t cap) {
+                                                    uint64_t *acc, uint64_t *pairs, uint64_t cap,
+                                                    int records) {
     __shared__ uint64_t red[NBLOCK / 64];
     __shared__ uint64_t s_base;
     const uint64_t nsent = acc[2];
@@ -158,7 +160,7 @@ __global__ __launch_bounds__(NBLOCK) void k_npo_probe(const uint64_t *S, uint64_
             if (valid && c) {
                 wpos = s_base + pre;
                 uint64_t dc = 0, dx = 0;
-                npo_probe_one<true>(s, tab, mask, nsent, dc, dx, pairs, wpos, cap);
+                npo_probe_one<true>(s, tab, mask, nsent, dc, dx, pairs, wpos, cap, records);
             }
             __syncthreads();
         }
@@ -208,10 +210,11 @@ int run_npo(const NpoDesc &d, hipStream_t st, const Marks *mk, int m0) {
         const unsigned g = grid_for(d.nS, d.num_cu);  // <= num_cu * 16 (npo_acc_words)
         if (d.pairs)
             hipLaunchKernelGGL(k_npo_probe<true>, dim3(g), dim3(NBLOCK), 0, st, d.S, d.nS,
-                               (const uint64_t *)d.table, mask, d.acc, d.pairs, d.pairs_cap);
+                               (const uint64_t *)d.table, mask, d.acc, d.pairs, d.pairs_cap,
+                               d.records ? 1 : 0);
         else
             hipLaunchKernelGGL(k_npo_probe<false>, dim3(g), dim3(NBLOCK), 0, st, d.S, d.nS,
-                               (const uint64_t *)d.table, mask, d.acc, nullptr, (uint64_t)0);
+                               (const uint64_t *)d.table, mask, d.acc, nullptr, (uint64_t)0, 0);
         hipLaunchKernelGGL(k_npo_reduce, dim3(1), dim3(NBLOCK), 0, st, d.acc, g);
     }
     if (mk) mk->mark(m0 + 2);

@@ -199,8 +199,8 @@ static void read_phases(Ctx *c, aj_join_out_t *out, int first, int last) {
 
 // scalars: [0]=count [1]=checksum [2]=pair cursor [3]=task counter (u32 in low half)
 int pro_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, uint64_t nS,
-               const ProConfig &cfg, uint64_t *pairs, uint64_t pairs_cap, aj_join_out_t *out,
-               hipStream_t st) {
+               const ProConfig &cfg, uint64_t *pairs, uint64_t pairs_cap, bool records,
+               aj_join_out_t *out, hipStream_t st) {
     ProPlan p = plan_pro(nR, nS, cfg, ctx->num_cu);
     AJ_TRY(ctx->plan.ensure(p.bytes));
     AJ_TRY(ctx->scan_tmp.ensure(p.scan_bytes));
@@ -359,6 +359,7 @@ int pro_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
     j.seg_sum = scal + 1;
     j.pairs = pairs;
     j.pairs_cap = pairs_cap;
+    j.records = records;
     j.pair_cursor = scal + 2;
     j.task_cnt = at<uint32_t>(B, p.o_tcnt);
     j.task_base = at<uint32_t>(B, p.o_tbase);
@@ -385,7 +386,8 @@ int pro_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
 }
 
 int npo_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, uint64_t nS,
-               uint64_t *pairs, uint64_t pairs_cap, aj_join_out_t *out, hipStream_t st) {
+               uint64_t *pairs, uint64_t pairs_cap, bool records, aj_join_out_t *out,
+               hipStream_t st) {
     uint64_t cap = 2;
     while (cap <= 2 * nR) cap <<= 1;
     AJ_TRY(ctx->table.ensure(cap * 8));
@@ -403,6 +405,7 @@ int npo_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
     d.acc = acc;
     d.pairs = pairs;
     d.pairs_cap = pairs_cap;
+    d.records = records;
     d.num_cu = ctx->num_cu;
     Marks mk;
     const Marks *mkp = nullptr;

@@ -47,8 +47,9 @@ int aj_last_status(void);
  * Host relations are uploaded, joined on the current default device and
  * never modified (stricter than the reference, whose two-pass PRO overwrites
  * its inputs, parallel_radix_join.cpp:1176-1181; no RELATION_PADDING needed).
- * Returns a malloc'ed result the caller frees with free(), or NULL on error
- * (see aj_last_error).  Extended results: aj_result_ext().
+ * Returns a malloc'ed result, or NULL on error (see aj_last_error).
+ * Extended results: aj_result_ext().  Release it with aj_free_result(), or
+ * with free() when it carries no record lists (option "materialize" 0 or 1).
  */
 result_t *PRO(relation_t *relR, relation_t *relS, int nthreads);
 
@@ -79,18 +80,45 @@ typedef struct aj_result_t {
 const aj_result_t *aj_result_ext(const result_t *r);
 
 /*
+ * Record lists in the reference's format (option "materialize" = 2): the
+ * result's resultlist has max(nthreads, 1) separately malloc'ed entries, as the
+ * reference's joins return (parallel_radix_join.cpp:1522-1524); entry i has
+ * threadid i and its results point at an aj_chainedtuplebuffer_t (types.h).
+ * Each match contributes one record, the matching S tuple <S.key, S.payloadID>
+ * (parallel_radix_join.cpp:333-338, common_functions.cpp:252-257).  All
+ * records are in entry 0; the other entries are empty chains.  Record order is
+ * unspecified, as across the reference's threads.
+ *
+ * aj_free_result releases a result of PRO/NPO with everything it owns (record
+ * chains, resultlist, pairs); NULL is a no-op.  Callers written for the
+ * reference, which frees resultlist and the result (benchmark.cpp:255-268) and
+ * not the chains, keep working but leak the chains.
+ */
+void aj_free_result(result_t *r);
+
+/*
+ * Writes a result's records as the reference's write_result_relation does
+ * (tuple_buffer.h:155-236, unsorted variant; called for "Out.tbl" at
+ * benchmark.cpp:262-265): per resultlist entry, newest record first, one
+ * "%d %d\n" line of key and payloadID.  AJ_ERR_STATE if the result counts
+ * matches but carries no record lists.
+ */
+int aj_write_result_file(const result_t *r, const char *path);
+
+/*
  * Process-wide options of the drop-in entry points (the reference sets the
  * equivalents at compile time: JOIN_RESULT_MATERIALIZE, NUM_RADIX_BITS,
  * NUM_PASSES; common_functions.h:18, prj_params.h:16-21).
- *   "materialize"  0/1   also return the pair list (default 0)
+ *   "materialize"  0..2  1: also return the pair list (aj_result_t.pairs);
+ *                        2: return the reference's per-thread record lists
+ *                        (result_t.resultlist, see aj_free_result); default 0
  *   "radix_bits"   0..24 total radix bits (0 = tuned default)
  *   "passes"       0..2  partition passes (0 = tuned default)
  *   "device"       >=0   HIP device of the default context
  *   "kernel_timing" 0/1  record per-kernel device times (aj_join_out_t.phase_ms)
  *   "cap"          0, 2048, 4096, 8192, 8704, 12288  R tuples per LDS build
  *                        round (0 = default 8704)
- *   "split"        >=0   S tuples per join task (0 = cap; larger values are clamped
- *                        to cap so that a task's S slice fits the registers)
+ *   "split"        >=0   S tuples per join task, probed in slices of cap (0 = 8 x cap)
  *   "part_block"   0, 256, 512  lanes per partition work-group (0 = 512)
  *   "part_chunk"   multiple of 8192  tuples per partition work-group (0 = derived)
  *   "part_hist2d"  0/1   two-pass PRO: pass 2 from a (d1, d2) histogram taken in
@@ -127,9 +155,12 @@ typedef struct aj_join_opts_t {
     int radix_bits;    /* PRO: total radix bits, 0 = tuned */
     int passes;        /* PRO: 1 or 2, 0 = tuned */
     int cap;           /* PRO: R tuples per LDS build round, 0 = default */
-    uint32_t split;    /* PRO: S tuples per join task, 0 = cap, at most cap */
+    uint32_t split;    /* PRO: S tuples per join task (probed in slices of cap), 0 = 8 x cap */
     uint64_t *pairs;   /* device buffer for pairs, NULL = count/checksum only */
     uint64_t pairs_capacity;
+    int records;       /* 1: write the reference's result records <S.key, S.payloadID>
+                          (the matching S tuple, one per match) into pairs instead of
+                          (R.payloadID << 32 | S.payloadID); count and checksum unchanged */
 } aj_join_opts_t;
 
 /* Per-kernel device times of the last join, filled when the option

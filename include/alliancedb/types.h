@@ -9,6 +9,7 @@
  *   relation_payload_t hashing/utils/types.h:84-88
  *   threadresult_t     hashing/utils/types.h:91-95
  *   result_t           hashing/utils/types.h:98-102
+ *   tuplebuffer_t, chainedtuplebuffer_t   hashing/utils/tuple_buffer.h:31-47
  *
  * Read as a little-endian uint64, a tuple is (key << 32) | payloadID.
  * Keys and payloads are compared and combined as 32-bit bit patterns; the
@@ -50,9 +51,34 @@ typedef struct threadresult_t {
 
 typedef struct result_t {
     int64_t totalresults;       /* number of matching (r, s) pairs */
-    threadresult_t *resultlist; /* NULL: this library never hands out per-thread lists */
+    threadresult_t *resultlist; /* NULL unless option "materialize" is 2 (see aj.h) */
     int nthreads;
 } result_t;
+
+/*
+ * Chained result buffers, layout-identical to the reference's
+ * (hashing/utils/tuple_buffer.h:31-47): threadresult_t.results points at an
+ * aj_chainedtuplebuffer_t when a result carries record lists.  Buffers hold
+ * AJ_CHAINEDBUFF_NUMTUPLESPERBUF tuples (CHAINEDBUFF_NUMTUPLESPERBUF,
+ * tuple_buffer.h:23); buf is the newest buffer and holds writepos tuples, each
+ * older buffer (next) is full.
+ */
+#define AJ_CHAINEDBUFF_NUMTUPLESPERBUF (1024 * 1024)
+
+typedef struct aj_tuplebuffer_t {
+    tuple_t *tuples;
+    struct aj_tuplebuffer_t *next;
+} aj_tuplebuffer_t;
+
+typedef struct aj_chainedtuplebuffer_t {
+    aj_tuplebuffer_t *buf;
+    aj_tuplebuffer_t *readcursor;
+    aj_tuplebuffer_t *writecursor;
+    uint32_t writepos;
+    uint32_t readpos;
+    uint32_t readlen;
+    uint32_t numbufs;
+} aj_chainedtuplebuffer_t;
 
 /*
  * 12-byte stream tuple used by the window paths (batched windows, serving).

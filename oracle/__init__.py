@@ -176,3 +176,13 @@ def ref_join(algo: str, R, S, s_list: bool = False):
         raise RuntimeError(f"ref_join failed: {rc}")
     lst = None if buf is None else np.sort(buf[:n.value])
     return int(cnt.value), lst
+
+
+def ref_write_result(res_ptr, path: str) -> None:
+    """The reference's own write_result_relation (Out.tbl writer) on a result_t*
+    (ctypes pointer or address) whose resultlist holds chained tuple buffers."""
+    f = ref().ref_write_result
+    f.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+    f.restype = ctypes.c_int
+    if f(ctypes.cast(res_ptr, ctypes.c_void_p), os.fsencode(path)):
+        raise RuntimeError("ref_write_result failed")

@@ -75,4 +75,13 @@ int ref_join(int algo, const void *R, uint64_t nR, const void *S, uint64_t nS,
     return rc;
 }
 
+// The reference's own write_result_relation (hashing/utils/tuple_buffer.h:155-236,
+// unsorted variant) on a result_t whose resultlist entries point at chained
+// tuple buffers -- the Out.tbl writer of benchmark.cpp:262-265.
+int ref_write_result(result_t *res, const char *path) {
+    if (!res || !path) return -1;
+    write_result_relation(res, const_cast<char *>(path));
+    return 0;
+}
+
 }  // extern "C"

@@ -158,6 +158,100 @@ def test_golden_fixtures(aj, algo):
         aj.set_option("materialize", 0)
 
 
+def _u64(a):
+    return np.ascontiguousarray(a).view(np.uint64)
+
+
+def _parse_tbl(path):
+    """Out.tbl lines "key payloadID" -> uint64 records (key << 32 | payloadID)."""
+    v = np.array(open(path).read().split(), dtype=np.int64).reshape(-1, 2)
+    k = v[:, 0].astype(np.int32).view(np.uint32).astype(np.uint64)
+    p = v[:, 1].astype(np.int32).view(np.uint32).astype(np.uint64)
+    return (k << np.uint64(32)) | p
+
+
+@pytest.mark.parametrize("algo", ["PRO", "NPO"])
+def test_golden_records(aj, algo, tmp_path):
+    """materialize = 2: per-match <S.key, S.payloadID> records in the reference's
+    chained result lists vs the reference's own materialised records (s_digest,
+    from its RJ_st / NPO_st), and the Out.tbl text written from them."""
+    import xxhash
+    aj.set_option("materialize", 2)
+    try:
+        for name, R, S, e in _fixtures():
+            print(f"case {algo} {name}", flush=True)
+            f = tmp_path / f"{algo}_{name}.tbl"
+            got = getattr(aj, algo)(R, S, 4, result_file=str(f))
+            assert got.count == e["count"] and got.checksum == int(e["checksum"]), name
+            assert got.pairs is None
+            assert [c[0] for c in got.chains] == [0, 1, 2, 3], name
+            assert all(len(r) == 0 for r in got.records[1:]), name
+            rec = got.records[0]
+            assert len(rec) == e["count"], name
+            assert xxhash.xxh64(np.sort(rec).tobytes()).hexdigest() == e["s_digest"], name
+            assert np.array_equal(_parse_tbl(f), rec[::-1]), name
+    finally:
+        aj.set_option("materialize", 0)
+
+
+@pytest.mark.parametrize("algo", ["PRO", "NPO"])
+def test_records_multi_buffer(aj, oracle, algo, tmp_path):
+    """A result of 2.6M records spans three 1M-tuple buffers: the chain layout,
+    the records (against S rows picked by the oracle's pair list) and the Out.tbl
+    bytes against the reference's own write_result_relation on the same result."""
+    import ctypes
+    from alliancedb_amd import _lib
+    n = 1 << 21
+    R = aj.gen_unique(n, 11)
+    S = aj.gen_uniform(2600000, 1, n, 12)  # payloadID = row index
+    pairs = oracle.pairs_sortmerge(R, S)
+    want = np.sort(_u64(S)[(pairs & np.uint64(0xFFFFFFFF)).astype(np.int64)])
+    aj.set_option("materialize", 2)
+    rr = _lib.relation_t(R.ctypes.data, len(R), None)
+    rs = _lib.relation_t(S.ctypes.data, len(S), None)
+    print(f"case {algo} records n={n}", flush=True)
+    try:
+        res = getattr(_lib.lib, algo)(ctypes.byref(rr), ctypes.byref(rs), 2)
+    finally:
+        aj.set_option("materialize", 0)
+    assert res, aj.last_error()
+    try:
+        assert res.contents.totalresults == len(pairs) and res.contents.nthreads == 2
+        tl = ctypes.cast(res.contents.resultlist, ctypes.POINTER(_lib.threadresult_t))
+        cb = ctypes.cast(tl[0].results, ctypes.POINTER(_lib.aj_chainedtuplebuffer_t)).contents
+        per = _lib.CHAINEDBUFF_NUMTUPLESPERBUF
+        assert cb.numbufs == 3 and cb.writepos == len(pairs) - 2 * per
+        assert tl[1].nresults == 0 and tl[1].threadid == 1
+        rec = _lib._read_chain(cb, len(pairs))
+        assert np.array_equal(np.sort(rec), want)
+        ours = tmp_path / "ours.tbl"
+        _lib.check(_lib.lib.aj_write_result_file(res, str(ours).encode()))
+        assert np.array_equal(_parse_tbl(ours), rec[::-1])
+        if oracle.ref_available():
+            theirs = tmp_path / "ref.tbl"
+            oracle.ref_write_result(res, str(theirs))
+            assert ours.read_bytes() == theirs.read_bytes()
+    finally:
+        _lib.lib.aj_free_result(res)
+
+
+def test_driver_persists_result(aj, oracle, tmp_path):
+    """ajbench --out writes the reference's Out.tbl: one "key payloadID" line per
+    match, the S tuple of the match."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(aj.LIB_PATH), "ajbench")
+    out = tmp_path / "Out.tbl"
+    r = subprocess.run([exe, "-a", "PRO", "-r", "100000", "-s", "150000", "-n", "3",
+                        "--out", str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    R = aj.gen_unique(100000, 12345)
+    S = aj.gen_uniform(150000, 1, 100000, 54321)
+    pairs = oracle.pairs_sortmerge(R, S)
+    want = np.sort(_u64(S)[(pairs & np.uint64(0xFFFFFFFF)).astype(np.int64)])
+    assert np.array_equal(np.sort(_parse_tbl(out)), want)
+
+
 @pytest.mark.parametrize("algo", ["PRO", "NPO"])
 def test_driver_runs_join(aj, oracle, algo):
     """The C++ driver joins generated relations and prints the reference's result line."""
