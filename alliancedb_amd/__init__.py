@@ -8,6 +8,6 @@ from ._lib import (  # noqa: F401
     AJ_ERR_NOMEM, AJ_ERR_STATE, AJ_OK, LIB_PATH, NPO, PRO, STREAM_DTYPE, TUPLE_DTYPE,
     AjError, BatchPlan, Context, DeviceArray, JoinResult, Server, as_tuples, check,
     device_sync, gen_stream_tuples, gen_uniform, gen_unique, gen_window_streams, gen_zipf,
-    get_option, last_error, lib, set_option, tune_clear, tune_get, tune_load, tune_pro,
-    tune_put, tune_save,
+    get_option, last_error, lib, load_relation, set_option, tune_clear, tune_get, tune_load,
+    tune_pro, tune_put, tune_save,
 )

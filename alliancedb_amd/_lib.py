@@ -32,6 +32,10 @@ AJ_ALGO_PRO = 0
 AJ_ALGO_NPO = 1
 
 
+class relation_payload_t(ctypes.Structure):
+    _fields_ = [("ts", ctypes.POINTER(ctypes.c_uint64)), ("num_tuples", ctypes.c_uint64)]
+
+
 class relation_t(ctypes.Structure):
     _fields_ = [("tuples", ctypes.c_void_p), ("num_tuples", ctypes.c_uint64),
                 ("payload", ctypes.c_void_p)]
@@ -151,6 +155,9 @@ def _load() -> ctypes.CDLL:
         "NPO": (c.POINTER(result_t), [c.POINTER(relation_t), c.POINTER(relation_t), c.c_int]),
         "aj_result_ext": (c.POINTER(aj_result_t), [c.POINTER(result_t)]),
         "aj_free_result": (None, [c.POINTER(result_t)]),
+        "aj_load_relation": (c.c_int, [c.POINTER(relation_t), c.POINTER(relation_payload_t),
+                                       c.c_int32, c.c_int32, c.c_char_p, c.c_uint64,
+                                       c.c_uint32]),
         "aj_write_result_file": (c.c_int, [c.POINTER(result_t), c.c_char_p]),
         "aj_set_option": (c.c_int, [c.c_char_p, c.c_int64]),
         "aj_get_option": (c.c_int64, [c.c_char_p]),
@@ -301,6 +308,25 @@ def _dropin(fn, R: np.ndarray, S: np.ndarray, nthreads: int,
                           float(ext.device_ms), float(ext.total_ms), records, chains)
     finally:
         lib.aj_free_result(res)
+
+
+def load_relation(path: str, num_tuples: int, keyby: int = 0, ts_key: int = 0,
+                  partitions: int = 1):
+    """A relation from the reference's text format (aj_load_relation; see aj.h).
+    Returns (tuples as TUPLE_DTYPE, timestamps as uint64)."""
+    rel = relation_t(None, 0, None)
+    pl = relation_payload_t(None, 0)
+    check(lib.aj_load_relation(ctypes.byref(rel), ctypes.byref(pl), int(keyby), int(ts_key),
+                               os.fsencode(path), int(num_tuples), int(partitions)))
+    n = int(rel.num_tuples)
+    try:
+        tup = np.ctypeslib.as_array(ctypes.cast(rel.tuples, ctypes.POINTER(ctypes.c_uint64)),
+                                    shape=(max(n, 1),))[:n].copy().view(TUPLE_DTYPE)
+        ts = np.ctypeslib.as_array(pl.ts, shape=(max(n, 1),))[:n].copy()
+    finally:
+        _libc.free(rel.tuples)
+        _libc.free(ctypes.cast(pl.ts, ctypes.c_void_p))
+    return tup, ts
 
 
 def PRO(R: np.ndarray, S: np.ndarray, nthreads: int = 1,

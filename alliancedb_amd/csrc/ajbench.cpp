@@ -4,8 +4,8 @@
 //
 //   ajbench -a PRO|NPO [-r |R|] [-s |S|] [-x r_seed] [-y s_seed]
 //           [-d key distribution: 0 unique/uniform, 2 zipf] [-z zipf theta]
-//           [-w window -e step -l interval] [-n nthreads] [-m] [-R repeats]
-//           [--pairs] [--out FILE]
+//           [-w window -e step -l interval] [-n nthreads] [-m] [--repeats n]
+//           [--pairs] [--out FILE] [-R fileR -J keyR -L tsR] [-S fileS -K keyS -M tsS]
 //   ajbench -a SERVE [--load f] [--peak Mtps] [--seconds s] [--streams n]
 //           [--budget bytes] [--slots n] [--batch-windows n] [--clients n]
 //           [--verify n] [--median n] [--sigma s] [--cap-tuples n]
@@ -20,7 +20,10 @@
 // records (<S.key, S.payloadID> per match in chained per-thread buffers, as a
 // JOIN_RESULT_MATERIALIZE build does); --out FILE also writes them as the
 // reference's Out.tbl (benchmark.cpp:262-265); --pairs returns the
-// (R.payloadID, S.payloadID) pair list instead.
+// (R.payloadID, S.payloadID) pair list instead.  -R/-S load R or S from the
+// reference's text files (main.cpp:612-628; key column -J/-K, timestamp column
+// -L/-M, |R| and |S| lines from -r/-s) through aj_load_relation, with the
+// reference's placement over -n partitions (benchmark.cpp:183-205).
 #include <getopt.h>
 
 #include <chrono>
@@ -75,12 +78,14 @@ struct params {
     int materialize = 0;
     int repeats = 1;
     const char *out_file = nullptr;
+    const char *file_r = nullptr, *file_s = nullptr;
+    int rkey = 0, skey = 0, rts = 0, sts = 0;
 };
 
 void usage(const char *prog) {
     printf("Usage: %s -a <PRO|NPO> [-r R] [-s S] [-x seed] [-y seed] [-d 0|2] [-z theta]\n"
-           "          [-w window -e step -l interval] [-n nthreads] [-m] [-R repeats]\n"
-           "          [--pairs] [--out FILE]\n"
+           "          [-w window -e step -l interval] [-n nthreads] [-m] [--repeats n]\n"
+           "          [--pairs] [--out FILE] [-R fileR -J keyR -L tsR] [-S fileS -K keyS -M tsS]\n"
            "       %s -a SERVE [--load f] [--peak Mtps] [--seconds s] [--streams n]\n"
            "          [--budget bytes] [--slots n] [--batch-windows n] [--clients n]\n"
            "          [--verify n] [--median n] [--sigma s] [--cap-tuples n] [--domain n]\n"
@@ -90,7 +95,7 @@ void usage(const char *prog) {
 
 enum { OPT_LOAD = 1000, OPT_PEAK, OPT_SECONDS, OPT_STREAMS, OPT_BUDGET, OPT_SLOTS, OPT_BWIN,
        OPT_CLIENTS, OPT_VERIFY, OPT_MEDIAN, OPT_SIGMA, OPT_CAPT, OPT_DOMAIN, OPT_BTUP,
-       OPT_PAIRS, OPT_OUT };
+       OPT_PAIRS, OPT_OUT, OPT_REPEATS };
 
 }  // namespace
 
@@ -127,11 +132,17 @@ int main(int argc, char **argv) {
                                 {"interval", required_argument, 0, 'l'},
                                 {"nthreads", required_argument, 0, 'n'},
                                 {"materialize", no_argument, 0, 'm'},
-                                {"repeats", required_argument, 0, 'R'},
+                                {"repeats", required_argument, 0, OPT_REPEATS},
+                                {"loadfileR", required_argument, 0, 'R'},
+                                {"loadfileS", required_argument, 0, 'S'},
+                                {"rkey", required_argument, 0, 'J'},
+                                {"skey", required_argument, 0, 'K'},
+                                {"rts", required_argument, 0, 'L'},
+                                {"sts", required_argument, 0, 'M'},
                                 {"help", no_argument, 0, 'h'},
                                 {0, 0, 0, 0}};
     int c;
-    while ((c = getopt_long(argc, argv, "a:r:s:x:y:d:z:w:e:l:n:mR:h", longopts, nullptr)) != -1) {
+    while ((c = getopt_long(argc, argv, "a:r:s:x:y:d:z:w:e:l:n:mR:S:J:K:L:M:h", longopts, nullptr)) != -1) {
         switch (c) {
             case 'a': {
                 if (strcmp(optarg, "SERVE") == 0) {
@@ -175,7 +186,13 @@ int main(int argc, char **argv) {
             case 'm': p.materialize = 2; break;
             case OPT_PAIRS: p.materialize = 1; break;
             case OPT_OUT: p.out_file = optarg; p.materialize = 2; break;
-            case 'R': p.repeats = atoi(optarg); break;
+            case OPT_REPEATS: p.repeats = atoi(optarg); break;
+            case 'R': p.file_r = optarg; break;
+            case 'S': p.file_s = optarg; break;
+            case 'J': p.rkey = atoi(optarg); break;
+            case 'K': p.skey = atoi(optarg); break;
+            case 'L': p.rts = atoi(optarg); break;
+            case 'M': p.sts = atoi(optarg); break;
             case 'h': usage(argv[0]); return EXIT_SUCCESS;
             default: usage(argv[0]); return EXIT_FAILURE;
         }
@@ -196,24 +213,33 @@ int main(int argc, char **argv) {
         fprintf(stderr, "[ERROR] key distribution must be 0 (unique/uniform) or 2 (zipf)\n");
         return EXIT_FAILURE;
     }
-    printf("[INFO ] Creating relations R=%llu S=%llu (seeds %llu/%llu)\n",
-           (unsigned long long)p.r_size, (unsigned long long)p.s_size,
-           (unsigned long long)p.r_seed, (unsigned long long)p.s_seed);
-    std::vector<tuple_t> R(p.r_size), S(p.s_size);
-    int rc = aj_gen_unique(R.data(), p.r_size, p.r_seed);
-    if (rc == AJ_OK) {
+    printf("[INFO ] %s relations R=%llu S=%llu (seeds %llu/%llu)\n",
+           p.file_r || p.file_s ? "Loading" : "Creating", (unsigned long long)p.r_size,
+           (unsigned long long)p.s_size, (unsigned long long)p.r_seed,
+           (unsigned long long)p.s_seed);
+    std::vector<tuple_t> R(p.file_r ? 0 : p.r_size), S(p.file_s ? 0 : p.s_size);
+    relation_t relR = {R.data(), p.r_size, nullptr};
+    relation_t relS = {S.data(), p.s_size, nullptr};
+    relation_payload_t plR = {nullptr, 0}, plS = {nullptr, 0};
+    const uint32_t parts = (uint32_t)(p.nthreads > 0 ? p.nthreads : 1);
+    int rc = AJ_OK;
+    if (p.file_r)
+        rc = aj_load_relation(&relR, &plR, p.rkey, p.rts, p.file_r, p.r_size, parts);
+    else
+        rc = aj_gen_unique(R.data(), p.r_size, p.r_seed);
+    if (rc == AJ_OK && p.file_s) {
+        rc = aj_load_relation(&relS, &plS, p.skey, p.sts, p.file_s, p.s_size, parts);
+    } else if (rc == AJ_OK) {
         if (p.key_distribution == 2)
             rc = aj_gen_zipf(S.data(), p.s_size, (uint32_t)p.r_size, p.skew, p.s_seed);
         else
             rc = aj_gen_uniform(S.data(), p.s_size, 1, (uint32_t)(p.r_size ? p.r_size : 1), p.s_seed);
     }
     if (rc != AJ_OK) {
-        fprintf(stderr, "[ERROR] generator failed (%d)\n", rc);
+        fprintf(stderr, "[ERROR] input relations: %s\n", aj_last_error());
         return EXIT_FAILURE;
     }
     aj_set_option("materialize", p.materialize);
-    relation_t relR = {R.data(), p.r_size, nullptr};
-    relation_t relS = {S.data(), p.s_size, nullptr};
     printf("[INFO ] Running join algorithm %s ...\n", p.algo->name);
     for (int rep = 0; rep < p.repeats; ++rep) {
         auto t0 = std::chrono::steady_clock::now();
@@ -237,6 +263,14 @@ int main(int argc, char **argv) {
             }
         }
         aj_free_result(res);
+    }
+    if (p.file_r) {
+        free(relR.tuples);
+        free(plR.ts);
+    }
+    if (p.file_s) {
+        free(relS.tuples);
+        free(plS.ts);
     }
     printf("[INFO ] DONE.\n");
     return EXIT_SUCCESS;

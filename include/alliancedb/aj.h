@@ -129,6 +129,31 @@ int aj_write_result_file(const result_t *r, const char *path);
 int aj_set_option(const char *name, int64_t value);
 int64_t aj_get_option(const char *name);
 
+/* ------------------------------------------------------------ input files */
+
+/*
+ * Loads a relation from the reference's text format.  Replaces
+ * `int load_relation(relation_t*, relation_payload_t*, int32_t keyby,
+ * int32_t tsKey, char *filename, uint64_t num_tuples, uint32_t partitions)`
+ * (hashing/utils/generator.h:124, generator.cpp:645-666, read_relation
+ * :899-1027, smooth :1050-1067), the loader behind the driver's -R/-S files:
+ *   - fields are separated by the first ',' or '|' found in the file; line i
+ *     (no header) gives the int32 key in field keyby and, when tsKey != 0, the
+ *     timestamp field tsKey * 2.1e6 (cycles at 2.1 GHz of a millisecond column);
+ *   - runs of equal timestamps are spread linearly towards the next value;
+ *   - line i lands in partition i % partitions of n / partitions tuples (the
+ *     last partition takes the remainder and any overflow), payloadID = its
+ *     index, pl->ts[index] = its timestamp.
+ * rel->tuples is allocated here (64-byte aligned; release with free()).  pl
+ * may be NULL (timestamps dropped); if pl->ts is NULL it is allocated here.
+ * Returns AJ_OK or an aj_status; inputs on which the reference's behaviour is
+ * undefined (no separator, too few lines, a missing or non-numeric field, a
+ * key outside int32, a negative timestamp, smoothing overflow) are AJ_ERR_ARG.
+ * With tsKey == 0 every timestamp is 0 (the reference leaves them unset).
+ */
+int aj_load_relation(relation_t *rel, relation_payload_t *pl, int32_t keyby, int32_t tsKey,
+                     const char *filename, uint64_t num_tuples, uint32_t partitions);
+
 /* ------------------------------------------------------------ device API */
 
 /* Device memory helpers for FFI callers that have no HIP binding of their own

@@ -186,3 +186,72 @@ def ref_write_result(res_ptr, path: str) -> None:
     f.restype = ctypes.c_int
     if f(ctypes.cast(res_ptr, ctypes.c_void_p), os.fsencode(path)):
         raise RuntimeError("ref_write_result failed")
+
+
+def ref_load_relation(path: str, keyby: int, ts_key: int, n: int, partitions: int):
+    """The reference's own load_relation on a text file -> (tuples, ts)."""
+    f = ref().ref_load_relation
+    f.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32,
+                  ctypes.c_void_p, ctypes.c_void_p]
+    f.restype = ctypes.c_int
+    tup = np.zeros(max(n, 1), np.uint64)
+    ts = np.zeros(max(n, 1), np.uint64)
+    rc = f(os.fsencode(path), keyby, ts_key, n, partitions, tup.ctypes.data, ts.ctypes.data)
+    if rc:
+        raise RuntimeError(f"ref_load_relation failed: {rc}")
+    return tup[:n].view(TUPLE_DTYPE), ts[:n]
+
+
+def load_relation_text(text: str, keyby: int, ts_key: int, n: int, partitions: int):
+    """Pure-Python restatement of the reference's read_relation + smooth
+    (hashing/utils/generator.cpp:899-1027, 1050-1067) for small inputs.
+    Returns (keys int32 array, payloadIDs, ts uint64 array) in placement order."""
+    sep = next((c for c in text if c in ",|"), None)
+    if sep is None:
+        raise ValueError("no separator")
+    lines = text.split("\n")
+    if text.endswith("\n"):
+        lines = lines[:-1]
+    if len(lines) < n:
+        raise ValueError("too few lines")
+    keys, ts = [], []
+    for ln in lines[:n]:
+        f = ln.split(sep)
+        keys.append(int(_c_long_prefix(f[keyby])))
+        ts.append(int(float(_c_long_prefix(f[ts_key])) * 2.1 * 1e6) if ts_key else 0)
+    if n:  # smooth (generator.cpp:1050-1067)
+        run, cur, head = 0, 0, ts[0]
+        for k in range(n):
+            if ts[k] == head:
+                run += 1
+                continue
+            inner = float((ts[k] - ts[cur]) % 2**64) / float(run)
+            for j in range(run):
+                ts[cur + j] = int(float(ts[cur + j]) + j * inner)
+            head, cur, run = ts[k], k, 1
+    per = n // partitions
+    fill = [0] * partitions
+    out_k = np.zeros(n, np.int32)
+    out_p = np.zeros(n, np.int32)
+    out_t = np.zeros(n, np.uint64)
+    for k in range(n):  # placement (generator.cpp:1011-1024)
+        p = k % partitions
+        end = n if p == partitions - 1 else per * (p + 1)
+        if per * p + fill[p] == end:
+            p = partitions - 1
+        idx = per * p + fill[p]
+        fill[p] += 1
+        out_k[idx], out_p[idx], out_t[idx] = keys[k], idx, ts[k]
+    return out_k, out_p, out_t
+
+
+def _c_long_prefix(tok: str) -> str:
+    """The prefix strtol(.., 10) parses: white space, sign, decimal digits."""
+    s = tok.lstrip(" \t\n\v\f\r")
+    i = 1 if s[:1] in "+-" else 0
+    j = i
+    while j < len(s) and s[j].isdigit():
+        j += 1
+    if j == i:
+        raise ValueError(f"not a number: {tok!r}")
+    return s[:j]

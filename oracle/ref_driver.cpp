@@ -75,6 +75,31 @@ int ref_join(int algo, const void *R, uint64_t nR, const void *S, uint64_t nS,
     return rc;
 }
 
+// The reference's own load_relation (hashing/utils/generator.cpp:645-666) on a
+// text file: the n tuples and timestamps it produces are copied out.  Only for
+// tsKey != 0 (with tsKey == 0 the reference leaves the timestamps unset) and
+// files that hold a ',' or '|' and at least n well-formed lines.
+int ref_load_relation(const char *path, int keyby, int tsKey, uint64_t n, uint32_t partitions,
+                      uint64_t *tuples_out, uint64_t *ts_out) {
+    if (!path || tsKey == 0 || partitions == 0 || !tuples_out || !ts_out) return -1;
+    relation_t rel;
+    relation_payload_t pl;
+    memset(&rel, 0, sizeof(rel));
+    pl.ts = (uint64_t *)malloc((n ? n : 1) * sizeof(uint64_t));
+    pl.num_tuples = n;
+    if (load_relation(&rel, &pl, keyby, tsKey, const_cast<char *>(path), n, partitions)) {
+        free(pl.ts);
+        return -2;
+    }
+    if (n) {
+        memcpy(tuples_out, rel.tuples, n * sizeof(tuple_t));
+        memcpy(ts_out, pl.ts, n * sizeof(uint64_t));
+    }
+    free(rel.tuples);
+    free(pl.ts);
+    return 0;
+}
+
 // The reference's own write_result_relation (hashing/utils/tuple_buffer.h:155-236,
 // unsorted variant) on a result_t whose resultlist entries point at chained
 // tuple buffers -- the Out.tbl writer of benchmark.cpp:262-265.

@@ -260,7 +260,7 @@ def test_driver_runs_join(aj, oracle, algo):
     import subprocess
     exe = os.path.join(os.path.dirname(aj.LIB_PATH), "ajbench")
     r = subprocess.run([exe, "-a", algo, "-r", "200000", "-s", "300000", "-d", "2", "-z", "1.0",
-                        "-R", "2"], capture_output=True, text=True, timeout=120)
+                        "--repeats", "2"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     got = re.findall(r"Results = (\d+)\. checksum = (\d+)", r.stdout)
     assert len(got) == 2
