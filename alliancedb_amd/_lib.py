@@ -70,10 +70,18 @@ class aj_chainedtuplebuffer_t(ctypes.Structure):
 CHAINEDBUFF_NUMTUPLESPERBUF = 1024 * 1024
 
 
+class aj_timing_summary_t(ctypes.Structure):
+    _fields_ = [("progress_ms", ctypes.c_double * 5), ("latency95_ms", ctypes.c_double),
+                ("latency99_ms", ctypes.c_double), ("samples", ctypes.c_uint64)]
+
+
 class aj_result_t(ctypes.Structure):
     _fields_ = [("base", result_t), ("checksum", ctypes.c_uint64), ("npairs", ctypes.c_uint64),
                 ("pairs", ctypes.POINTER(ctypes.c_uint64)), ("status", ctypes.c_int),
-                ("device_ms", ctypes.c_float), ("total_ms", ctypes.c_float)]
+                ("device_ms", ctypes.c_float), ("total_ms", ctypes.c_float),
+                ("ntimed", ctypes.c_uint64), ("timed", ctypes.POINTER(ctypes.c_uint64)),
+                ("clock_hz", ctypes.c_double), ("algo", ctypes.c_int),
+                ("phase_ms", ctypes.c_float * 12)]
 
 
 class aj_join_opts_t(ctypes.Structure):
@@ -155,6 +163,10 @@ def _load() -> ctypes.CDLL:
         "NPO": (c.POINTER(result_t), [c.POINTER(relation_t), c.POINTER(relation_t), c.c_int]),
         "aj_result_ext": (c.POINTER(aj_result_t), [c.POINTER(result_t)]),
         "aj_free_result": (None, [c.POINTER(result_t)]),
+        "aj_write_timing_files": (c.c_int, [c.POINTER(result_t), c.POINTER(relation_t),
+                                            c.POINTER(relation_t), c.c_char_p, c.c_char_p,
+                                            c.c_int, c.c_long, c.c_int,
+                                            c.POINTER(aj_timing_summary_t)]),
         "aj_load_relation": (c.c_int, [c.POINTER(relation_t), c.POINTER(relation_payload_t),
                                        c.c_int32, c.c_int32, c.c_char_p, c.c_uint64,
                                        c.c_uint32]),
@@ -258,6 +270,11 @@ class JoinResult:
     records: Optional[list] = None
     # per resultlist entry: (threadid, numbufs, writepos)
     chains: Optional[list] = None
+    # option timing = 1: (device clock ticks since the join started << 32 | S.payloadID)
+    # per match, the clock's rate and the per-phase device times (enum aj_phase)
+    timed: Optional[np.ndarray] = None
+    clock_hz: float = 0.0
+    phase_ms: Optional[list] = None
 
 
 def _read_chain(cb: aj_chainedtuplebuffer_t, n: int) -> np.ndarray:
@@ -304,8 +321,12 @@ def _dropin(fn, R: np.ndarray, S: np.ndarray, nthreads: int,
                 chains.append((int(tl[i].threadid), int(cb.numbufs), int(cb.writepos)))
         if result_file is not None:
             check(lib.aj_write_result_file(res, os.fsencode(result_file)))
+        timed = None
+        if ext.ntimed:
+            timed = np.ctypeslib.as_array(ext.timed, shape=(ext.ntimed,)).copy()
         return JoinResult(int(ext.base.totalresults), int(ext.checksum), pairs,
-                          float(ext.device_ms), float(ext.total_ms), records, chains)
+                          float(ext.device_ms), float(ext.total_ms), records, chains, timed,
+                          float(ext.clock_hz), list(ext.phase_ms))
     finally:
         lib.aj_free_result(res)
 

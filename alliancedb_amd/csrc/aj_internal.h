@@ -78,6 +78,8 @@ struct Ctx {
     hipEvent_t evk[AJ_NPHASES + 1] = {};  // per-kernel timing (option kernel_timing)
 };
 bool kernel_timing_enabled();
+// Writes the device's constant-rate clock to *slot (stream-ordered).
+int device_clock_stamp(uint64_t *slot, hipStream_t st);
 
 // Optional event marks between kernels: mark(i) records event i on the stream.
 struct Marks {
@@ -169,7 +171,7 @@ struct JoinDesc {
     uint64_t *seg_count, *seg_sum;  // nseg accumulators (zeroed by the caller)
     uint64_t *pairs;                // nullable: materialise pairs
     uint64_t pairs_cap;
-    bool records = false;           // materialise <S.key, S.payloadID> records instead
+    int records = 0;                // 1: <S.key, S.payloadID> records; 2: timed records
     uint64_t *pair_cursor;          // device scalar (zeroed by the caller)
     uint32_t *task_cnt;             // workspace: max_parts+1
     uint32_t *task_base;            // workspace: max_parts+1
@@ -198,7 +200,7 @@ struct NpoDesc {
     uint64_t *acc;       // [0]=count [1]=sum [2]=sentinel R tuples [3]=pair cursor
     uint64_t *pairs;
     uint64_t pairs_cap;
-    bool records = false;  // materialise <S.key, S.payloadID> records instead of pairs
+    int records = 0;  // 1: <S.key, S.payloadID> records; 2: (clock << 32 | S.payloadID)
     int num_cu;
 };
 int run_npo(const NpoDesc &d, hipStream_t st, const Marks *marks = nullptr, int m0 = 0);

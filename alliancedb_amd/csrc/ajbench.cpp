@@ -6,6 +6,7 @@
 //           [-d key distribution: 0 unique/uniform, 2 zipf] [-z zipf theta]
 //           [-w window -e step -l interval] [-n nthreads] [-m] [--repeats n]
 //           [--pairs] [--out FILE] [-R fileR -J keyR -L tsR] [-S fileS -K keyS -M tsS]
+//           [--timing EXP_DIR [--exp-id n] [--gap n]]
 //   ajbench -a SERVE [--load f] [--peak Mtps] [--seconds s] [--streams n]
 //           [--budget bytes] [--slots n] [--batch-windows n] [--clients n]
 //           [--verify n] [--median n] [--sigma s] [--cap-tuples n]
@@ -24,6 +25,10 @@
 // reference's text files (main.cpp:612-628; key column -J/-K, timestamp column
 // -L/-M, |R| and |S| lines from -r/-s) through aj_load_relation, with the
 // reference's placement over -n partitions (benchmark.cpp:183-205).
+// --timing writes the reference's progressive-result files under EXP_DIR
+// (records, timestamps, latency, gaps, breakdown; aj_write_timing_files) with
+// lastTS = -w (the window, ms) and prints the progress figures the reference
+// prints (t_timer.cpp:67-89).
 #include <getopt.h>
 
 #include <chrono>
@@ -80,12 +85,15 @@ struct params {
     const char *out_file = nullptr;
     const char *file_r = nullptr, *file_s = nullptr;
     int rkey = 0, skey = 0, rts = 0, sts = 0;
+    const char *timing_dir = nullptr;
+    int exp_id = 0, gap = 1;
 };
 
 void usage(const char *prog) {
     printf("Usage: %s -a <PRO|NPO> [-r R] [-s S] [-x seed] [-y seed] [-d 0|2] [-z theta]\n"
            "          [-w window -e step -l interval] [-n nthreads] [-m] [--repeats n]\n"
            "          [--pairs] [--out FILE] [-R fileR -J keyR -L tsR] [-S fileS -K keyS -M tsS]\n"
+           "          [--timing EXP_DIR [--exp-id n] [--gap n]]\n"
            "       %s -a SERVE [--load f] [--peak Mtps] [--seconds s] [--streams n]\n"
            "          [--budget bytes] [--slots n] [--batch-windows n] [--clients n]\n"
            "          [--verify n] [--median n] [--sigma s] [--cap-tuples n] [--domain n]\n"
@@ -95,7 +103,7 @@ void usage(const char *prog) {
 
 enum { OPT_LOAD = 1000, OPT_PEAK, OPT_SECONDS, OPT_STREAMS, OPT_BUDGET, OPT_SLOTS, OPT_BWIN,
        OPT_CLIENTS, OPT_VERIFY, OPT_MEDIAN, OPT_SIGMA, OPT_CAPT, OPT_DOMAIN, OPT_BTUP,
-       OPT_PAIRS, OPT_OUT, OPT_REPEATS };
+       OPT_PAIRS, OPT_OUT, OPT_REPEATS, OPT_TIMING, OPT_EXPID, OPT_GAP };
 
 }  // namespace
 
@@ -133,6 +141,9 @@ int main(int argc, char **argv) {
                                 {"nthreads", required_argument, 0, 'n'},
                                 {"materialize", no_argument, 0, 'm'},
                                 {"repeats", required_argument, 0, OPT_REPEATS},
+                                {"timing", required_argument, 0, OPT_TIMING},
+                                {"exp-id", required_argument, 0, OPT_EXPID},
+                                {"gap", required_argument, 0, OPT_GAP},
                                 {"loadfileR", required_argument, 0, 'R'},
                                 {"loadfileS", required_argument, 0, 'S'},
                                 {"rkey", required_argument, 0, 'J'},
@@ -187,6 +198,9 @@ int main(int argc, char **argv) {
             case OPT_PAIRS: p.materialize = 1; break;
             case OPT_OUT: p.out_file = optarg; p.materialize = 2; break;
             case OPT_REPEATS: p.repeats = atoi(optarg); break;
+            case OPT_TIMING: p.timing_dir = optarg; break;
+            case OPT_EXPID: p.exp_id = atoi(optarg); break;
+            case OPT_GAP: p.gap = atoi(optarg); break;
             case 'R': p.file_r = optarg; break;
             case 'S': p.file_s = optarg; break;
             case 'J': p.rkey = atoi(optarg); break;
@@ -240,6 +254,7 @@ int main(int argc, char **argv) {
         return EXIT_FAILURE;
     }
     aj_set_option("materialize", p.materialize);
+    aj_set_option("timing", p.timing_dir ? 1 : 0);
     printf("[INFO ] Running join algorithm %s ...\n", p.algo->name);
     for (int rep = 0; rep < p.repeats; ++rep) {
         auto t0 = std::chrono::steady_clock::now();
@@ -254,6 +269,23 @@ int main(int argc, char **argv) {
                "total_ms = %.3f wall_ms = %.3f\n",
                (long long)res->totalresults, (unsigned long long)x->checksum,
                (unsigned long long)x->npairs, x->device_ms, x->total_ms, ms);
+        if (p.timing_dir && rep + 1 == p.repeats) {
+            const bool pro = strcmp(p.algo->name, "PRO") == 0;
+            aj_timing_summary_t sm;
+            relR.payload = p.file_r ? &plR : nullptr;
+            relS.payload = p.file_s ? &plS : nullptr;
+            if (aj_write_timing_files(res, &relR, &relS, p.timing_dir, pro ? "PRJ" : "NPJ",
+                                      p.exp_id, p.window, p.gap, &sm) != AJ_OK) {
+                fprintf(stderr, "[ERROR] %s\n", aj_last_error());
+                aj_free_result(res);
+                return EXIT_FAILURE;
+            }
+            printf("Time to obtain 1%%, 5%%, 10%%, 25%%, 50%% of results (MSECS): \n");
+            printf("(%.2f) \t (%.2f) \t (%.2f) \t (%.2f) \t (%.2f)\n\n", sm.progress_ms[0],
+                   sm.progress_ms[1], sm.progress_ms[2], sm.progress_ms[3], sm.progress_ms[4]);
+            printf("95th latency: (%.2f)\t 99th latency: (%.2f)\n", sm.latency95_ms,
+                   sm.latency99_ms);
+        }
         if (p.out_file && rep + 1 == p.repeats) {
             printf("[INFO ] Persisting the join result to \"%s\" ...\n", p.out_file);
             if (aj_write_result_file(res, p.out_file) != AJ_OK) {

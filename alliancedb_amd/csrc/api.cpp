@@ -12,10 +12,10 @@
 namespace aj {
 
 int pro_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, uint64_t nS,
-               const ProConfig &cfg, uint64_t *pairs, uint64_t pairs_cap, bool records,
+               const ProConfig &cfg, uint64_t *pairs, uint64_t pairs_cap, int records,
                aj_join_out_t *out, hipStream_t st);
 int npo_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, uint64_t nS,
-               uint64_t *pairs, uint64_t pairs_cap, bool records, aj_join_out_t *out,
+               uint64_t *pairs, uint64_t pairs_cap, int records, aj_join_out_t *out,
                hipStream_t st);
 
 // ---------------------------------------------------------------- errors
@@ -95,9 +95,10 @@ void DevBuf::release() {
 // ---------------------------------------------------------------- options
 static std::atomic<int64_t> opt_materialize{0}, opt_radix_bits{0}, opt_passes{0}, opt_device{0},
     opt_cap{0}, opt_split{0}, opt_kernel_timing{0}, opt_part_block{0}, opt_part_chunk{0},
-    opt_part_hist2d{1}, opt_tuned{1};
+    opt_part_hist2d{1}, opt_tuned{1}, opt_timing{0};
 
-bool kernel_timing_enabled() { return opt_kernel_timing.load() != 0; }
+static thread_local bool g_force_timing = false;  // phase timing for one drop-in call
+bool kernel_timing_enabled() { return g_force_timing || opt_kernel_timing.load() != 0; }
 
 static int ctx_init(Ctx *c, int device) {
     int n = 0;
@@ -189,7 +190,11 @@ static int join_dev_locked(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint
     const int algo = opts ? opts->algo : AJ_ALGO_PRO;
     uint64_t *pairs = opts ? opts->pairs : nullptr;
     const uint64_t pcap = opts ? opts->pairs_capacity : 0;
-    const bool records = opts && opts->records;
+    const int records = opts ? opts->records : 0;
+    if (records < 0 || records > 2) {
+        set_error(AJ_ERR_ARG, "records must be 0, 1 or 2");
+        return AJ_ERR_ARG;
+    }
     int rc;
     if (algo == AJ_ALGO_NPO) {
         rc = npo_device(ctx, dR, nR, dS, nS, pairs, pcap, records, out, st);
@@ -309,6 +314,35 @@ static threadresult_t *build_resultlist(const uint64_t *dsrc, uint64_t n, int nt
     return l;
 }
 
+// Timed records: one per match, (device clock ticks since the join started << 32)
+// | S.payloadID, from a join run with records = 2 after a clock stamp.
+static int timed_run(Ctx *ctx, aj_join_opts_t o, uint64_t nR, uint64_t nS, uint64_t count,
+                     std::vector<uint64_t> *timed, double *clock_hz) {
+    AJ_TRY(ctx->pairs.ensure((count + 1) * 8));
+    uint64_t *stamp = ctx->pairs.as<uint64_t>() + count;
+    AJ_TRY(device_clock_stamp(stamp, ctx->stream));
+    o.pairs = ctx->pairs.as<uint64_t>();
+    o.pairs_capacity = count;
+    o.records = 2;
+    aj_join_out_t out;
+    AJ_TRY(join_dev_locked(ctx, ctx->inR.as<uint64_t>(), nR, ctx->inS.as<uint64_t>(), nS, &o,
+                           &out, ctx->stream));
+    if (out.count != count || out.npairs != count)
+        return set_error(AJ_ERR_STATE, "timed join disagrees with the counting join");
+    timed->resize(count + 1);
+    if (hipMemcpyAsync(timed->data(), ctx->pairs.p, (count + 1) * 8, hipMemcpyDeviceToHost,
+                       ctx->stream) ||
+        hipStreamSynchronize(ctx->stream))
+        return set_error(AJ_ERR_HIP, "device-to-host copy of timed records failed");
+    const uint32_t t0 = (uint32_t)(*timed)[count];
+    timed->pop_back();
+    for (uint64_t &v : *timed) v = ((uint64_t)((uint32_t)(v >> 32) - t0) << 32) | (uint32_t)v;
+    int khz = 0;
+    AJ_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device));
+    *clock_hz = khz * 1e3;
+    return AJ_OK;
+}
+
 static result_t *dropin(int algo, relation_t *relR, relation_t *relS, int nthreads) {
     clear_error();
     if (check_relation(relR, "R") || check_relation(relS, "S")) return nullptr;
@@ -325,9 +359,11 @@ static result_t *dropin(int algo, relation_t *relR, relation_t *relS, int nthrea
     aj_join_out_t out;
     memset(&out, 0, sizeof(out));
     const int64_t mat = opt_materialize.load();
+    const bool timing = opt_timing.load() != 0;
     const int nt = std::max(nthreads, 1);
     uint64_t npairs = 0;
-    std::vector<uint64_t> host_pairs;
+    std::vector<uint64_t> host_pairs, timed;
+    double clock_hz = 0;
     threadresult_t *rlist = nullptr;
     int rc = AJ_OK;
     do {
@@ -343,47 +379,60 @@ static result_t *dropin(int algo, relation_t *relR, relation_t *relS, int nthrea
         aj_join_opts_t o;
         memset(&o, 0, sizeof(o));
         o.algo = algo;
+        g_force_timing = timing;  // per-phase device times of the counting run
         rc = join_dev_locked(ctx, ctx->inR.as<uint64_t>(), nR, ctx->inS.as<uint64_t>(), nS, &o,
                              &out, ctx->stream);
-        if (rc || !mat || out.count == 0) break;
-        if ((rc = ctx->pairs.ensure(out.count * 8))) break;
-        o.pairs = ctx->pairs.as<uint64_t>();
-        o.pairs_capacity = out.count;
-        o.records = mat == 2;
-        aj_join_out_t out2;
-        rc = join_dev_locked(ctx, ctx->inR.as<uint64_t>(), nR, ctx->inS.as<uint64_t>(), nS, &o,
-                             &out2, ctx->stream);
-        if (rc) break;
-        if (out2.count != out.count || out2.npairs != out.count) {
-            rc = set_error(AJ_ERR_STATE, "materialising join disagrees with the counting join");
-            break;
-        }
-        if (mat == 2) {
+        g_force_timing = false;
+        if (rc || out.count == 0) break;
+        if (mat) {
+            if ((rc = ctx->pairs.ensure(out.count * 8))) break;
+            aj_join_opts_t om = o;
+            om.pairs = ctx->pairs.as<uint64_t>();
+            om.pairs_capacity = out.count;
+            om.records = mat == 2;
+            aj_join_out_t out2;
+            rc = join_dev_locked(ctx, ctx->inR.as<uint64_t>(), nR, ctx->inS.as<uint64_t>(), nS,
+                                 &om, &out2, ctx->stream);
+            if (rc) break;
+            if (out2.count != out.count || out2.npairs != out.count) {
+                rc = set_error(AJ_ERR_STATE, "materialising join disagrees with the counting join");
+                break;
+            }
             if (hipStreamSynchronize(ctx->stream)) {
                 rc = set_error(AJ_ERR_HIP, "join stream failed");
                 break;
             }
-            rlist = build_resultlist(ctx->pairs.as<uint64_t>(), out2.npairs, nt, &rc);
-            break;
+            if (mat == 2) {
+                rlist = build_resultlist(ctx->pairs.as<uint64_t>(), out2.npairs, nt, &rc);
+                if (!rlist) break;
+            } else {
+                npairs = out2.npairs;
+                host_pairs.resize(npairs);
+                if (hipMemcpy(host_pairs.data(), ctx->pairs.p, npairs * 8, hipMemcpyDeviceToHost)) {
+                    rc = set_error(AJ_ERR_HIP, "device-to-host copy of pairs failed");
+                    break;
+                }
+            }
         }
-        npairs = out2.npairs;
-        host_pairs.resize(npairs);
-        if (hipMemcpy(host_pairs.data(), ctx->pairs.p, npairs * 8, hipMemcpyDeviceToHost)) {
-            set_error(AJ_ERR_HIP, "device-to-host copy of pairs failed");
-            rc = AJ_ERR_HIP;
-        }
+        if (timing) rc = timed_run(ctx, o, nR, nS, out.count, &timed, &clock_hz);
     } while (0);
-    if (rc) return nullptr;
+    if (rc) {
+        free_resultlist(rlist, nt);
+        return nullptr;
+    }
     if (mat == 2 && !rlist) {  // no matches (or an empty input): nt empty chains
         rlist = build_resultlist(nullptr, 0, nt, &rc);
         if (!rlist) return nullptr;
     }
-    aj_result_t *res = (aj_result_t *)malloc(sizeof(aj_result_t) + npairs * sizeof(uint64_t));
+    const uint64_t ntimed = timed.size();
+    aj_result_t *res =
+        (aj_result_t *)malloc(sizeof(aj_result_t) + (npairs + ntimed) * sizeof(uint64_t));
     if (!res) {
         free_resultlist(rlist, nt);
         set_error(AJ_ERR_NOMEM, "host allocation of the result failed");
         return nullptr;
     }
+    memset(res, 0, sizeof(aj_result_t));
     res->base.totalresults = (int64_t)out.count;
     res->base.resultlist = rlist;
     res->base.nthreads = rlist ? nt : nthreads;
@@ -393,6 +442,12 @@ static result_t *dropin(int algo, relation_t *relR, relation_t *relS, int nthrea
     if (npairs) memcpy(res->pairs, host_pairs.data(), npairs * 8);
     res->status = AJ_OK;
     res->device_ms = out.device_ms;
+    res->ntimed = ntimed;
+    res->timed = ntimed ? reinterpret_cast<uint64_t *>(res + 1) + npairs : nullptr;
+    if (ntimed) memcpy(res->timed, timed.data(), ntimed * 8);
+    res->clock_hz = clock_hz;
+    res->algo = algo;
+    if (timing) memcpy(res->phase_ms, out.phase_ms, sizeof(res->phase_ms));
     res->total_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0)
                         .count();
     return &res->base;
@@ -509,7 +564,8 @@ int aj_set_option(const char *name, int64_t value) {
                {"part_block", &opt_part_block, 0, 512},
                {"part_chunk", &opt_part_chunk, 0, 1ll << 24},
                {"part_hist2d", &opt_part_hist2d, 0, 1},
-               {"tuned", &opt_tuned, 0, 1}};
+               {"tuned", &opt_tuned, 0, 1},
+               {"timing", &opt_timing, 0, 1}};
     for (auto &e : tab) {
         if (strcmp(e.n, name) == 0) {
             if (value < e.lo || value > e.hi) {
@@ -554,6 +610,7 @@ int64_t aj_get_option(const char *name) {
     if (!strcmp(name, "part_chunk")) return opt_part_chunk.load();
     if (!strcmp(name, "part_hist2d")) return opt_part_hist2d.load();
     if (!strcmp(name, "tuned")) return opt_tuned.load();
+    if (!strcmp(name, "timing")) return opt_timing.load();
     if (!strcmp(name, "device_budget")) return (int64_t)g_dev_budget.load();
     if (!strcmp(name, "device_used")) return (int64_t)g_dev_used.load();
     return -1;

@@ -69,7 +69,7 @@ struct JoinArgs {
     uint64_t *pairs;
     uint64_t pairs_cap;
     uint64_t *pair_cursor;
-    int records;  // write the S tuple <S.key, S.payloadID> per match instead of the pair
+    int records;  // 0: pairs; 1: the S tuple <S.key, S.payloadID>; 2: (clock << 32 | S.payloadID)
     uint32_t *task_cnt, *task_base, *task_next;
     uint4 *task_info;
     ulonglong2 *task_res;  // per task {count, checksum}
@@ -247,7 +247,7 @@ __device__ __forceinline__ void probe_part(const uint64_t (&sv)[N], uint32_t ns,
                                            const uint64_t *tab, const uint16_t *bst, int lg,
                                            uint64_t &cnt, uint64_t &sum, uint64_t *pairs,
                                            uint64_t &wpos, uint64_t cap, uint32_t tid,
-                                           int records = 0) {
+                                           int records = 0, uint32_t tnow = 0) {
     if constexpr (G0 < N) {
         constexpr int G = N - G0 < PGROUP ? N - G0 : PGROUP;
         // groups that no lane of this wave holds are skipped (wave-uniform)
@@ -268,7 +268,9 @@ __device__ __forceinline__ void probe_part(const uint64_t (&sv)[N], uint32_t ns,
                     if (key_of(r[j]) == key_of(sv[G0 + j])) {
                         const uint64_t v = (r[j] << 32) | payload_of(sv[G0 + j]);
                         if (WRITE) {
-                            if (wpos < cap) pairs[wpos] = records ? sv[G0 + j] : v;
+                            if (wpos < cap)
+                                pairs[wpos] = records == 2 ? ((uint64_t)tnow << 32) | payload_of(sv[G0 + j])
+                                              : records ? sv[G0 + j] : v;
                             ++wpos;
                         } else {
                             ++cnt;
@@ -282,7 +284,7 @@ __device__ __forceinline__ void probe_part(const uint64_t (&sv)[N], uint32_t ns,
             if (!more) break;
         }
         probe_part<JB, CAP, WRITE, Q0, N, G0 + PGROUP>(sv, ns, tab, bst, lg, cnt, sum, pairs,
-                                                       wpos, cap, tid, records);
+                                                       wpos, cap, tid, records, tnow);
     }
 }
 
@@ -382,10 +384,15 @@ __global__ __launch_bounds__(JB, 4) void k_join(JoinArgs a) {
                     __syncthreads();
                     wpos = s_base + pre;
                     uint64_t dc = 0, dx = 0;
+                    // timed records: the device's constant-rate clock when this
+                    // slice's matches are produced (low 32 bits)
+                    const uint32_t tnow =
+                        a.records == 2 ? (uint32_t)__builtin_amdgcn_s_memrealtime() : 0u;
                     probe_part<JB, CAP, true, 0, SA>(sa, sub, tab, bst, lg, dc, dx, a.pairs, wpos,
-                                                    a.pairs_cap, opaque_tid(), a.records);
+                                                    a.pairs_cap, opaque_tid(), a.records, tnow);
                     probe_part<JB, CAP, true, SA, SB>(sb, sub, tab, bst, lg, dc, dx, a.pairs,
-                                                     wpos, a.pairs_cap, opaque_tid(), a.records);
+                                                     wpos, a.pairs_cap, opaque_tid(), a.records,
+                                                     tnow);
                 }
                 cnt += c;
                 sum += x;
@@ -479,7 +486,7 @@ int run_join(const JoinDesc &d, hipStream_t st, const Marks *mk, int m0) {
     a.seg_sum = d.seg_sum;
     a.pairs = d.pairs;
     a.pairs_cap = d.pairs_cap;
-    a.records = d.records ? 1 : 0;
+    a.records = d.records;
     a.pair_cursor = d.pair_cursor;
     a.task_cnt = d.task_cnt;
     a.task_base = d.task_base;

@@ -103,7 +103,7 @@ template <bool WRITE>
 __device__ __forceinline__ void npo_probe_one(uint64_t s, const uint64_t *tab, uint64_t mask,
                                               uint64_t nsent, uint64_t &cnt, uint64_t &sum,
                                               uint64_t *pairs, uint64_t &wpos, uint64_t cap,
-                                              int records = 0) {
+                                              int records = 0, uint32_t tnow = 0) {
     const uint32_t k = key_of(s);
     uint64_t h = fmix32(k) & mask;
     for (;;) {
@@ -111,7 +111,9 @@ __device__ __forceinline__ void npo_probe_one(uint64_t s, const uint64_t *tab, u
         if (v == EMPTY_SLOT) break;
         if (key_of(v) == k) {
             if (WRITE) {
-                if (wpos < cap) pairs[wpos] = records ? s : pair_value(v, s);
+                if (wpos < cap)
+                    pairs[wpos] = records == 2 ? ((uint64_t)tnow << 32) | (uint32_t)s
+                                  : records ? s : pair_value(v, s);
                 ++wpos;
             } else {
                 ++cnt;
@@ -124,7 +126,9 @@ __device__ __forceinline__ void npo_probe_one(uint64_t s, const uint64_t *tab, u
         const uint64_t pv = pair_value(EMPTY_SLOT, s);
         if (WRITE) {
             for (uint64_t q = 0; q < nsent; ++q) {
-                if (wpos < cap) pairs[wpos] = records ? s : pv;
+                if (wpos < cap)
+                    pairs[wpos] = records == 2 ? ((uint64_t)tnow << 32) | (uint32_t)s
+                                  : records ? s : pv;
                 ++wpos;
             }
         } else {
@@ -160,7 +164,9 @@ __global__ __launch_bounds__(NBLOCK) void k_npo_probe(const uint64_t *S, uint64_
             if (valid && c) {
                 wpos = s_base + pre;
                 uint64_t dc = 0, dx = 0;
-                npo_probe_one<true>(s, tab, mask, nsent, dc, dx, pairs, wpos, cap, records);
+                const uint32_t tnow =
+                    records == 2 ? (uint32_t)__builtin_amdgcn_s_memrealtime() : 0u;
+                npo_probe_one<true>(s, tab, mask, nsent, dc, dx, pairs, wpos, cap, records, tnow);
             }
             __syncthreads();
         }
@@ -211,7 +217,7 @@ int run_npo(const NpoDesc &d, hipStream_t st, const Marks *mk, int m0) {
         if (d.pairs)
             hipLaunchKernelGGL(k_npo_probe<true>, dim3(g), dim3(NBLOCK), 0, st, d.S, d.nS,
                                (const uint64_t *)d.table, mask, d.acc, d.pairs, d.pairs_cap,
-                               d.records ? 1 : 0);
+                               d.records);
         else
             hipLaunchKernelGGL(k_npo_probe<false>, dim3(g), dim3(NBLOCK), 0, st, d.S, d.nS,
                                (const uint64_t *)d.table, mask, d.acc, nullptr, (uint64_t)0, 0);

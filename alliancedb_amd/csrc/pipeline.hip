@@ -198,8 +198,18 @@ static void read_phases(Ctx *c, aj_join_out_t *out, int first, int last) {
 }
 
 // scalars: [0]=count [1]=checksum [2]=pair cursor [3]=task counter (u32 in low half)
+__global__ void k_clock_stamp(uint64_t *slot) {
+    if (threadIdx.x == 0) *slot = __builtin_amdgcn_s_memrealtime();
+}
+
+int device_clock_stamp(uint64_t *slot, hipStream_t st) {
+    hipLaunchKernelGGL(k_clock_stamp, dim3(1), dim3(64), 0, st, slot);
+    AJ_HIP(hipGetLastError());
+    return AJ_OK;
+}
+
 int pro_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, uint64_t nS,
-               const ProConfig &cfg, uint64_t *pairs, uint64_t pairs_cap, bool records,
+               const ProConfig &cfg, uint64_t *pairs, uint64_t pairs_cap, int records,
                aj_join_out_t *out, hipStream_t st) {
     ProPlan p = plan_pro(nR, nS, cfg, ctx->num_cu);
     AJ_TRY(ctx->plan.ensure(p.bytes));
@@ -386,7 +396,7 @@ int pro_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
 }
 
 int npo_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, uint64_t nS,
-               uint64_t *pairs, uint64_t pairs_cap, bool records, aj_join_out_t *out,
+               uint64_t *pairs, uint64_t pairs_cap, int records, aj_join_out_t *out,
                hipStream_t st) {
     uint64_t cap = 2;
     while (cap <= 2 * nR) cap <<= 1;

@@ -74,6 +74,12 @@ typedef struct aj_result_t {
     int status;
     float device_ms;   /* join time on the device, inputs resident */
     float total_ms;    /* including host<->device copies */
+    /* option "timing" = 1 (see aj_write_timing_files): */
+    uint64_t ntimed;   /* timed records, one per match */
+    uint64_t *timed;   /* (device clock ticks since the join started << 32) | S.payloadID */
+    double clock_hz;   /* tick rate of timed (the device's constant-rate clock) */
+    int algo;          /* AJ_ALGO_PRO or AJ_ALGO_NPO */
+    float phase_ms[12]; /* per-phase device times, indexed by enum aj_phase */
 } aj_result_t;
 
 /* Returns the extension of a result from PRO/NPO, or NULL if r is NULL. */
@@ -124,10 +130,47 @@ int aj_write_result_file(const result_t *r, const char *path);
  *   "part_hist2d"  0/1   two-pass PRO: pass 2 from a (d1, d2) histogram taken in
  *                        pass 1 instead of its own histogram pass (default 1)
  *   "tuned"        0/1   use the autotuner's table for open geometries (default 1)
+ *   "timing"       0/1   drop-in results also carry timed records and per-phase
+ *                        device times for aj_write_timing_files (default 0)
  *   "device_budget" bytes of device memory the library may hold (0 = unlimited)
  */
 int aj_set_option(const char *name, int64_t value);
 int64_t aj_get_option(const char *name);
+
+/*
+ * Progressive-result files of a drop-in result taken with option "timing" = 1,
+ * in the layout the reference writes under its EXP_DIR when built with timing
+ * (dump_timing, merge and sortRecords: timer/t_timer.cpp:31-99, 401-452;
+ * breakdown_global: t_timer.cpp:113-140; call sites
+ * parallel_radix_join.cpp:1628-1656 and no_partitioning_join.cpp:433-459):
+ *   <exp_dir>/results/records/<name>_<exp_id>.txt     inputs, matches
+ *   <exp_dir>/results/timestamps/<name>_<exp_id>.txt  per sampled match, the time
+ *       it was produced (ms since the join started) + lastTS, ascending
+ *   <exp_dir>/results/latency/<name>_<exp_id>.txt     per sampled match,
+ *       (produced - S tuple's ts + lastTS) in ms, + lastTS again, ascending
+ *   <exp_dir>/results/gaps/<name>_<exp_id>.txt        S.payloadID - sample index
+ *       (+ lastTS), ascending
+ *   <exp_dir>/results/breakdown/<name>_<exp_id>.txt   wait, partition, build,
+ *       sort, merge, join, others in 2.1 GHz cycles per input tuple
+ * name is the reference's "PRJ" (PRO) or "NPJ" (NPO); lastTS is the window in
+ * ms (0 for a relation without timestamps); S timestamps are relS->payload->ts
+ * in 2.1 GHz cycles as aj_load_relation gives them (0 when relS->payload is
+ * NULL).  Every gap-th match in production order is sampled (the reference's
+ * record_gap; 1 = all).  Matches are timed per join task (PRO) or per probe
+ * block (NPO) with the device clock; the production order of records with the
+ * same time is by S.payloadID.  Folders are created as needed.  Fills
+ * summary (nullable) with what the reference prints: the time to 1, 5, 10, 25
+ * and 50% of the sampled results and the 95th / 99th latency, in ms.
+ */
+typedef struct aj_timing_summary_t {
+    double progress_ms[5]; /* 1, 5, 10, 25, 50 % */
+    double latency95_ms, latency99_ms;
+    uint64_t samples;
+} aj_timing_summary_t;
+
+int aj_write_timing_files(const result_t *r, const relation_t *relR, const relation_t *relS,
+                          const char *exp_dir, const char *name, int exp_id, long lastTS,
+                          int gap, aj_timing_summary_t *summary);
 
 /* ------------------------------------------------------------ input files */
 

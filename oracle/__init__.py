@@ -255,3 +255,35 @@ def _c_long_prefix(tok: str) -> str:
     if j == i:
         raise ValueError(f"not a number: {tok!r}")
     return s[:j]
+
+
+def ref_breakdown_file(path: str, nR: int, nS: int, last_ts: int, overall: int, part: int,
+                       build: int, join: int) -> None:
+    """The reference's own breakdown_global (lazy joins) written to path."""
+    f = ref().ref_breakdown_file
+    f.argtypes = [ctypes.c_char_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_long,
+                  ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64]
+    f.restype = ctypes.c_int
+    if f(os.fsencode(path), nR, nS, last_ts, overall, part, build, join):
+        raise RuntimeError("ref_breakdown_file failed")
+
+
+def timing_records(cycles, sids, s_ts, last_ts: int):
+    """Restatement of the reference's merge + sortRecords for S-side samples
+    (timer/t_timer.cpp:401-452): per sample, the match time in ms, the latency
+    (unsigned cycle arithmetic, then signed) in ms and the disorder gap
+    S.payloadID - sample index; each list sorted ascending."""
+    wait = int(last_ts * 2.1 * 1e6)
+    times, lat, gaps = [], [], []
+    for i, (c, s) in enumerate(zip(cycles, sids)):
+        c, s = int(c), int(s)
+        times.append(c / (2.1 * 1e6))
+        u = (c - (int(s_ts[s]) if s_ts is not None else 0) + wait) % 2**64
+        lat.append((u - 2**64 if u >= 2**63 else u) / (2.1 * 1e6))
+        gaps.append(float(s - i))
+    return sorted(times), sorted(lat), sorted(gaps)
+
+
+def timing_text(values, last_ts: int) -> str:
+    """dump_timing's lines: std::to_string(element + lastTS) (t_timer.cpp:58-61)."""
+    return "".join("%f\n" % (v + last_ts) for v in values)

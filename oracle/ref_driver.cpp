@@ -20,6 +20,7 @@
 #include "joins/no_partitioning_join.h"
 #include "joins/parallel_radix_join.h"
 #include "utils/tuple_buffer.h"
+#include "timer/t_timer.h"
 
 #include <cstdint>
 #include <cstdlib>
@@ -97,6 +98,26 @@ int ref_load_relation(const char *path, int keyby, int tsKey, uint64_t n, uint32
     }
     free(rel.tuples);
     free(pl.ts);
+    return 0;
+}
+
+// The reference's own breakdown_global of the lazy joins (timer/t_timer.cpp:
+// 113-140) for the given phase cycles, written to path.  (merge / dump_timing,
+// the progressive records, compile to nothing here: the reference's
+// common_functions.h:21 defines NO_TIMING and its scripts flip that line in
+// place, benchmark.sh:88.)
+int ref_breakdown_file(const char *path, uint64_t nR, uint64_t nS, long lastTS,
+                       uint64_t overall, uint64_t part, uint64_t build, uint64_t join) {
+    if (!path) return -1;
+    T_TIMER t;
+    t.overall_timer = overall;
+    t.partition_timer = part;
+    t.buildtimer = build;
+    t.join_timer = join;
+    FILE *fp = fopen(path, "w");
+    if (!fp) return -2;
+    breakdown_global((int64_t)(nR + nS), 1, &t, lastTS, fp);
+    fclose(fp);
     return 0;
 }
 

@@ -71,3 +71,57 @@ def test_dataset_known_answers(aj, name):
     S, _ = aj.load_relation(os.path.join(root, fs), nS, keys[1], tss[1], 1)
     for algo in ("PRO", "NPO"):
         assert getattr(aj, algo)(R, S, 1).count == want, algo
+
+
+@pytest.mark.parametrize("algo", ["PRO", "NPO"])
+def test_timed_records(aj, oracle, algo):
+    """Option timing = 1: one timed record per match whose S.payloadIDs are the
+    matches' S rows (from the oracle's pair list), times inside the join, and
+    per-phase device times."""
+    R = aj.gen_unique(1 << 18, 5)
+    S = aj.gen_uniform(300000, 1, 1 << 18, 6)
+    aj.set_option("timing", 1)
+    try:
+        print(f"case {algo} timed", flush=True)
+        got = getattr(aj, algo)(R, S, 4)
+    finally:
+        aj.set_option("timing", 0)
+    pairs = oracle.pairs_sortmerge(R, S)
+    assert got.count == len(pairs) and len(got.timed) == got.count
+    sids = np.sort(got.timed & np.uint64(0xFFFFFFFF))
+    assert np.array_equal(sids, np.sort(pairs & np.uint64(0xFFFFFFFF)))
+    assert 1e6 <= got.clock_hz <= 1e10
+    ms = (got.timed >> np.uint64(32)).astype(np.float64) / got.clock_hz * 1e3
+    assert ms.min() >= 0 and ms.max() < 1000
+    ph = got.phase_ms
+    if algo == "PRO":
+        assert sum(ph[0:8]) > 0 and ph[9] > 0
+    else:
+        assert ph[10] > 0 and ph[11] > 0
+
+
+def test_driver_timing_files(aj, tmp_path):
+    """ajbench --timing on loaded relations writes the reference's five result
+    files: counts, one line per match in timestamps / latency / gaps, and the
+    7-line breakdown."""
+    rng = np.random.default_rng(3)
+    nR, nS = 20000, 30000
+    pr, ps = str(tmp_path / "r.txt"), str(tmp_path / "s.txt")
+    _write_rel(pr, rng.integers(0, 5000, nR), np.sort(rng.integers(0, 1000, nR)))
+    _write_rel(ps, rng.integers(0, 5000, nS), np.sort(rng.integers(0, 1000, nS)))
+    exe = os.path.join(os.path.dirname(aj.LIB_PATH), "ajbench")
+    d = tmp_path / "exp"
+    r = subprocess.run([exe, "-a", "PRO", "-r", str(nR), "-s", str(nS), "-n", "4", "-R", pr,
+                        "-L", "1", "-S", ps, "-M", "1", "-w", "1000", "--timing", str(d),
+                        "--exp-id", "3"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "95th latency" in r.stdout
+    count = int(re.findall(r"Results = (\d+)", r.stdout)[0])
+    res = d / "results"
+    assert (res / "records" / "PRJ_3.txt").read_text().split() == [str(nR + nS), str(count)]
+    for sub in ("timestamps", "latency", "gaps"):
+        v = np.array((res / sub / "PRJ_3.txt").read_text().split(), float)
+        assert len(v) == count and np.all(np.diff(v) >= 0), sub
+    t = np.array((res / "timestamps" / "PRJ_3.txt").read_text().split(), float)
+    assert t.min() >= 1000  # lastTS = the window
+    assert len((res / "breakdown" / "PRJ_3.txt").read_text().split()) == 7
