@@ -30,6 +30,7 @@ AJ_ERR_STATE = -5
 AJ_ERR_NODEV = -6
 AJ_ALGO_PRO = 0
 AJ_ALGO_NPO = 1
+AJ_ALGO_SHJ = 2
 
 
 class relation_payload_t(ctypes.Structure):
@@ -88,7 +89,8 @@ class aj_join_opts_t(ctypes.Structure):
     _fields_ = [("algo", ctypes.c_int), ("radix_bits", ctypes.c_int), ("passes", ctypes.c_int),
                 ("cap", ctypes.c_int), ("split", ctypes.c_uint32),
                 ("pairs", ctypes.c_void_p), ("pairs_capacity", ctypes.c_uint64),
-                ("records", ctypes.c_int)]
+                ("records", ctypes.c_int), ("batch", ctypes.c_uint64),
+                ("progress", ctypes.c_void_p)]
 
 
 AJ_NPHASES = 12
@@ -161,6 +163,11 @@ def _load() -> ctypes.CDLL:
         "aj_last_status": (c.c_int, []),
         "PRO": (c.POINTER(result_t), [c.POINTER(relation_t), c.POINTER(relation_t), c.c_int]),
         "NPO": (c.POINTER(result_t), [c.POINTER(relation_t), c.POINTER(relation_t), c.c_int]),
+        "SHJ_JM_NP": (c.POINTER(result_t), [c.POINTER(relation_t), c.POINTER(relation_t),
+                                            c.c_int]),
+        "SHJ_JBCR_NP": (c.POINTER(result_t), [c.POINTER(relation_t), c.POINTER(relation_t),
+                                              c.c_int]),
+        "SHJ_st": (c.POINTER(result_t), [c.POINTER(relation_t), c.POINTER(relation_t), c.c_int]),
         "aj_result_ext": (c.POINTER(aj_result_t), [c.POINTER(result_t)]),
         "aj_free_result": (None, [c.POINTER(result_t)]),
         "aj_write_timing_files": (c.c_int, [c.POINTER(result_t), c.POINTER(relation_t),
@@ -363,6 +370,21 @@ def NPO(R: np.ndarray, S: np.ndarray, nthreads: int = 1,
     return _dropin(lib.NPO, R, S, nthreads, result_file)
 
 
+def SHJ_JM_NP(R: np.ndarray, S: np.ndarray, nthreads: int = 1) -> JoinResult:
+    """Drop-in eager symmetric hash join (join-matrix name); see aj.h."""
+    return _dropin(lib.SHJ_JM_NP, R, S, nthreads)
+
+
+def SHJ_JBCR_NP(R: np.ndarray, S: np.ndarray, nthreads: int = 1) -> JoinResult:
+    """Drop-in eager symmetric hash join (join-biclique ContRand name); see aj.h."""
+    return _dropin(lib.SHJ_JBCR_NP, R, S, nthreads)
+
+
+def SHJ_st(R: np.ndarray, S: np.ndarray, nthreads: int = 1) -> JoinResult:
+    """Drop-in eager symmetric hash join (single-thread name); see aj.h."""
+    return _dropin(lib.SHJ_st, R, S, nthreads)
+
+
 class Context:
     """An aj_ctx_t: device scratch reused across joins on one GPU."""
 
@@ -384,10 +406,11 @@ class Context:
 
     def join(self, dR: int, nR: int, dS: int, nS: int, algo: int = AJ_ALGO_PRO,
              radix_bits: int = 0, passes: int = 0, cap: int = 0, split: int = 0,
-             pairs_ptr: int = 0, pairs_capacity: int = 0, stream: int = 0) -> aj_join_out_t:
+             pairs_ptr: int = 0, pairs_capacity: int = 0, stream: int = 0, records: int = 0,
+             batch: int = 0, progress_ptr: int = 0) -> aj_join_out_t:
         """Join device-resident tuple arrays (device pointers as ints)."""
         o = aj_join_opts_t(algo, radix_bits, passes, cap, split, pairs_ptr or None,
-                           pairs_capacity)
+                           pairs_capacity, records, batch, progress_ptr or None)
         out = aj_join_out_t()
         check(lib.aj_join_dev(self.handle, dR or None, nR, dS or None, nS, ctypes.byref(o),
                               ctypes.byref(out), stream or None))

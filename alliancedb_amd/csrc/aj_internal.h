@@ -74,10 +74,15 @@ struct Ctx {
     DevBuf p1R, p1S, p2R, p2S;  // partitioned copies
     DevBuf hist, scan_tmp, plan, seg, tasks, acc, pairs, table, misc;
     DevBuf hist2d;              // per-chunk (d1, d2) counts of a two-pass PRO
+    DevBuf table2, steps;       // SHJ: S-side table, per-step match counts
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t evk[AJ_NPHASES + 1] = {};  // per-kernel timing (option kernel_timing)
 };
 bool kernel_timing_enabled();
+// Eager symmetric hash join (shj.hip): steps of `batch` tuples per side.
+uint64_t shj_steps(uint64_t nR, uint64_t nS, uint64_t batch);
+int shj_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, uint64_t nS,
+               uint64_t batch, uint64_t *progress, aj_join_out_t *out, hipStream_t st);
 // Writes the device's constant-rate clock to *slot (stream-ordered).
 int device_clock_stamp(uint64_t *slot, hipStream_t st);
 

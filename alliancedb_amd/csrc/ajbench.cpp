@@ -2,7 +2,7 @@
 // runs it, like the reference's `hashing` binary (main -> parse_args ->
 // benchmark(), hashing/main.cpp:256-296, 444-705; hashing/benchmark.cpp:164-257).
 //
-//   ajbench -a PRO|NPO [-r |R|] [-s |S|] [-x r_seed] [-y s_seed]
+//   ajbench -a PRO|NPO|SHJ_st|SHJ_JM_NP|SHJ_JBCR_NP [-r |R|] [-s |S|] [-x r_seed] [-y s_seed]
 //           [-d key distribution: 0 unique/uniform, 2 zipf] [-z zipf theta]
 //           [-w window -e step -l interval] [-n nthreads] [-m] [--repeats n]
 //           [--pairs] [--out FILE] [-R fileR -J keyR -L tsR] [-S fileS -K keyS -M tsS]
@@ -69,6 +69,9 @@ struct algo_t {
 const algo_t algos[] = {
     {"PRO", PRO},
     {"NPO", NPO},
+    {"SHJ_st", SHJ_st},
+    {"SHJ_JM_NP", SHJ_JM_NP},
+    {"SHJ_JBCR_NP", SHJ_JBCR_NP},
     {nullptr, nullptr},
 };
 
@@ -90,7 +93,7 @@ struct params {
 };
 
 void usage(const char *prog) {
-    printf("Usage: %s -a <PRO|NPO> [-r R] [-s S] [-x seed] [-y seed] [-d 0|2] [-z theta]\n"
+    printf("Usage: %s -a <PRO|NPO|SHJ_st|SHJ_JM_NP|SHJ_JBCR_NP> [-r R] [-s S] [-x seed] [-y seed] [-d 0|2] [-z theta]\n"
            "          [-w window -e step -l interval] [-n nthreads] [-m] [--repeats n]\n"
            "          [--pairs] [--out FILE] [-R fileR -J keyR -L tsR] [-S fileS -K keyS -M tsS]\n"
            "          [--timing EXP_DIR [--exp-id n] [--gap n]]\n"

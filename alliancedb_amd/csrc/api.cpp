@@ -127,7 +127,7 @@ static void ctx_fini(Ctx *c) {
     (void)hipSetDevice(c->device);
     DevBuf *bufs[] = {&c->inR, &c->inS, &c->p1R, &c->p1S, &c->p2R, &c->p2S, &c->hist,
                       &c->scan_tmp, &c->plan, &c->seg, &c->tasks, &c->acc, &c->pairs,
-                      &c->table, &c->misc, &c->hist2d};
+                      &c->table, &c->misc, &c->hist2d, &c->table2, &c->steps};
     for (DevBuf *b : bufs) b->release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -182,11 +182,26 @@ static ProConfig config_from(uint64_t nR, uint64_t nS, const aj_join_opts_t *o) 
     return c;
 }
 
+// Tuples per side per eager step: the caller's, else 256 steps of >= 65536.
+static uint64_t shj_batch(uint64_t nR, uint64_t nS, const aj_join_opts_t *opts) {
+    if (opts && opts->batch) return opts->batch;
+    const uint64_t n = std::max(nR, nS);
+    return std::max<uint64_t>(65536, (n + 255) / 256);
+}
+
 static int join_dev_locked(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS,
                            uint64_t nS, const aj_join_opts_t *opts, aj_join_out_t *out,
                            hipStream_t st) {
     memset(out, 0, sizeof(*out));
-    if (nR == 0 || nS == 0) return AJ_OK;
+    if (nR == 0 || nS == 0) {
+        // every step of an eager join over an empty side matches nothing
+        if (opts && opts->algo == AJ_ALGO_SHJ && opts->progress) {
+            const uint64_t n = shj_steps(nR, nS, shj_batch(nR, nS, opts));
+            std::vector<uint64_t> z(n, 0);
+            if (n) AJ_HIP(hipMemcpy(opts->progress, z.data(), 8 * n, hipMemcpyDefault));
+        }
+        return AJ_OK;
+    }
     const int algo = opts ? opts->algo : AJ_ALGO_PRO;
     uint64_t *pairs = opts ? opts->pairs : nullptr;
     const uint64_t pcap = opts ? opts->pairs_capacity : 0;
@@ -196,7 +211,14 @@ static int join_dev_locked(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint
         return AJ_ERR_ARG;
     }
     int rc;
-    if (algo == AJ_ALGO_NPO) {
+    if (algo == AJ_ALGO_SHJ) {
+        if (pairs) {
+            set_error(AJ_ERR_ARG, "SHJ returns counts only (no pairs or records)");
+            return AJ_ERR_ARG;
+        }
+        rc = shj_device(ctx, dR, nR, dS, nS, shj_batch(nR, nS, opts), opts ? opts->progress : nullptr,
+                        out, st);
+    } else if (algo == AJ_ALGO_NPO) {
         rc = npo_device(ctx, dR, nR, dS, nS, pairs, pcap, records, out, st);
     } else if (algo == AJ_ALGO_PRO) {
         rc = pro_device(ctx, dR, nR, dS, nS, config_from(nR, nS, opts), pairs, pcap, records,
@@ -379,6 +401,11 @@ static result_t *dropin(int algo, relation_t *relR, relation_t *relS, int nthrea
         aj_join_opts_t o;
         memset(&o, 0, sizeof(o));
         o.algo = algo;
+        if (algo == AJ_ALGO_SHJ && (mat || timing)) {
+            rc = set_error(AJ_ERR_ARG, "SHJ drop-ins return counts only (options materialize "
+                                       "and timing must be 0)");
+            break;
+        }
         g_force_timing = timing;  // per-phase device times of the counting run
         rc = join_dev_locked(ctx, ctx->inR.as<uint64_t>(), nR, ctx->inS.as<uint64_t>(), nS, &o,
                              &out, ctx->stream);
@@ -476,6 +503,18 @@ result_t *PRO(relation_t *relR, relation_t *relS, int nthreads) {
 
 result_t *NPO(relation_t *relR, relation_t *relS, int nthreads) {
     return dropin(AJ_ALGO_NPO, relR, relS, nthreads);
+}
+
+result_t *SHJ_JM_NP(relation_t *relR, relation_t *relS, int nthreads) {
+    return dropin(AJ_ALGO_SHJ, relR, relS, nthreads);
+}
+
+result_t *SHJ_JBCR_NP(relation_t *relR, relation_t *relS, int nthreads) {
+    return dropin(AJ_ALGO_SHJ, relR, relS, nthreads);
+}
+
+result_t *SHJ_st(relation_t *relR, relation_t *relS, int nthreads) {
+    return dropin(AJ_ALGO_SHJ, relR, relS, nthreads);
 }
 
 const aj_result_t *aj_result_ext(const result_t *r) {

@@ -1,0 +1,198 @@
+// shj.hip -- eager symmetric hash join (SHJ) on two HBM-resident hash tables.
+//
+// Reference: every arriving tuple is inserted into its own side's table and
+// probed against the other side's, so each match is produced as soon as its
+// later tuple arrives (SHJJoiner::join, helper/localjoiner.cpp:73-95; the
+// single-thread driver shj alternates R[i], S[i] in index order, :33-64).  The
+// multithreaded variants only change which thread sees which tuples: join
+// matrix (every thread holds all of R and a slice of S, fetcher.h:146-190) or
+// join biclique with content-sensitive groups and random S placement
+// (ContRandShuffler, shuffler.cpp:117-166); the matches are the same.
+//
+// CDNA4 form: arrivals are taken in steps of `batch` tuples per side.  Step k
+// runs two kernels: R[k*batch, (k+1)*batch) is inserted into TR and probed
+// against TS, then S[k*batch, (k+1)*batch) is inserted into TS and probed
+// against TR.  The kernel boundary is the only ordering needed: a pair (r_i,
+// s_j) is found exactly once, in the step of its later tuple (r first on a
+// tie), which for batch = 1 is the reference's alternating order.  Tables are
+// the NPO layout (open addressing over whole 64-bit tuples, capacity the next
+// power of two >= 2n, murmur3-finalised home slot, all-ones empty marker with
+// all-ones tuples counted on the side).  Per-step cumulative match counts give
+// the progressive result curve.
+//
+// Bytes per tuple: read 8 + own slot write 8 + probe slot reads (>= 8).
+#include "aj_internal.h"
+#include "device_common.h"
+
+namespace aj {
+
+namespace {
+
+constexpr int SBLOCK = 256;
+
+// Insert v into tab (the all-ones tuple is counted in *own_sent instead) and
+// count / checksum its matches in other (other_sent all-ones tuples there).
+// Work-group b adds its matches to step_cnt[b] (this step's row) and its
+// checksum to sum_part[b]: every launch has the same grid and launches are
+// stream-ordered, so each slot has one writer at a time and no atomics contend.
+__global__ __launch_bounds__(SBLOCK) void k_shj_step(const uint64_t *in, uint64_t lo,
+                                                     uint64_t hi, uint64_t *tab, uint64_t mask,
+                                                     uint64_t *own_sent, const uint64_t *other,
+                                                     uint64_t omask, const uint64_t *other_sent,
+                                                     int is_r, uint64_t *sum_part,
+                                                     uint64_t *step_cnt) {
+    __shared__ uint64_t red[SBLOCK / 64];
+    const uint64_t nsent = *other_sent;  // not written during this kernel
+    uint64_t cnt = 0, sum = 0, sent = 0;
+    for (uint64_t i = lo + blockIdx.x * (uint64_t)SBLOCK + threadIdx.x; i < hi;
+         i += (uint64_t)gridDim.x * SBLOCK) {
+        const uint64_t v = in[i];
+        const uint32_t k = key_of(v);
+        // probe the other side first (its table does not change in this kernel)
+        uint64_t h = fmix32(k) & omask;
+        for (;;) {
+            const uint64_t t = other[h];
+            if (t == EMPTY_SLOT) break;
+            if (key_of(t) == k) {
+                ++cnt;
+                sum += is_r ? pair_value(v, t) : pair_value(t, v);
+            }
+            h = (h + 1) & omask;
+        }
+        if (k == 0xFFFFFFFFu && nsent) {  // all-ones tuples of the other side
+            cnt += nsent;
+            const uint64_t pv = is_r ? pair_value(v, EMPTY_SLOT) : pair_value(EMPTY_SLOT, v);
+            sum += nsent * pv;
+        }
+        if (v == EMPTY_SLOT) {
+            ++sent;
+            continue;
+        }
+        h = fmix32(k) & mask;
+        // terminates: fewer tuples than slots
+        while (atomicCAS((unsigned long long *)&tab[h], (unsigned long long)EMPTY_SLOT,
+                         (unsigned long long)v) != EMPTY_SLOT)
+            h = (h + 1) & mask;
+    }
+    sent = block_sum<SBLOCK>(sent, red);
+    cnt = block_sum<SBLOCK>(cnt, red);
+    sum = block_sum<SBLOCK>(sum, red);
+    if (threadIdx.x == 0) {
+        if (sent) atomicAdd((unsigned long long *)own_sent, (unsigned long long)sent);
+        if (cnt) {
+            step_cnt[blockIdx.x] += cnt;
+            sum_part[blockIdx.x] += sum;
+        }
+    }
+}
+
+// One work-group: prog[k] = matches of steps 0..k from the per-block rows
+// step_cnt[k * G + b] (a thread per step, 256 steps per round, carried scan);
+// acc[0] = all matches, acc[1] = the checksum.
+__global__ __launch_bounds__(SBLOCK) void k_shj_finish(const uint64_t *step_cnt, uint64_t steps,
+                                                       unsigned G, const uint64_t *sum_part,
+                                                       uint64_t *prog, uint64_t *acc) {
+    __shared__ uint64_t red[SBLOCK / 64 + 1];
+    uint64_t carry = 0;
+    for (uint64_t base = 0; base < steps; base += SBLOCK) {
+        const uint64_t k = base + threadIdx.x;
+        uint64_t c = 0;
+        if (k < steps)
+            for (unsigned b = 0; b < G; ++b) c += step_cnt[k * G + b];
+        uint64_t total;
+        const uint64_t pre = block_exclusive_scan<SBLOCK>(c, red, &total);
+        if (k < steps) prog[k] = carry + pre + c;
+        carry += total;
+        __syncthreads();
+    }
+    uint64_t x = 0;
+    for (unsigned b = threadIdx.x; b < G; b += SBLOCK) x += sum_part[b];
+    x = block_sum<SBLOCK>(x, red);
+    if (threadIdx.x == 0) {
+        acc[0] = carry;
+        acc[1] = x;
+    }
+}
+
+unsigned shj_grid(uint64_t n, int num_cu) {
+    uint64_t g = (n + SBLOCK - 1) / SBLOCK;
+    const uint64_t cap = (uint64_t)num_cu * 8;
+    if (g > cap) g = cap;
+    return (unsigned)(g ? g : 1);
+}
+
+uint64_t table_cap(uint64_t n) {
+    uint64_t c = 2;
+    while (c <= 2 * n) c <<= 1;
+    return c;
+}
+
+}  // namespace
+
+uint64_t shj_steps(uint64_t nR, uint64_t nS, uint64_t batch) {
+    const uint64_t n = nR > nS ? nR : nS;
+    return batch ? (n + batch - 1) / batch : 0;
+}
+
+int shj_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, uint64_t nS,
+               uint64_t batch, uint64_t *progress, aj_join_out_t *out, hipStream_t st) {
+    if (batch == 0) {
+        set_error(AJ_ERR_ARG, "SHJ batch must be >= 1");
+        return AJ_ERR_ARG;
+    }
+    const uint64_t capR = table_cap(nR), capS = table_cap(nS);
+    const uint64_t steps = shj_steps(nR, nS, batch);
+    if (steps > (1ull << 20)) {
+        set_error(AJ_ERR_ARG, "SHJ batch %llu gives %llu steps (limit 2^20)",
+                  (unsigned long long)batch, (unsigned long long)steps);
+        return AJ_ERR_ARG;
+    }
+    const unsigned G = shj_grid(batch, ctx->num_cu);  // one grid for every launch
+    AJ_TRY(ctx->table.ensure(capR * 8));
+    AJ_TRY(ctx->table2.ensure(capS * 8));
+    AJ_TRY(ctx->acc.ensure(8 * 8));
+    AJ_TRY(ctx->steps.ensure(8 * (steps * G + G + steps)));
+    uint64_t *acc = ctx->acc.as<uint64_t>();  // [0] count [1] sum [2] R sentinels [3] S sentinels
+    uint64_t *tr = ctx->table.as<uint64_t>(), *ts = ctx->table2.as<uint64_t>();
+    uint64_t *scnt = ctx->steps.as<uint64_t>();  // [steps][G] per-block step matches
+    uint64_t *sum_part = scnt + steps * G;       // [G] per-block checksums
+    uint64_t *prog = sum_part + G;               // [steps] cumulative matches
+    AJ_HIP(hipEventRecord(ctx->ev0, st));
+    AJ_TRY(fill_u64(acc, 8, 0, st));
+    AJ_TRY(fill_u64(tr, capR, EMPTY_SLOT, st));
+    AJ_TRY(fill_u64(ts, capS, EMPTY_SLOT, st));
+    AJ_TRY(fill_u64(scnt, steps * G + G, 0, st));
+    for (uint64_t k = 0; k < steps; ++k) {
+        const uint64_t lo = k * batch;
+        if (lo < nR) {
+            const uint64_t hi = lo + batch < nR ? lo + batch : nR;
+            hipLaunchKernelGGL(k_shj_step, dim3(G), dim3(SBLOCK), 0, st, dR, lo, hi, tr, capR - 1,
+                               acc + 2, (const uint64_t *)ts, capS - 1,
+                               (const uint64_t *)(acc + 3), 1, sum_part, scnt + k * G);
+        }
+        if (lo < nS) {
+            const uint64_t hi = lo + batch < nS ? lo + batch : nS;
+            hipLaunchKernelGGL(k_shj_step, dim3(G), dim3(SBLOCK), 0, st, dS, lo, hi, ts, capS - 1,
+                               acc + 3, (const uint64_t *)tr, capR - 1,
+                               (const uint64_t *)(acc + 2), 0, sum_part, scnt + k * G);
+        }
+    }
+    hipLaunchKernelGGL(k_shj_finish, dim3(1), dim3(SBLOCK), 0, st, (const uint64_t *)scnt, steps,
+                       G, (const uint64_t *)sum_part, prog, acc);
+    AJ_HIP(hipGetLastError());
+    AJ_HIP(hipEventRecord(ctx->ev1, st));
+    uint64_t host[2];
+    AJ_HIP(hipMemcpyAsync(host, acc, sizeof(host), hipMemcpyDeviceToHost, st));
+    if (progress)
+        AJ_HIP(hipMemcpyAsync(progress, prog, 8 * steps, hipMemcpyDefault, st));
+    AJ_HIP(hipStreamSynchronize(st));
+    float ms = 0;
+    AJ_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    out->count = host[0];
+    out->checksum = host[1];
+    out->npairs = 0;
+    out->device_ms = ms;
+    return AJ_OK;
+}
+
+}  // namespace aj

@@ -38,16 +38,18 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 PRO_BYTES_PER_TUPLE = 32  # histogram read 8 + scatter read 8 + write 8 + build/probe read 8
 NPO_BYTES_PER_TUPLE = 16  # R read 8 + slot write 8 / S read 8 + slot read 8
+SHJ_BYTES_PER_TUPLE = 24  # read 8 + own-table slot write 8 + other-table slot read 8
 WINDOW_BYTES_PER_TUPLE = 40  # 12 hist read + 12 scatter read + 8 write + 8 build/probe read
 # algorithmic bytes per input tuple of each kernel (8-byte tuples).  With the
 # default two-pass geometry pass 1's histogram kernel (k_hist2d) also counts the
 # pass-2 digits, and the "hist2" phase is only the pass-2 unit planning.
 KERNEL_BYTES = {"hist1": 8, "scatter1": 16, "hist2": 0, "scatter2": 16, "join": 8,
-                "npo_build": 16, "npo_probe": 16}
+                "npo_build": 16, "npo_probe": 16, "shj_step": 24}
 KERNEL_NAMES = {"hist1": "k_hist2d (pass 1)", "scatter1": "k_scatter_pipe (pass 1)",
                 "hist2": "pass-2 planning (k_p2_units, k_h2_usum, k_h2_base)",
                 "scatter2": "k_scatter_p2 (pass 2)",
-                "join": "k_join", "npo_build": "k_npo_build", "npo_probe": "k_npo_probe"}
+                "join": "k_join", "npo_build": "k_npo_build", "npo_probe": "k_npo_probe",
+                "shj_step": "k_shj_step (all eager steps)"}
 
 
 def parse():
@@ -60,7 +62,10 @@ def parse():
                     help="serve: batched-window peak (M tuples/s) the offered load is relative to")
     ap.add_argument("--loads", default="0.5,0.7,0.9,1.0,1.1,1.2")
     ap.add_argument("--seconds", type=float, default=2.0)
-    ap.add_argument("--algo", choices=["PRO", "NPO"], default="PRO")
+    ap.add_argument("--algo", choices=["PRO", "NPO", "SHJ"], default="PRO",
+                    help="pro workload: lazy PRO / NPO, or the eager symmetric hash join")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="SHJ: tuples per side per eager step (0 = library default)")
     ap.add_argument("--log2n", type=int, default=27)
     ap.add_argument("--theta", type=float, default=1.0)
     ap.add_argument("--nwin", type=int, default=1024)
@@ -74,6 +79,28 @@ def parse():
     ap.add_argument("--calib", action="store_true",
                     help="also run aj_copy_kernel over 1 GiB (traffic-counter calibration)")
     return ap.parse_args()
+
+
+def cpu_baseline_shj(aj, log2n: int):
+    """The reference's own single-thread symmetric hash join (shj, compiled from
+    its sources into oracle/_ref) on a bounded sample of the same workload."""
+    import oracle
+    if not oracle.ref_available():
+        return None
+    n = 1 << log2n
+    R = aj.gen_unique(n, 12345)
+    S = aj.gen_uniform(n, 1, n, 54321)
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        assert oracle.ref_shj(R, S) == n
+        times.append(time.perf_counter() - t0)
+    sec = sorted(times)[1]
+    return {"value": round(2 * n / sec / 1e6, 2), "unit": "M tuples/s", "cores": 1,
+            "kind": "reference",
+            "sample": f"reference shj (helper/localjoiner.cpp:33-64, one thread) "
+                      f"|R|=|S|=2^{log2n}, same key distribution, median of 3 "
+                      f"(input copies included)"}
 
 
 def cpu_baseline(aj, log2n: int):
@@ -155,9 +182,12 @@ def timed_joins(aj, ctx, dR, nR, dS, nS, algo, kw, steps, warmup, want):
 def join_line(aj, args, workload_name, nR, nS, algo, out, dev, ms_step, phases, metric):
     tuples = nR + nS
     mean_ph = {k: float(np.mean([p[k] for p in phases])) for k in phases[0]}
+    if algo == aj.AJ_ALGO_SHJ:  # one kernel type: the steps are the whole device time
+        mean_ph = {"shj_step": dev}
     dom = max(KERNEL_BYTES, key=lambda k: mean_ph.get(k, 0.0))
     achieved = KERNEL_BYTES[dom] * tuples / (mean_ph[dom] * 1e-3) / 1e9
-    bpt = PRO_BYTES_PER_TUPLE if algo == aj.AJ_ALGO_PRO else NPO_BYTES_PER_TUPLE
+    bpt = {aj.AJ_ALGO_PRO: PRO_BYTES_PER_TUPLE, aj.AJ_ALGO_NPO: NPO_BYTES_PER_TUPLE,
+           aj.AJ_ALGO_SHJ: SHJ_BYTES_PER_TUPLE}[algo]
     pipeline_gbs = bpt * tuples / (dev * 1e-3) / 1e9
     return {
         "metric": metric,
@@ -200,8 +230,10 @@ def run_pro(aj, args):
         dst = aj.DeviceArray(dR.nbytes)
         aj.check(aj.lib.aj_copy_d2d(dst.ptr, dR.ptr, dR.nbytes))
         dst.free()
-    algo = aj.AJ_ALGO_PRO if args.algo == "PRO" else aj.AJ_ALGO_NPO
+    algo = {"PRO": aj.AJ_ALGO_PRO, "NPO": aj.AJ_ALGO_NPO, "SHJ": aj.AJ_ALGO_SHJ}[args.algo]
     kw = dict(radix_bits=args.radix_bits, passes=args.passes, cap=args.cap)
+    if algo == aj.AJ_ALGO_SHJ:
+        kw = dict(batch=args.batch)
     with aj.Context(0) as ctx:
         tuned = autotune(aj, ctx, dR, n, dS, n, algo, kw, args)
         out, dev, ms_step, phases = timed_joins(aj, ctx, dR, n, dS, n, algo, kw, args.steps,
@@ -218,8 +250,12 @@ def run_pro(aj, args):
         not tuned or (tuned["radix_bits"], tuned["passes"], tuned["cap"]) == (14, 2, 8704))
     if default_geom:
         line["roofline"]["traffic"] = measured_traffic(line["roofline"]["kernel"], name)
+    if algo == aj.AJ_ALGO_SHJ:
+        line["metric"] = "eager join throughput M tuples/s (|R|+|S|)"
+        line["config"]["batch"] = args.batch or max(65536, -(-n // 256))
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(aj, args.cpu_log2n)
+        line["cpu_baseline"] = (cpu_baseline_shj(aj, args.cpu_log2n) if algo == aj.AJ_ALGO_SHJ
+                                else cpu_baseline(aj, args.cpu_log2n))
     return line
 
 

@@ -82,6 +82,23 @@ typedef struct aj_result_t {
     float phase_ms[12]; /* per-phase device times, indexed by enum aj_phase */
 } aj_result_t;
 
+/*
+ * Eager symmetric hash join (SHJ) on the GPU: every tuple is inserted into its
+ * side's hash table and probed against the other side's, in arrival (index)
+ * order, in steps of aj_join_opts_t.batch tuples per side (R part first), so
+ * each match is produced in the step of its later tuple.  Replaces the
+ * reference's eager joins `result_t *SHJ_JM_NP(relation_t*, relation_t*,
+ * param_t)` and `SHJ_JBCR_NP(...)` (hashing/joins/onlinejoins.h:33,62,
+ * onlinejoins.cpp:166-181, 220-235) and the single-thread `SHJ_st`
+ * (onlinejoins.h:94, onlinejoins.cpp:132-162): the join-matrix and
+ * join-biclique/ContRand layouts only decide which CPU thread sees which tuple,
+ * and all three return the same matches.  nthreads is recorded only.  Count
+ * and checksum as PRO; options materialize and timing must be 0.
+ */
+result_t *SHJ_JM_NP(relation_t *relR, relation_t *relS, int nthreads);
+result_t *SHJ_JBCR_NP(relation_t *relR, relation_t *relS, int nthreads);
+result_t *SHJ_st(relation_t *relR, relation_t *relS, int nthreads);
+
 /* Returns the extension of a result from PRO/NPO, or NULL if r is NULL. */
 const aj_result_t *aj_result_ext(const result_t *r);
 
@@ -216,7 +233,7 @@ typedef struct aj_ctx aj_ctx_t;
 int aj_ctx_create(int device, aj_ctx_t **out);
 int aj_ctx_destroy(aj_ctx_t *ctx);
 
-enum aj_algo { AJ_ALGO_PRO = 0, AJ_ALGO_NPO = 1 };
+enum aj_algo { AJ_ALGO_PRO = 0, AJ_ALGO_NPO = 1, AJ_ALGO_SHJ = 2 };
 
 typedef struct aj_join_opts_t {
     int algo;          /* AJ_ALGO_PRO or AJ_ALGO_NPO */
@@ -228,7 +245,11 @@ typedef struct aj_join_opts_t {
     uint64_t pairs_capacity;
     int records;       /* 1: write the reference's result records <S.key, S.payloadID>
                           (the matching S tuple, one per match) into pairs instead of
-                          (R.payloadID << 32 | S.payloadID); count and checksum unchanged */
+                          (R.payloadID << 32 | S.payloadID); 2: timed records
+                          (device clock << 32 | S.payloadID); count and checksum unchanged */
+    uint64_t batch;    /* SHJ: tuples per side per eager step, 0 = max(65536, n / 256) */
+    uint64_t *progress; /* SHJ (nullable, host or device): matches after each step,
+                          ceil(max(|R|, |S|) / batch) entries */
 } aj_join_opts_t;
 
 /* Per-kernel device times of the last join, filled when the option

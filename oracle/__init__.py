@@ -287,3 +287,34 @@ def timing_records(cycles, sids, s_ts, last_ts: int):
 def timing_text(values, last_ts: int) -> str:
     """dump_timing's lines: std::to_string(element + lastTS) (t_timer.cpp:58-61)."""
     return "".join("%f\n" % (v + last_ts) for v in values)
+
+
+def ref_shj(R, S) -> int:
+    """The reference's own single-thread symmetric hash join (shj,
+    helper/localjoiner.cpp:33-64) -> match count.  Needs |R|, |S| >= 2."""
+    R, S = _c(R), _c(S)
+    f = ref().ref_shj
+    f.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64,
+                  ctypes.POINTER(ctypes.c_int64)]
+    f.restype = ctypes.c_int
+    c = ctypes.c_int64()
+    if f(_p(R), len(R), _p(S), len(S), ctypes.byref(c)):
+        raise ValueError("the reference's SHJ needs |R|, |S| >= 2")
+    return int(c.value)
+
+
+def shj_progress(R, S, batch: int) -> np.ndarray:
+    """Matches found after each eager step of `batch` tuples per side (R part
+    first): a pair (r_i, s_j) belongs to step max(i // batch, j // batch) --
+    the step in which its later tuple arrives.  From the sort-merge pair list;
+    needs payloadID = row index on both sides."""
+    R, S = _c(R), _c(S)
+    assert np.array_equal(R["payloadID"], np.arange(len(R)))
+    assert np.array_equal(S["payloadID"], np.arange(len(S)))
+    n = max(len(R), len(S))
+    steps = (n + batch - 1) // batch
+    pairs = pairs_sortmerge(R, S)
+    i = (pairs >> np.uint64(32)).astype(np.int64)
+    j = (pairs & np.uint64(0xFFFFFFFF)).astype(np.int64)
+    k = np.maximum(i // batch, j // batch)
+    return np.cumsum(np.bincount(k, minlength=steps)[:steps]).astype(np.uint64)

@@ -21,6 +21,7 @@
 #include "joins/parallel_radix_join.h"
 #include "utils/tuple_buffer.h"
 #include "timer/t_timer.h"
+#include "helper/localjoiner.h"
 
 #include <cstdint>
 #include <cstdlib>
@@ -98,6 +99,32 @@ int ref_load_relation(const char *path, int keyby, int tsKey, uint64_t n, uint32
     }
     free(rel.tuples);
     free(pl.ts);
+    return 0;
+}
+
+// The reference's own symmetric hash join of one thread, shj
+// (helper/localjoiner.cpp:33-101): R and S tuples alternate in index order,
+// each inserted into its side's table and probed against the other's.  The
+// per-match records <S.key, S.payloadID> are not used here (the probing side
+// varies), only the count.  Inputs are copied (the tables point into them).
+int ref_shj(const void *R, uint64_t nR, const void *S, uint64_t nS, int64_t *count) {
+    if (!count || nR < 2 || nS < 2) return -1;  // the tables need >= 1 bucket
+    relation_t r, s;
+    r.tuples = (tuple_t *)malloc(nR * sizeof(tuple_t));
+    s.tuples = (tuple_t *)malloc(nS * sizeof(tuple_t));
+    memcpy(r.tuples, R, nR * sizeof(tuple_t));
+    memcpy(s.tuples, S, nS * sizeof(tuple_t));
+    r.num_tuples = nR;
+    s.num_tuples = nS;
+    r.payload = s.payload = nullptr;
+    chainedtuplebuffer_t *cb = chainedtuplebuffer_init();
+    {
+        SHJJoiner j = shj(0, &r, &s, cb);
+        *count = j.matches;
+    }
+    chainedtuplebuffer_free(cb);
+    free(r.tuples);
+    free(s.tuples);
     return 0;
 }
 

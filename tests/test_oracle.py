@@ -137,3 +137,25 @@ def test_sort_u64(oracle):
     rng = np.random.default_rng(5)
     a = rng.integers(0, 2**63, 10000, dtype=np.int64).astype(np.uint64) * np.uint64(2)
     assert np.array_equal(oracle.sort_u64(a), np.sort(a))
+
+
+def test_reference_shj_counts_the_equi_join(aj, oracle):
+    """The reference's eager single-thread SHJ produces exactly the equi-join
+    matches (the closed-form count), so the lazy joins' oracle pins SHJ too."""
+    if not oracle.ref_available():
+        pytest.skip("oracle/_ref/libref.so not built")
+    rng = np.random.default_rng(11)
+    for nR, nS, dom in ((2, 2, 1), (500, 700, 50), (4096, 3000, 4096), (1000, 5000, 7)):
+        R = aj.as_tuples(rng.integers(-5, dom, nR))
+        S = aj.as_tuples(rng.integers(-5, dom, nS))
+        assert oracle.ref_shj(R, S) == oracle.aggregate(R, S)[0]
+
+
+def test_shj_progress_closed_form(aj, oracle):
+    """Progress of a one-key join: with |R| = |S| = n all on one key, pairs with
+    max(i, j) < (k + 1) b number ((k + 1) b)^2 after step k."""
+    n, b = 64, 8
+    R = aj.as_tuples(np.full(n, 3))
+    S = aj.as_tuples(np.full(n, 3))
+    want = np.array([((k + 1) * b) ** 2 for k in range(n // b)], np.uint64)
+    assert np.array_equal(oracle.shj_progress(R, S, b), want)
