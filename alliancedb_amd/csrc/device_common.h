@@ -84,6 +84,15 @@ template <class T> __device__ __forceinline__ T wave_inclusive_scan(T x) {
     return x;
 }
 
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t y = __shfl_xor(x, o);
+        x = x > y ? x : y;
+    }
+    return x;
+}
+
 template <class T> __device__ __forceinline__ T wave_sum(T x) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);

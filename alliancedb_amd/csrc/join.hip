@@ -288,6 +288,64 @@ __device__ __forceinline__ void probe_part(const uint64_t (&sv)[N], uint32_t ns,
     }
 }
 
+// Counting probe (no pair output): PGROUP S tuples of a lane step through their
+// buckets together, for as many steps as the longest bucket of any tuple in the
+// wave (a wave-uniform trip count, so the loop control is scalar and the body is
+// branch-free).  Step i reads candidate lo+i of every tuple (clamped to the
+// spare slot) and counts it when i < len and the keys are equal.  Matches of
+// tuple j are counted in m[j]; the checksum is rebuilt from its two halves:
+// sum of R payloads (mod 2^32, the high word) and m[j] * S payload.
+template <int JB, int CAP, int Q0, int N, int G0 = 0>
+__device__ __forceinline__ void probe_count(const uint64_t (&sv)[N], uint32_t ns,
+                                            const uint64_t *tab, const uint16_t *bst, int lg,
+                                            uint64_t &cnt, uint64_t &sum, uint32_t tid) {
+    if constexpr (G0 < N) {
+        constexpr int G = N - G0 < PGROUP ? N - G0 : PGROUP;
+        // groups that no lane of this wave holds are skipped (wave-uniform)
+        if (__builtin_amdgcn_readfirstlane(tid) + (uint32_t)(Q0 + G0) * JB >= ns) return;
+        // candidates are read as {payload, key} word pairs so that the key test
+        // is one 32-bit compare
+        const uint2 *tab2 = reinterpret_cast<const uint2 *>(tab);
+        const uint2 *spare = tab2 + CAP;  // reads past a bucket are clamped here
+        const uint2 *lo[G];
+        uint32_t len[G], sk[G];
+        uint32_t mx = 0;
+        unroll<G>([&](auto j) {
+            uint32_t k = key_of(sv[G0 + j]);
+            asm volatile("" : "+v"(k));  // a 32-bit value to the compiler: 32-bit key tests
+            sk[j] = k;
+            const uint32_t h = bucket_of(k, lg);
+            const uint32_t a = bst[h ? h - 1 : 0], e = bst[h];
+            const uint32_t b = h ? a : 0u;
+            lo[j] = tab2 + b;
+            len[j] = tid + (uint32_t)(Q0 + G0 + j) * JB < ns ? e - b : 0u;
+            mx = mx > len[j] ? mx : len[j];
+        });
+        const uint32_t steps = __builtin_amdgcn_readfirstlane(wave_max_u32(mx));
+        uint32_t m[G];
+        uint32_t rsum = 0;
+        unroll<G>([&](auto j) { m[j] = 0; });
+        for (uint32_t i = 0; i < steps; ++i) {
+            uint2 r[G];
+            unroll<G>([&](auto j) {
+                const uint2 *q = lo[j] + i;
+                r[j] = *(q < spare ? q : spare);
+            });
+            unroll<G>([&](auto j) {
+                const bool hit = (int)(i < len[j]) & (int)(r[j].y == sk[j]);
+                m[j] += hit ? 1u : 0u;
+                rsum += hit ? r[j].x : 0u;
+            });
+        }
+        unroll<G>([&](auto j) {
+            cnt += m[j];
+            sum += (uint64_t)m[j] * payload_of(sv[G0 + j]);
+        });
+        sum += (uint64_t)rsum << 32;
+        probe_count<JB, CAP, Q0, N, G0 + PGROUP>(sv, ns, tab, bst, lg, cnt, sum, tid);
+    }
+}
+
 // Tuples tid + (Q0 + j)*JB, j < N, of `p` (n >= 1 tuples) into registers.
 // The loads are unconditional so that all of them are in flight at once: an
 // index at or past n reads tuple n-1 again, and callers ignore those elements.
@@ -371,10 +429,8 @@ __global__ __launch_bounds__(JB, 4) void k_join(JoinArgs a) {
                 uint64_t sb[SB];
                 load_part<JB, SA, SB>(sb, a.S + cur.s0 + so, sub, opaque_tid());
                 uint64_t c = 0, x = 0, wpos = 0;
-                probe_part<JB, CAP, false, 0, SA>(sa, sub, tab, bst, lg, c, x, nullptr, wpos, 0,
-                                                 opaque_tid());
-                probe_part<JB, CAP, false, SA, SB>(sb, sub, tab, bst, lg, c, x, nullptr, wpos, 0,
-                                                  opaque_tid());
+                probe_count<JB, CAP, 0, SA>(sa, sub, tab, bst, lg, c, x, opaque_tid());
+                probe_count<JB, CAP, SA, SB>(sb, sub, tab, bst, lg, c, x, opaque_tid());
                 if (PAIRS) {
                     uint32_t total_c;
                     const uint32_t pre = block_exclusive_scan<JB>((uint32_t)c, scan_tmp, &total_c);
