@@ -167,9 +167,14 @@ __device__ __forceinline__ void load_tile(const void *in, int stride, uint64_t t
             }
             return;
         }
+        // the lane index behind an empty asm statement: the per-item offsets
+        // below are recomputed here instead of being hoisted out of the tile
+        // loop and kept live (or spilled) across it
+        uint32_t lane = threadIdx.x;
+        asm volatile("" : "+v"(lane));
 #pragma unroll
         for (int j = 0; j < PITEMS / 2; ++j) {
-            const uint64_t i0 = tb + 2 * ((uint64_t)j * PBLOCK + threadIdx.x);
+            const uint64_t i0 = tb + 2 * ((uint64_t)j * PBLOCK + lane);
             uint64_t a = 0, b = 0;
             if (i0 + 1 < hi) {
                 typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
@@ -295,8 +300,12 @@ __global__ __launch_bounds__(PBLOCK) void k_scatter(PassArgs a) {
 constexpr int PIPE_MAX_BITS = 10;
 
 // Tile loop of the pipelined scatter over input tuples [begin, end): digit
-// (key >> shift) & (F-1); `cur` holds the F output cursors and `cnt` is zero
-// (both visible to the whole work-group) on entry.
+// (key >> shift) & (F-1); `cur` holds the F output cursors and `cnt` (F + 1
+// counters, the last one a spare) is zero (both visible to the whole
+// work-group) on entry; `buf` holds PTILE + 1 tuples, the last one a spare.
+// Every phase is branch-free over a lane's items: a tuple outside the range
+// counts on the spare counter and is placed in the spare slot, so the LDS
+// reads of a phase are issued in batches (WB items) and waited for once.
 template <int STRIDE, int PBLOCK>
 __device__ __forceinline__ void scatter_tiles(const void *in, uint64_t begin, uint64_t end,
                                               uint64_t *out, int shift, uint32_t F,
@@ -304,6 +313,7 @@ __device__ __forceinline__ void scatter_tiles(const void *in, uint64_t begin, ui
                                               uint32_t *cur, uint32_t *s_len) {
     constexpr int PTILE = PBLOCK * PITEMS;
     constexpr int MAXPER = (1 << PIPE_MAX_BITS) / 64;
+    constexpr int WB = 4;  // items per batch of LDS reads in the place / write phases
     const uint32_t mask = F - 1;
     uint64_t base = tile_start(begin, STRIDE);
     if (base >= end) return;
@@ -311,11 +321,16 @@ __device__ __forceinline__ void scatter_tiles(const void *in, uint64_t begin, ui
     bool v[PITEMS];
     load_tile<PBLOCK>(in, STRIDE, base, begin, end, t, v);
     for (;;) {
-        uint32_t rank[PITEMS];
+        // ranks within the tile (< PTILE <= 2^13) packed two per register
+        uint32_t rank2[PITEMS / 2];
 #pragma unroll
         for (int j = 0; j < PITEMS; ++j) {
             const uint32_t d = (key_of(t[j]) >> shift) & mask;
-            rank[j] = v[j] ? atomicAdd(&cnt[d], 1u) : 0u;
+            const uint32_t r = atomicAdd(&cnt[v[j] ? d : F], 1u) & 0xffffu;
+            if (j & 1)
+                rank2[j / 2] |= r << 16;
+            else
+                rank2[j / 2] = r;
         }
         __syncthreads();
         if (threadIdx.x < 64) {
@@ -344,21 +359,37 @@ __device__ __forceinline__ void scatter_tiles(const void *in, uint64_t begin, ui
         }
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < PITEMS; ++j) {
-            if (v[j]) {
-                const uint32_t d = (key_of(t[j]) >> shift) & mask;
-                buf[cnt[d] + rank[j]] = t[j];
-            }
+        for (int j0 = 0; j0 < PITEMS; j0 += WB) {
+            uint32_t st[WB];
+#pragma unroll
+            for (int j = 0; j < WB; ++j)
+                st[j] = cnt[(key_of(t[j0 + j]) >> shift) & mask];
+#pragma unroll
+            for (int j = 0; j < WB; ++j)
+                buf[v[j0 + j] ? st[j] + ((rank2[(j0 + j) / 2] >> (16 * ((j0 + j) & 1))) & 0xffffu)
+                              : (uint32_t)PTILE] = t[j0 + j];
         }
         __syncthreads();
         const uint32_t len = *s_len;
         const uint64_t nbase = base + PTILE;
         const bool more = nbase < end;
         if (more) load_tile<PBLOCK>(in, STRIDE, nbase, begin, end, t, v);
-        for (uint32_t i = threadIdx.x; i < len; i += PBLOCK) {
-            const uint64_t u = buf[i];
-            const uint32_t d = (key_of(u) >> shift) & mask;
-            out[(uint64_t)(uint32_t)(dst[d] + i)] = u;
+#pragma unroll
+        for (int k0 = 0; k0 < PITEMS; k0 += WB) {
+            uint64_t u[WB];
+            uint32_t o[WB];
+#pragma unroll
+            for (int k = 0; k < WB; ++k) {
+                const uint32_t i = threadIdx.x + (uint32_t)(k0 + k) * PBLOCK;
+                u[k] = buf[i < len ? i : (uint32_t)PTILE];
+            }
+#pragma unroll
+            for (int k = 0; k < WB; ++k) o[k] = dst[(key_of(u[k]) >> shift) & mask];
+#pragma unroll
+            for (int k = 0; k < WB; ++k) {
+                const uint32_t i = threadIdx.x + (uint32_t)(k0 + k) * PBLOCK;
+                if (i < len) out[(uint64_t)(uint32_t)(o[k] + i)] = u[k];
+            }
         }
         for (uint32_t d = threadIdx.x; d < F; d += PBLOCK) cnt[d] = 0;
         __syncthreads();
@@ -375,10 +406,10 @@ __global__ __launch_bounds__(PBLOCK, 4) void k_scatter_pipe(PassArgs a) {
     ChunkLoc L;
     if (!locate_chunk(a, blockIdx.x, L)) return;
     const uint32_t F = 1u << a.bits[L.seg];
-    uint64_t *buf = smem;                                        // PTILE tuples
-    uint32_t *cnt = reinterpret_cast<uint32_t *>(buf + PTILE);  // F: counts, then tile starts
-    uint32_t *dst = cnt + F;                                     // F: cursor - tile start
-    uint32_t *cur = dst + F;                                     // F: global write cursors
+    uint64_t *buf = smem;                                            // PTILE + 1 tuples
+    uint32_t *cnt = reinterpret_cast<uint32_t *>(buf + PTILE + 1);  // F + 1: counts, tile starts
+    uint32_t *dst = cnt + F + 1;                                     // F: cursor - tile start
+    uint32_t *cur = dst + F;                                         // F: global write cursors
     // S offsets follow every R tuple that lies in a segment
     const uint32_t rel_base = L.rel ? (uint32_t)a.obase0[a.nseg] : 0u;
     const uint64_t hb = (L.rel ? a.hbase1 : a.hbase0)[L.seg];
@@ -386,6 +417,7 @@ __global__ __launch_bounds__(PBLOCK, 4) void k_scatter_pipe(PassArgs a) {
         cur[d] = a.hist_scan[hb + d * L.nchunk + L.c] - rel_base;
         cnt[d] = 0;
     }
+    if (threadIdx.x == 0) cnt[F] = 0;
     __syncthreads();
     scatter_tiles<STRIDE, PBLOCK>(L.rel ? a.in1 : a.in0, L.begin, L.end, L.rel ? a.out1 : a.out0,
                                   a.shift, F, buf, cnt, dst, cur, &s_len);
@@ -399,21 +431,23 @@ __global__ __launch_bounds__(PBLOCK, 4) void k_scatter_pipe(PassArgs a) {
 //                row sums
 //   k_p2_units   pass-2 work units: pass-1 partition d1 of a relation is
 //                split at pass-1 chunk boundaries into about size/unit units
-//   k_h2_usum    per unit and d2: the count over the unit's chunks (G), added
-//                to the (d1, d2) totals
-//   k_h2_base    final partition offsets: exclusive scan of the totals
+//   k_h2_usum    per unit and d2: the count over the unit's chunks (G)
+//   k_h2_base    one work-group per pass-1 partition: the (d1, d2) totals
+//                from its units' counts, their exclusive scan from the
+//                pass-1 partition start (the final partition offsets), and
+//                each unit's start cursor per d2 (written over G)
 //   k_scatter_p2 one work-group per unit: its pass-1 chunks' runs of
 //                partition d1 are contiguous in the pass-1 output; the
-//                cursors start at the final offsets plus the counts of the
-//                earlier units of the partition, so no pass-2 histogram is read
+//                cursors start where k_h2_base put them, so no pass-2
+//                histogram is read
 // This removes the pass-2 histogram read (8 bytes per tuple).
 struct P2Args {
     const uint64_t *in0, *in1;   // pass-1 output
     uint64_t *out0, *out1;       // pass-2 output
     int b1, b2;
     uint32_t *H;                 // [rel][d1][chunk][d2] counts
-    uint32_t *G;                 // [unit][d2] counts over the unit's chunks
-    uint32_t *T;                 // [rel][d1][d2] totals (zeroed, then summed from G)
+    uint32_t *G;                 // [unit][d2] counts over the unit's chunks, then start cursors
+    uint32_t *T;                 // unused (kept for the workspace layout)
     uint64_t *poff2_0, *poff2_1; // F1*F2+1 final offsets per relation
     uint32_t *ubase;             // 2*F1+1 unit bases
     const uint64_t *poff1_0, *poff1_1;  // pass-1 partition offsets
@@ -517,63 +551,79 @@ __global__ __launch_bounds__(H2B_BLOCK) void k_p2_units(P2Args p) {
     if (threadIdx.x == 0) p.ubase[2 * F1] = run;
 }
 
-// G[u][d2] = sum of H[rel][d1][c][d2] over the unit's chunks: lanes split the
-// chunks (stride lanes = USUM_BLOCK / F2 rows per step), then an LDS reduce.
-constexpr int USUM_BLOCK = 512;
+// G[u][d2] = sum of H[rel][d1][c][d2] over the unit's chunks: one lane per d2
+// (several when F2 exceeds the work-group), four independent loads in flight.
+constexpr int USUM_BLOCK = 128;
 __global__ __launch_bounds__(USUM_BLOCK) void k_h2_usum(P2Args p) {
-    __shared__ uint32_t red[USUM_BLOCK];
     const uint32_t F1 = 1u << p.b1, F2 = 1u << p.b2;
     const uint32_t u = blockIdx.x;
     if (u >= p.ubase[2 * F1]) return;
     const Unit x = unit_of(p, u);
     const uint32_t *row0 = p.H + h_row(p, x.rel, x.d1, 0);
-    uint32_t *tot = p.T + ((uint64_t)x.idx << p.b2);  // [rel][d1] row
-    if (F2 >= USUM_BLOCK) {
-        for (uint32_t d2 = threadIdx.x; d2 < F2; d2 += USUM_BLOCK) {
-            uint32_t sum = 0;
-            for (uint64_t c = x.c_lo; c < x.c_hi; ++c) sum += row0[(c << p.b2) + d2];
-            p.G[((uint64_t)u << p.b2) + d2] = sum;
-            if (sum) atomicAdd(&tot[d2], sum);
+    for (uint32_t d2 = threadIdx.x; d2 < F2; d2 += USUM_BLOCK) {
+        uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+        uint64_t c = x.c_lo;
+        for (; c + 4 <= x.c_hi; c += 4) {
+            s0 += row0[(c << p.b2) + d2];
+            s1 += row0[((c + 1) << p.b2) + d2];
+            s2 += row0[((c + 2) << p.b2) + d2];
+            s3 += row0[((c + 3) << p.b2) + d2];
         }
-        return;
-    }
-    const uint32_t lanes = USUM_BLOCK >> p.b2;  // chunk rows per step
-    const uint32_t d2 = threadIdx.x & (F2 - 1), cl = threadIdx.x >> p.b2;
-    uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-    uint64_t c = x.c_lo + cl;
-    for (; c + 3 * lanes < x.c_hi; c += 4 * lanes) {  // four independent loads in flight
-        s0 += row0[(c << p.b2) + d2];
-        s1 += row0[((c + lanes) << p.b2) + d2];
-        s2 += row0[((c + 2 * lanes) << p.b2) + d2];
-        s3 += row0[((c + 3 * lanes) << p.b2) + d2];
-    }
-    for (; c < x.c_hi; c += lanes) s0 += row0[(c << p.b2) + d2];
-    red[threadIdx.x] = s0 + s1 + s2 + s3;
-    __syncthreads();
-    if (threadIdx.x < F2) {
-        uint32_t sum = 0;
-        for (uint32_t k = 0; k < lanes; ++k) sum += red[(k << p.b2) + threadIdx.x];
-        p.G[((uint64_t)u << p.b2) + threadIdx.x] = sum;
-        if (sum) atomicAdd(&tot[threadIdx.x], sum);
+        for (; c < x.c_hi; ++c) s0 += row0[(c << p.b2) + d2];
+        p.G[((uint64_t)u << p.b2) + d2] = s0 + s1 + s2 + s3;
     }
 }
 
-// Final offsets per relation: exclusive scan of the (d1, d2) totals.
-__global__ __launch_bounds__(H2B_BLOCK) void k_h2_base(P2Args p) {
-    __shared__ uint64_t tmp[H2B_BLOCK / 64];
-    const uint32_t F1 = 1u << p.b1, F2 = 1u << p.b2, FF = F1 * F2;
-    const int rel = blockIdx.x;
+// One work-group per (relation, pass-1 partition d1), one lane per d2 (F2 <=
+// H2B_LANES, or several d2 per lane): the (d1, d2) total is the sum of the
+// counts of d1's units; the final offsets are the pass-1 partition start plus
+// the exclusive scan of the totals over d2; unit j of d1 starts writing d2 at
+// the final offset plus the counts of units 0..j-1, which replace the counts
+// in G.  The last work-group of each relation also writes the end offset.
+constexpr int H2B_LANES = 256;
+__global__ __launch_bounds__(H2B_LANES) void k_h2_base(P2Args p) {
+    __shared__ uint64_t tmp[H2B_LANES / 64];
+    const uint32_t F1 = 1u << p.b1, F2 = 1u << p.b2;
+    const uint32_t idx = blockIdx.x;  // rel * F1 + d1
+    if (idx >= 2 * F1) return;
+    const int rel = idx >= F1;
+    const uint32_t d1 = idx - (rel ? F1 : 0);
+    const uint64_t *po1 = rel ? p.poff1_1 : p.poff1_0;
     uint64_t *po = rel ? p.poff2_1 : p.poff2_0;
-    uint64_t run = 0;
-    for (uint32_t base = 0; base < FF; base += H2B_BLOCK) {
-        const uint32_t i = base + threadIdx.x;
-        const uint64_t v = i < FF ? p.T[(rel ? (uint64_t)FF : 0) + i] : 0;
-        uint64_t total;
-        const uint64_t pre = block_exclusive_scan<H2B_BLOCK>(v, tmp, &total);
-        if (i < FF) po[i] = run + pre;
-        run += total;
+    const uint32_t u0 = p.ubase[idx], u1 = p.ubase[idx + 1];
+    uint64_t run = po1[d1];
+    for (uint32_t base = 0; base < F2; base += H2B_LANES) {
+        const uint32_t d2 = base + threadIdx.x;
+        const bool ok = d2 < F2;
+        uint32_t total = 0;
+        uint32_t k = u0;
+        for (; k + 4 <= u1; k += 4) {  // four independent loads in flight
+            const uint32_t *g = p.G + ((uint64_t)k << p.b2) + (ok ? d2 : 0);
+            total += g[0] + g[F2] + g[2 * F2] + g[3 * F2];
+        }
+        for (; k < u1; ++k) total += p.G[((uint64_t)k << p.b2) + (ok ? d2 : 0)];
+        if (!ok) total = 0;
+        uint64_t sum;
+        const uint64_t pre = block_exclusive_scan<H2B_LANES>((uint64_t)total, tmp, &sum);
+        const uint64_t start = run + pre;
+        if (ok) {
+            po[((uint64_t)d1 << p.b2) + d2] = start;
+            uint64_t c = start;
+            for (uint32_t k2 = u0; k2 < u1; k2 += 8) {  // eight loads in flight
+                uint32_t n[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    n[q] = k2 + q < u1 ? p.G[((uint64_t)(k2 + q) << p.b2) + d2] : 0u;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    if (k2 + q < u1) p.G[((uint64_t)(k2 + q) << p.b2) + d2] = (uint32_t)c;
+                    c += n[q];
+                }
+            }
+        }
+        run += sum;
     }
-    if (threadIdx.x == 0) po[FF] = run;
+    if (d1 == F1 - 1 && threadIdx.x == 0) po[(uint64_t)F1 << p.b2] = po1[F1];
 }
 
 template <int PBLOCK>
@@ -592,18 +642,15 @@ __global__ __launch_bounds__(PBLOCK, 4) void k_scatter_p2(P2Args p) {
     const uint64_t begin = p.hist_scan[hb + x.d1 * x.C + x.c_lo] - rel_base;
     const uint64_t end =
         x.c_hi < x.C ? p.hist_scan[hb + x.d1 * x.C + x.c_hi] - rel_base : po1[x.d1 + 1];
-    uint64_t *buf = smem;
-    uint32_t *cnt = reinterpret_cast<uint32_t *>(buf + PTILE);
-    uint32_t *dst = cnt + F2;
+    uint64_t *buf = smem;                                            // PTILE + 1 tuples
+    uint32_t *cnt = reinterpret_cast<uint32_t *>(buf + PTILE + 1);  // F2 + 1 counters
+    uint32_t *dst = cnt + F2 + 1;
     uint32_t *cur = dst + F2;
-    const uint64_t *po2 = x.rel ? p.poff2_1 : p.poff2_0;
-    const uint32_t u0 = p.ubase[x.idx];  // first unit of this partition
     for (uint32_t d = threadIdx.x; d < F2; d += PBLOCK) {
-        uint64_t c = po2[((uint64_t)x.d1 << p.b2) + d];
-        for (uint32_t k = u0; k < u; ++k) c += p.G[((uint64_t)k << p.b2) + d];
-        cur[d] = (uint32_t)c;
+        cur[d] = p.G[((uint64_t)u << p.b2) + d];
         cnt[d] = 0;
     }
+    if (threadIdx.x == 0) cnt[F2] = 0;
     __syncthreads();
     scatter_tiles<8, PBLOCK>(x.rel ? (const void *)p.in1 : (const void *)p.in0, begin, end,
                              x.rel ? p.out1 : p.out0, p.b1, F2, buf, cnt, dst, cur, &s_len);
@@ -671,7 +718,8 @@ static int launch_pass_kernels(const PassDesc &d, const PassWork &w, const PassA
         sizeof(uint64_t) * PBLOCK * PITEMS + sizeof(uint32_t) * (2 * F + 1);
     AJ_TRY(lds_limit_set((const void *)k_hist<STRIDE, PBLOCK>, lds_hist));
     AJ_TRY(lds_limit_set((const void *)k_scatter<STRIDE, PBLOCK>, lds_scatter));
-    const size_t lds_pipe = sizeof(uint64_t) * PBLOCK * PITEMS + sizeof(uint32_t) * 3 * F;
+    const size_t lds_pipe =
+        sizeof(uint64_t) * (PBLOCK * PITEMS + 1) + sizeof(uint32_t) * (3 * F + 1);
     if (d.max_bits <= PIPE_MAX_BITS)
         AJ_TRY(lds_limit_set((const void *)k_scatter_pipe<STRIDE, PBLOCK>, lds_pipe));
     if (mk) mk->mark(m0);
@@ -794,7 +842,8 @@ int run_partition_pass(const PassDesc &d, const PassWork &w, hipStream_t st, con
 
 template <int PBLOCK>
 static int launch_scatter_p2(const P2Args &p, uint64_t max_units, hipStream_t st) {
-    const size_t lds = sizeof(uint64_t) * PBLOCK * PITEMS + sizeof(uint32_t) * 3 * (1u << p.b2);
+    const size_t lds =
+        sizeof(uint64_t) * (PBLOCK * PITEMS + 1) + sizeof(uint32_t) * (3 * (1u << p.b2) + 1);
     AJ_TRY(lds_limit_set((const void *)k_scatter_p2<PBLOCK>, lds));
     hipLaunchKernelGGL((k_scatter_p2<PBLOCK>), dim3((unsigned)max_units), dim3(PBLOCK), lds, st, p);
     return AJ_OK;
@@ -832,10 +881,9 @@ int run_pass2_2d(const P2Desc &d, hipStream_t st, const Marks *mk, int m0) {
         return AJ_ERR_ARG;
     }
     if (mk) mk->mark(m0);
-    AJ_TRY(fill_u32(p.T, 2ull << (d.b1 + d.b2), 0u, st));
     hipLaunchKernelGGL(k_p2_units, dim3(1), dim3(H2B_BLOCK), 0, st, p);
     hipLaunchKernelGGL(k_h2_usum, dim3((unsigned)d.max_units), dim3(USUM_BLOCK), 0, st, p);
-    hipLaunchKernelGGL(k_h2_base, dim3(2), dim3(H2B_BLOCK), 0, st, p);
+    hipLaunchKernelGGL(k_h2_base, dim3(2u << d.b1), dim3(H2B_LANES), 0, st, p);
     if (mk) {
         mk->mark(m0 + 1);
         mk->mark(m0 + 2);
