@@ -149,16 +149,16 @@ __device__ __forceinline__ bool locate_chunk(const PassArgs &a, uint64_t g, Chun
 // 8-byte tuples are read two per lane with 16-byte loads (consecutive lanes on
 // consecutive 16 bytes); a load never reaches past `hi`, so nothing outside
 // the segment's array is touched.  12-byte stream tuples are read one per lane.
-template <int PBLOCK>
+template <int PBLOCK, int NI = PITEMS>
 __device__ __forceinline__ void load_tile(const void *in, int stride, uint64_t tb, uint64_t lo,
-                                          uint64_t hi, uint64_t (&t)[PITEMS],
-                                          bool (&v)[PITEMS]) {
+                                          uint64_t hi, uint64_t (&t)[NI],
+                                          bool (&v)[NI]) {
     if (stride == 8) {
         typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
         const uint64_t *p = static_cast<const uint64_t *>(in);
-        if (tb >= lo && tb + PBLOCK * PITEMS <= hi) {  // whole tile inside: unconditional 16-byte loads
+        if (tb >= lo && tb + PBLOCK * NI <= hi) {  // whole tile inside: unconditional 16-byte loads
 #pragma unroll
-            for (int j = 0; j < PITEMS / 2; ++j) {
+            for (int j = 0; j < NI / 2; ++j) {
                 const u64x2 q = __builtin_nontemporal_load(
                     reinterpret_cast<const u64x2 *>(p + tb + 2 * ((uint64_t)j * PBLOCK + threadIdx.x)));
                 t[2 * j] = q.x;
@@ -173,7 +173,7 @@ __device__ __forceinline__ void load_tile(const void *in, int stride, uint64_t t
         uint32_t lane = threadIdx.x;
         asm volatile("" : "+v"(lane));
 #pragma unroll
-        for (int j = 0; j < PITEMS / 2; ++j) {
+        for (int j = 0; j < NI / 2; ++j) {
             const uint64_t i0 = tb + 2 * ((uint64_t)j * PBLOCK + lane);
             uint64_t a = 0, b = 0;
             if (i0 + 1 < hi) {
@@ -191,7 +191,7 @@ __device__ __forceinline__ void load_tile(const void *in, int stride, uint64_t t
         }
     } else {
 #pragma unroll
-        for (int j = 0; j < PITEMS; ++j) {
+        for (int j = 0; j < NI; ++j) {
             const uint64_t i = tb + (uint64_t)j * PBLOCK + threadIdx.x;
             v[j] = i >= lo && i < hi;
             t[j] = v[j] ? load_tuple(in, stride, i) : 0;
@@ -488,7 +488,8 @@ __device__ __forceinline__ Unit unit_of(const P2Args &p, uint32_t u) {
 
 template <int PBLOCK>
 __global__ __launch_bounds__(PBLOCK) void k_hist2d(PassArgs a, P2Args p) {
-    constexpr int PTILE = PBLOCK * PITEMS;
+    constexpr int HITEMS = 8;  // tuples per lane per tile (fewer registers than the scatter)
+    constexpr int PTILE = PBLOCK * HITEMS;
     extern __shared__ uint32_t h[];
     ChunkLoc L;
     if (!locate_chunk(a, blockIdx.x, L)) return;
@@ -498,11 +499,11 @@ __global__ __launch_bounds__(PBLOCK) void k_hist2d(PassArgs a, P2Args p) {
     __syncthreads();
     const void *in = L.rel ? a.in1 : a.in0;
     for (uint64_t base = tile_start(L.begin, 8); base < L.end; base += PTILE) {
-        uint64_t t[PITEMS];
-        bool v[PITEMS];
-        load_tile<PBLOCK>(in, 8, base, L.begin, L.end, t, v);
+        uint64_t t[HITEMS];
+        bool v[HITEMS];
+        load_tile<PBLOCK, HITEMS>(in, 8, base, L.begin, L.end, t, v);
 #pragma unroll
-        for (int j = 0; j < PITEMS; ++j) {
+        for (int j = 0; j < HITEMS; ++j) {
             const uint32_t k = key_of(t[j]);
             if (v[j]) atomicAdd(&h[((k & m1) << p.b2) | ((k >> p.b1) & m2)], 1u);
         }
