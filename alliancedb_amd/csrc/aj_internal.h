@@ -113,6 +113,8 @@ struct PassDesc {
     int max_bits;
     uint32_t chunk;           // tuples per work-group (multiple of the largest tile)
     int block = 512;          // lanes per histogram/scatter work-group: 256 or 512
+    bool exact_single = false;  // one segment of exactly n_total[0] / n_total[1] tuples
+                                // (the plan is then computed from the host's sizes)
     uint64_t n_total[2];      // total tuples of R and S in this pass
     uint64_t max_chunks;      // upper bound of chunks over R+S (grid size)
     uint64_t max_hist;        // upper bound of histogram entries

@@ -78,14 +78,21 @@ struct JoinArgs {
 // Per segment k: count/checksum = sums over its tasks, which are contiguous:
 // [task_base[part_seg_base[k]], task_base[part_seg_base[k+1]]).  One block per
 // segment (grid-stride), plain stores: the accumulators need no zeroing.
-constexpr int RBLOCK = 256;
+constexpr int RBLOCK = 1024;
 __global__ __launch_bounds__(RBLOCK) void k_seg_reduce(JoinArgs a) {
     __shared__ uint64_t red[2 * (RBLOCK / 64)];
     for (int k = blockIdx.x; k < a.nseg; k += gridDim.x) {
         const uint32_t tb = a.task_base[a.part_seg_base[k]];
         const uint32_t te = a.task_base[a.part_seg_base[k + 1]];
         uint64_t c = 0, x = 0;
-        for (uint32_t t = tb + threadIdx.x; t < te; t += RBLOCK) {
+        uint32_t t = tb + threadIdx.x;
+        for (; t + 3 * RBLOCK < te; t += 4 * RBLOCK) {  // four loads in flight
+            const ulonglong2 r0 = a.task_res[t], r1 = a.task_res[t + RBLOCK];
+            const ulonglong2 r2 = a.task_res[t + 2 * RBLOCK], r3 = a.task_res[t + 3 * RBLOCK];
+            c += r0.x + r1.x + r2.x + r3.x;
+            x += r0.y + r1.y + r2.y + r3.y;
+        }
+        for (; t < te; t += RBLOCK) {
             const ulonglong2 r = a.task_res[t];
             c += r.x;
             x += r.y;

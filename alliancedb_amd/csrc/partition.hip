@@ -111,6 +111,30 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_plan(PassArgs a) {
     }
 }
 
+// k_plan for one segment of known sizes nR, nS (the host computed the chunk
+// counts c0, c1 and the digit count F): one lane writes the same plan.
+__global__ void k_plan_single(PassArgs a, uint64_t nR, uint64_t nS, uint64_t c0, uint64_t c1,
+                              uint32_t F) {
+    if (threadIdx.x != 0) return;
+    a.cpre0[0] = 0;
+    a.cpre0[1] = c0;
+    a.cpre1[0] = 0;
+    a.cpre1[1] = c1;
+    a.hbase0[0] = 0;
+    a.hbase1[0] = (uint64_t)F * c0;
+    a.obase0[0] = 0;
+    a.obase0[1] = nR;
+    a.obase1[0] = 0;
+    a.obase1[1] = nS;
+    a.pbase[0] = 0;
+    a.pbase[1] = F;
+    a.header[0] = c0;
+    a.header[1] = c1;
+    a.header[2] = (uint64_t)F * c0;
+    a.header[3] = (uint64_t)F * c1;
+    a.header[4] = F;
+}
+
 // Zero the histogram tail beyond the entries this launch uses (so the flat
 // scan over the host-side bound stays correct).
 __global__ void k_zero_tail(uint32_t *hist, const uint64_t *header, uint64_t max_hist) {
@@ -835,8 +859,20 @@ int run_partition_pass(const PassDesc &d, const PassWork &w, hipStream_t st, con
     a.max_hist = d.max_hist;
     a.max_parts = d.max_parts;
 
-    hipLaunchKernelGGL(k_plan, dim3(1), dim3(PLAN_BLOCK), 0, st, a);
-    hipLaunchKernelGGL(k_zero_tail, dim3(256), dim3(256), 0, st, a.hist, a.header, d.max_hist);
+    if (d.exact_single && d.nseg == 1) {
+        const uint32_t F = 1u << d.max_bits;
+        const uint64_t c0 = (d.n_total[0] + d.chunk - 1) / d.chunk;
+        const uint64_t c1 = (d.n_total[1] + d.chunk - 1) / d.chunk;
+        hipLaunchKernelGGL(k_plan_single, dim3(1), dim3(64), 0, st, a, d.n_total[0],
+                           d.n_total[1], c0, c1, F);
+        if ((c0 + c1) * F < d.max_hist)  // the scan covers max_hist entries
+            hipLaunchKernelGGL(k_zero_tail, dim3(256), dim3(256), 0, st, a.hist, a.header,
+                               d.max_hist);
+    } else {
+        hipLaunchKernelGGL(k_plan, dim3(1), dim3(PLAN_BLOCK), 0, st, a);
+        hipLaunchKernelGGL(k_zero_tail, dim3(256), dim3(256), 0, st, a.hist, a.header,
+                           d.max_hist);
+    }
     const bool s8 = d.in_stride == 8;
     int rc;
     if (d.block == 512)

@@ -272,6 +272,7 @@ int pro_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
         }
         d1.n_total[0] = nR;
         d1.n_total[1] = nS;
+        d1.exact_single = true;
         d1.max_chunks = p.max_chunks1;
         d1.max_hist = p.max_hist1;
         d1.max_parts = p.max_parts1;
