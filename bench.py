@@ -246,8 +246,10 @@ def run_pro(aj, args):
         line["config"]["geometry"] = {k: tuned[k] for k in ("radix_bits", "passes", "cap",
                                                             "part_block")}
         line["config"]["geometry"]["chosen_by"] = "autotuner (aj_tune_pro)"
-    default_geom = not (args.radix_bits or args.passes or args.cap) and (
-        not tuned or (tuned["radix_bits"], tuned["passes"], tuned["cap"]) == (14, 2, 8704))
+    # the traffic summary covers the partition geometry (14 bits in 7 + 7, 512-lane
+    # pass 2); the LDS round size only changes the join kernel
+    default_geom = algo == aj.AJ_ALGO_PRO and not (args.radix_bits or args.passes) and (
+        not tuned or (tuned["radix_bits"], tuned["passes"], tuned["part_block"]) == (14, 2, 512))
     if default_geom:
         line["roofline"]["traffic"] = measured_traffic(line["roofline"]["kernel"], name)
     if algo == aj.AJ_ALGO_SHJ:
