@@ -284,7 +284,16 @@ def run_skew(aj, args):
     if tuned:
         line["config"]["geometry"] = {k: tuned[k] for k in ("radix_bits", "passes", "cap",
                                                             "part_block")}
+        relabel_scatter(line, tuned)
     return line
+
+
+def relabel_scatter(line, geom):
+    """Passes of more than 10 radix bits run the non-pipelined k_scatter."""
+    per_pass = -(-geom["radix_bits"] // max(geom["passes"], 1))
+    if per_pass > 10 and line["roofline"]["kernel"].startswith("k_scatter_pipe"):
+        line["roofline"]["kernel"] = line["roofline"]["kernel"].replace("k_scatter_pipe",
+                                                                        "k_scatter")
 
 
 def run_windows(aj, args):
