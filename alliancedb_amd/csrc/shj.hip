@@ -26,8 +26,6 @@
 
 namespace aj {
 
-namespace {
-
 constexpr int SBLOCK = 256;
 
 // Insert v into tab (the all-ones tuple is counted in *own_sent instead) and
@@ -86,24 +84,33 @@ __global__ __launch_bounds__(SBLOCK) void k_shj_step(const uint64_t *in, uint64_
     }
 }
 
-// One work-group: prog[k] = matches of steps 0..k from the per-block rows
-// step_cnt[k * G + b] (a thread per step, 256 steps per round, carried scan);
-// acc[0] = all matches, acc[1] = the checksum.
-__global__ __launch_bounds__(SBLOCK) void k_shj_finish(const uint64_t *step_cnt, uint64_t steps,
-                                                       unsigned G, const uint64_t *sum_part,
-                                                       uint64_t *prog, uint64_t *acc) {
+// Work-group k: rows[k] = matches of step k, the sum of the per-block row
+// step_cnt[k * G + b] (coalesced reads across the work-group).
+__global__ __launch_bounds__(SBLOCK) void k_shj_rows(const uint64_t *step_cnt, unsigned G,
+                                                     uint64_t *rows) {
+    __shared__ uint64_t red[SBLOCK / 64];
+    const uint64_t k = blockIdx.x;
+    uint64_t c = 0;
+    for (unsigned b = threadIdx.x; b < G; b += SBLOCK) c += step_cnt[k * G + b];
+    c = block_sum<SBLOCK>(c, red);
+    if (threadIdx.x == 0) rows[k] = c;
+}
+
+// One work-group: prog[k] = matches of steps 0..k (in place over the step
+// rows, a carried block scan 256 steps at a time); acc[0] = all matches,
+// acc[1] = the checksum from the per-block partials.
+__global__ __launch_bounds__(SBLOCK) void k_shj_finish(uint64_t steps, unsigned G,
+                                                       const uint64_t *sum_part, uint64_t *prog,
+                                                       uint64_t *acc) {
     __shared__ uint64_t red[SBLOCK / 64 + 1];
     uint64_t carry = 0;
     for (uint64_t base = 0; base < steps; base += SBLOCK) {
         const uint64_t k = base + threadIdx.x;
-        uint64_t c = 0;
-        if (k < steps)
-            for (unsigned b = 0; b < G; ++b) c += step_cnt[k * G + b];
+        const uint64_t c = k < steps ? prog[k] : 0;
         uint64_t total;
         const uint64_t pre = block_exclusive_scan<SBLOCK>(c, red, &total);
         if (k < steps) prog[k] = carry + pre + c;
         carry += total;
-        __syncthreads();
     }
     uint64_t x = 0;
     for (unsigned b = threadIdx.x; b < G; b += SBLOCK) x += sum_part[b];
@@ -113,6 +120,8 @@ __global__ __launch_bounds__(SBLOCK) void k_shj_finish(const uint64_t *step_cnt,
         acc[1] = x;
     }
 }
+
+namespace {
 
 unsigned shj_grid(uint64_t n, int num_cu) {
     uint64_t g = (n + SBLOCK - 1) / SBLOCK;
@@ -177,8 +186,10 @@ int shj_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
                                (const uint64_t *)(acc + 2), 0, sum_part, scnt + k * G);
         }
     }
-    hipLaunchKernelGGL(k_shj_finish, dim3(1), dim3(SBLOCK), 0, st, (const uint64_t *)scnt, steps,
-                       G, (const uint64_t *)sum_part, prog, acc);
+    hipLaunchKernelGGL(k_shj_rows, dim3((unsigned)steps), dim3(SBLOCK), 0, st,
+                       (const uint64_t *)scnt, G, prog);
+    hipLaunchKernelGGL(k_shj_finish, dim3(1), dim3(SBLOCK), 0, st, steps, G,
+                       (const uint64_t *)sum_part, prog, acc);
     AJ_HIP(hipGetLastError());
     AJ_HIP(hipEventRecord(ctx->ev1, st));
     uint64_t host[2];
