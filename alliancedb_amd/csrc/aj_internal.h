@@ -152,7 +152,7 @@ struct P2Desc {
     int b1, b2;
     uint32_t *hist2d;            // as filled by pass 1
     uint32_t *unit_counts;       // workspace: max_units * 2^b2
-    uint32_t *totals;            // workspace: 2 * 2^(b1+b2)
+    uint64_t *unit_span;         // workspace: 2 * max_units (input begin | rel << 63, end)
     uint64_t *poff2[2];          // out: 2^(b1+b2)+1 final partition offsets per relation
     uint32_t *ubase;             // workspace: 2 * 2^b1 + 1
     const PassWork *w1;          // pass-1 workspace (offsets, header)

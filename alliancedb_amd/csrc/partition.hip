@@ -455,7 +455,8 @@ __global__ __launch_bounds__(PBLOCK, 4) void k_scatter_pipe(PassArgs a) {
 //                row sums
 //   k_p2_units   pass-2 work units: pass-1 partition d1 of a relation is
 //                split at pass-1 chunk boundaries into about size/unit units
-//   k_h2_usum    per unit and d2: the count over the unit's chunks (G)
+//   k_h2_usum    per unit and d2: the count over the unit's chunks (G);
+//                per unit its input span in the pass-1 output
 //   k_h2_base    one work-group per pass-1 partition: the (d1, d2) totals
 //                from its units' counts, their exclusive scan from the
 //                pass-1 partition start (the final partition offsets), and
@@ -471,7 +472,7 @@ struct P2Args {
     int b1, b2;
     uint32_t *H;                 // [rel][d1][chunk][d2] counts
     uint32_t *G;                 // [unit][d2] counts over the unit's chunks, then start cursors
-    uint32_t *T;                 // unused (kept for the workspace layout)
+    uint64_t *span;              // [unit] {input begin | rel << 63, input end}
     uint64_t *poff2_0, *poff2_1; // F1*F2+1 final offsets per relation
     uint32_t *ubase;             // 2*F1+1 unit bases
     const uint64_t *poff1_0, *poff1_1;  // pass-1 partition offsets
@@ -584,6 +585,17 @@ __global__ __launch_bounds__(USUM_BLOCK) void k_h2_usum(P2Args p) {
     const uint32_t u = blockIdx.x;
     if (u >= p.ubase[2 * F1]) return;
     const Unit x = unit_of(p, u);
+    if (threadIdx.x == 0) {
+        // the runs of partition d1 written by pass-1 chunks [c_lo, c_hi)
+        const uint64_t hb = (x.rel ? p.hbase1 : p.hbase0)[0];
+        const uint64_t rel_base = x.rel ? p.obase0[1] : 0;
+        const uint64_t *po1 = x.rel ? p.poff1_1 : p.poff1_0;
+        const uint64_t begin = p.hist_scan[hb + x.d1 * x.C + x.c_lo] - rel_base;
+        const uint64_t end =
+            x.c_hi < x.C ? p.hist_scan[hb + x.d1 * x.C + x.c_hi] - rel_base : po1[x.d1 + 1];
+        p.span[2 * (uint64_t)u] = begin | ((uint64_t)x.rel << 63);
+        p.span[2 * (uint64_t)u + 1] = end;
+    }
     const uint32_t *row0 = p.H + h_row(p, x.rel, x.d1, 0);
     for (uint32_t d2 = threadIdx.x; d2 < F2; d2 += USUM_BLOCK) {
         uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
@@ -659,14 +671,10 @@ __global__ __launch_bounds__(PBLOCK, 4) void k_scatter_p2(P2Args p) {
     const uint32_t F1 = 1u << p.b1, F2 = 1u << p.b2;
     const uint32_t u = blockIdx.x;
     if (u >= p.ubase[2 * F1]) return;
-    const Unit x = unit_of(p, u);
-    // input: the runs of partition d1 written by pass-1 chunks [c_lo, c_hi)
-    const uint64_t hb = (x.rel ? p.hbase1 : p.hbase0)[0];
-    const uint64_t rel_base = x.rel ? p.obase0[1] : 0;
-    const uint64_t *po1 = x.rel ? p.poff1_1 : p.poff1_0;
-    const uint64_t begin = p.hist_scan[hb + x.d1 * x.C + x.c_lo] - rel_base;
-    const uint64_t end =
-        x.c_hi < x.C ? p.hist_scan[hb + x.d1 * x.C + x.c_hi] - rel_base : po1[x.d1 + 1];
+    // input span from the planning (one load instead of the unit search)
+    const uint64_t s0 = p.span[2 * (uint64_t)u], end = p.span[2 * (uint64_t)u + 1];
+    const int rel = (int)(s0 >> 63);
+    const uint64_t begin = s0 & ~(1ull << 63);
     uint64_t *buf = smem;                                            // PTILE + 1 tuples
     uint32_t *cnt = reinterpret_cast<uint32_t *>(buf + PTILE + 1);  // F2 + 1 counters
     uint32_t *dst = cnt + F2 + 1;
@@ -677,8 +685,8 @@ __global__ __launch_bounds__(PBLOCK, 4) void k_scatter_p2(P2Args p) {
     }
     if (threadIdx.x == 0) cnt[F2] = 0;
     __syncthreads();
-    scatter_tiles<8, PBLOCK>(x.rel ? (const void *)p.in1 : (const void *)p.in0, begin, end,
-                             x.rel ? p.out1 : p.out0, p.b1, F2, buf, cnt, dst, cur, &s_len);
+    scatter_tiles<8, PBLOCK>(rel ? (const void *)p.in1 : (const void *)p.in0, begin, end,
+                             rel ? p.out1 : p.out0, p.b1, F2, buf, cnt, dst, cur, &s_len);
 }
 
 // ---------------------------------------------------------------- k_part_off
@@ -912,7 +920,7 @@ int run_pass2_2d(const P2Desc &d, hipStream_t st, const Marks *mk, int m0) {
     p.b2 = d.b2;
     p.H = d.hist2d;
     p.G = d.unit_counts;
-    p.T = d.totals;
+    p.span = d.unit_span;
     p.poff2_0 = d.poff2[0];
     p.poff2_1 = d.poff2[1];
     p.ubase = d.ubase;
@@ -924,7 +932,7 @@ int run_pass2_2d(const P2Desc &d, hipStream_t st, const Marks *mk, int m0) {
     p.obase0 = d.w1->seg_out_base[0];
     p.unit = d.unit;
     p.hist_scan = d.hist_scan;
-    if (!p.hist_scan || !p.H || !p.G || !p.T || !p.ubase) {
+    if (!p.hist_scan || !p.H || !p.G || !p.span || !p.ubase) {
         set_error(AJ_ERR_ARG, "run_pass2_2d: missing workspace");
         return AJ_ERR_ARG;
     }

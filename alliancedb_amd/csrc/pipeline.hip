@@ -110,7 +110,7 @@ struct ProPlan {
     bool p2d;                 // pass 2 from the 2-D pass-1 histogram
     size_t o_tot, o_ub;
     uint64_t max_units, unit, max_tasks;
-    size_t o_h2tot;
+    size_t o_uspan;
     size_t bytes;
     size_t scan_bytes;
 };
@@ -171,7 +171,7 @@ static ProPlan plan_pro(uint64_t nR, uint64_t nS, const ProConfig &c, int num_cu
         p.max_units = (nR + nS) / p.unit + 2 * F1 + 1;
         p.o_tot = L.add<uint32_t>(p.max_units * F2);
         p.o_ub = L.add<uint32_t>(2 * F1 + 2);
-        p.o_h2tot = L.add<uint32_t>(2 * F1 * F2);
+        p.o_uspan = L.add<uint64_t>(2 * p.max_units);
     }
     p.bytes = L.off;
     p.scan_bytes = std::max({pass_scan_tmp_bytes(std::max<uint64_t>(p.max_hist1, 1)),
@@ -303,7 +303,7 @@ int pro_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
             q.b2 = p.b2;
             q.hist2d = ctx->hist2d.as<uint32_t>();
             q.unit_counts = at<uint32_t>(B, p.o_tot);
-            q.totals = at<uint32_t>(B, p.o_h2tot);
+            q.unit_span = at<uint64_t>(B, p.o_uspan);
             for (int r = 0; r < 2; ++r) q.poff2[r] = at<uint64_t>(B, p.o_poff2[r]);
             q.ubase = at<uint32_t>(B, p.o_ub);
             q.w1 = &w1;
