@@ -172,7 +172,8 @@ struct JoinDesc {
     uint64_t max_parts;
     const uint32_t *part_seg_base;  // nseg+1: first partition of each segment
     int nseg;
-    int hash_shift;                 // unused bits below this are partition bits
+    int hash_shift;                 // radix bits of every partition (bucket hash skips them)
+    const uint32_t *seg_bits = nullptr;  // nullable: radix bits per segment instead
     int cap;                        // R tuples per LDS round
     uint32_t split;                 // S tuples per task
     uint64_t *seg_count, *seg_sum;  // nseg accumulators (zeroed by the caller)

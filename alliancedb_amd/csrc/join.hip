@@ -70,6 +70,8 @@ struct JoinArgs {
     uint64_t pairs_cap;
     uint64_t *pair_cursor;
     int records;  // 0: pairs; 1: the S tuple <S.key, S.payloadID>; 2: (clock << 32 | S.payloadID)
+    int hash_shift;             // radix bits of every partition (when seg_bits is null)
+    const uint32_t *seg_bits;   // nullable: radix bits per segment
     uint32_t *task_cnt, *task_base, *task_next;
     uint4 *task_info;
     ulonglong2 *task_res;  // per task {count, checksum}
@@ -144,8 +146,9 @@ __global__ void k_task_fill(JoinArgs a) {
         const uint64_t r0 = a.roff[p], nr = a.roff[p + 1] - r0;
         const uint64_t s0 = a.soff[p] + (uint64_t)j * a.split;
         const uint64_t ns = min<uint64_t>(a.split, a.soff[p + 1] - s0);
+        const uint32_t sh = min(31u, a.seg_bits ? a.seg_bits[seg] : (uint32_t)a.hash_shift);
         a.task_info[2 * t] = make_uint4((uint32_t)r0, (uint32_t)(r0 >> 32), (uint32_t)nr, seg);
-        a.task_info[2 * t + 1] = make_uint4((uint32_t)s0, (uint32_t)(s0 >> 32), (uint32_t)ns, 0u);
+        a.task_info[2 * t + 1] = make_uint4((uint32_t)s0, (uint32_t)(s0 >> 32), (uint32_t)ns, sh);
     }
 }
 
@@ -160,10 +163,18 @@ __device__ __forceinline__ uint32_t opaque_tid() {
     return t;
 }
 
-// Bucket of a key among 2^lg (0 <= lg <= 31): the top lg bits of a
-// multiplicative hash (high word of hash * 2^lg, which is 0 for lg = 0).
-__device__ __forceinline__ uint32_t bucket_of(uint32_t key, int lg) {
-    return __umulhi(key * 0x9E3779B1u, 1u << lg);
+// Bucket of a key among 2^lg (hp = lg | sh << 8, lg <= 12, sh <= 31): every
+// key of a partition has the same sh low bits (the partition bits), so the
+// bucket is taken from x = key >> sh, as in bucket_chaining_join
+// (HASH_BIT_MODULO, parallel_radix_join.cpp:68,309): the low lg bits of x,
+// xor the top lg bits of a multiplicative hash of the bits above them.  Keys
+// that differ only in those low bits never share a bucket (a dense key range
+// fills every bucket evenly), and keys that agree in them are spread by the
+// hash of the rest.
+__device__ __forceinline__ uint32_t bucket_of(uint32_t key, uint32_t hp) {
+    const uint32_t lg = hp & 31u, x = key >> (hp >> 8);
+    const uint32_t hi = __umulhi((x >> lg) * 0x9E3779B1u, 1u << lg);
+    return (x ^ hi) & ((1u << lg) - 1u);
 }
 
 // Exclusive scan of the nb (a power of two, at most 8*JB) bucket counters
@@ -203,14 +214,16 @@ __device__ __forceinline__ void bucket_scan(const uint32_t *cnt, uint16_t *bst, 
 // h: bucket h spans [h ? bst[h-1] : 0, bst[h]).  No per-tuple rank is kept in
 // registers.  Branch-free: lanes without a tuple count into the spare counter
 // nb+1, advance the spare cursor nb+1 and store into the spare slot tab[CAP].
-// Returns log2(nb).
+// `sh` is the partition's radix bits.  Returns the hash parameters
+// hp = log2(nb) | sh << 8 for bucket_of.
 template <int JB, int CAP>
-__device__ __forceinline__ int build_regs(const uint64_t (&rv)[CAP / JB], uint32_t m,
-                                          uint64_t *tab, uint16_t *bst, uint32_t *scan_tmp,
-                                          uint32_t tid) {
+__device__ __forceinline__ uint32_t build_regs(const uint64_t (&rv)[CAP / JB], uint32_t m,
+                                               uint32_t sh, uint64_t *tab, uint16_t *bst,
+                                               uint32_t *scan_tmp, uint32_t tid) {
     constexpr int ITEMS = CAP / JB;
     static_assert(max_buckets(CAP) <= 8u * JB, "bucket scan takes at most 8 counters per lane");
     const int lg = bucket_lg(m);
+    const uint32_t hp = (uint32_t)lg | (sh << 8);
     const uint32_t nb = 1u << lg;
     uint32_t *cnt = reinterpret_cast<uint32_t *>(tab);
     for (uint32_t i = threadIdx.x; i < nb + 2; i += JB) cnt[i] = 0;
@@ -222,7 +235,7 @@ __device__ __forceinline__ int build_regs(const uint64_t (&rv)[CAP / JB], uint32
     const uint32_t wave0 = __builtin_amdgcn_readfirstlane(tid);
     unroll<ITEMS>([&](auto q) {
         if (wave0 + q * JB < m) {
-            const uint32_t h = bucket_of(key_of(rv[q]), lg);
+            const uint32_t h = bucket_of(key_of(rv[q]), hp);
             atomicAdd(&cnt[tid + q * JB < m ? h : nb + 1], 1u);
         }
     });
@@ -232,15 +245,15 @@ __device__ __forceinline__ int build_regs(const uint64_t (&rv)[CAP / JB], uint32
     unroll<ITEMS>([&](auto q) {
         if (wave0 + q * JB < m) {
             const bool ok = tid + q * JB < m;
-            const uint32_t hb = bucket_of(key_of(rv[q]), lg);
+            const uint32_t hb = bucket_of(key_of(rv[q]), hp);
             const uint32_t h = ok ? hb : nb + 1;
-            const uint32_t sh = (h & 1u) * 16;
-            const uint32_t old = atomicAdd(&cur[h >> 1], 1u << sh);
-            tab[ok ? (old >> sh) & 0xffffu : (uint32_t)CAP] = rv[q];
+            const uint32_t half = (h & 1u) * 16;
+            const uint32_t old = atomicAdd(&cur[h >> 1], 1u << half);
+            tab[ok ? (old >> half) & 0xffffu : (uint32_t)CAP] = rv[q];
         }
     });
     __syncthreads();
-    return lg;
+    return hp;
 }
 
 // Probe S tuples held in registers against the round's table (bucket ends in
@@ -251,7 +264,7 @@ __device__ __forceinline__ int build_regs(const uint64_t (&rv)[CAP / JB], uint32
 // and then compares, so a step costs one LDS round trip.
 template <int JB, int CAP, bool WRITE, int Q0, int N, int G0 = 0>
 __device__ __forceinline__ void probe_part(const uint64_t (&sv)[N], uint32_t ns,
-                                           const uint64_t *tab, const uint16_t *bst, int lg,
+                                           const uint64_t *tab, const uint16_t *bst, uint32_t hp,
                                            uint64_t &cnt, uint64_t &sum, uint64_t *pairs,
                                            uint64_t &wpos, uint64_t cap, uint32_t tid,
                                            int records = 0, uint32_t tnow = 0) {
@@ -261,7 +274,7 @@ __device__ __forceinline__ void probe_part(const uint64_t (&sv)[N], uint32_t ns,
         if (__builtin_amdgcn_readfirstlane(tid) + (uint32_t)(Q0 + G0) * JB >= ns) return;
         uint32_t be[G];  // bucket cursor | end << 16
         unroll<G>([&](auto j) {
-            const uint32_t h = bucket_of(key_of(sv[G0 + j]), lg);
+            const uint32_t h = bucket_of(key_of(sv[G0 + j]), hp);
             const uint32_t lo = bst[h ? h - 1 : 0], hi = bst[h];
             be[j] = tid + (uint32_t)(Q0 + G0 + j) * JB < ns ? (h ? lo : 0u) | (hi << 16) : 0u;
         });
@@ -290,7 +303,7 @@ __device__ __forceinline__ void probe_part(const uint64_t (&sv)[N], uint32_t ns,
             });
             if (!more) break;
         }
-        probe_part<JB, CAP, WRITE, Q0, N, G0 + PGROUP>(sv, ns, tab, bst, lg, cnt, sum, pairs,
+        probe_part<JB, CAP, WRITE, Q0, N, G0 + PGROUP>(sv, ns, tab, bst, hp, cnt, sum, pairs,
                                                        wpos, cap, tid, records, tnow);
     }
 }
@@ -304,7 +317,7 @@ __device__ __forceinline__ void probe_part(const uint64_t (&sv)[N], uint32_t ns,
 // sum of R payloads (mod 2^32, the high word) and m[j] * S payload.
 template <int JB, int CAP, int Q0, int N, int G0 = 0>
 __device__ __forceinline__ void probe_count(const uint64_t (&sv)[N], uint32_t ns,
-                                            const uint64_t *tab, const uint16_t *bst, int lg,
+                                            const uint64_t *tab, const uint16_t *bst, uint32_t hp,
                                             uint64_t &cnt, uint64_t &sum, uint32_t tid) {
     if constexpr (G0 < N) {
         constexpr int G = N - G0 < PGROUP ? N - G0 : PGROUP;
@@ -321,7 +334,7 @@ __device__ __forceinline__ void probe_count(const uint64_t (&sv)[N], uint32_t ns
             uint32_t k = key_of(sv[G0 + j]);
             asm volatile("" : "+v"(k));  // a 32-bit value to the compiler: 32-bit key tests
             sk[j] = k;
-            const uint32_t h = bucket_of(k, lg);
+            const uint32_t h = bucket_of(k, hp);
             const uint32_t a = bst[h ? h - 1 : 0], e = bst[h];
             const uint32_t b = h ? a : 0u;
             lo[j] = tab2 + b;
@@ -349,7 +362,7 @@ __device__ __forceinline__ void probe_count(const uint64_t (&sv)[N], uint32_t ns
             sum += (uint64_t)m[j] * payload_of(sv[G0 + j]);
         });
         sum += (uint64_t)rsum << 32;
-        probe_count<JB, CAP, Q0, N, G0 + PGROUP>(sv, ns, tab, bst, lg, cnt, sum, tid);
+        probe_count<JB, CAP, Q0, N, G0 + PGROUP>(sv, ns, tab, bst, hp, cnt, sum, tid);
     }
 }
 
@@ -366,10 +379,11 @@ __device__ __forceinline__ void load_part(uint64_t (&v)[N], const uint64_t *p, u
 }
 
 // A task record is two uint4: {r0 lo, r0 hi, |R_p|, segment}, {s0 lo, s0 hi,
-// |S slice|, 0}; everything a worker needs in one independent load.
+// |S slice|, radix bits of the partition}; everything a worker needs in one
+// independent load.
 struct TaskRec {
     uint64_t r0, s0;
-    uint32_t nr, ns, seg;
+    uint32_t nr, ns, seg, sh;
 };
 __device__ __forceinline__ TaskRec decode_task(uint4 a, uint4 b) {
     TaskRec r;
@@ -378,6 +392,7 @@ __device__ __forceinline__ TaskRec decode_task(uint4 a, uint4 b) {
     r.seg = a.w;
     r.s0 = ((uint64_t)b.y << 32) | b.x;
     r.ns = b.z;
+    r.sh = b.w;
     return r;
 }
 
@@ -418,11 +433,11 @@ __global__ __launch_bounds__(JB, 4) void k_join(JoinArgs a) {
             // one LDS round of R: CAP tuples (more rounds only for partitions
             // larger than CAP)
             const uint32_t m = (uint32_t)min<uint64_t>(CAP, cur.r0 + cur.nr - rb);
-            int lg;
+            uint32_t hp;
             {
                 uint64_t rv[ITEMS];
                 load_part<JB, 0, ITEMS>(rv, a.R + rb, m, opaque_tid());
-                lg = build_regs<JB, CAP>(rv, m, tab, bst, scan_tmp, opaque_tid());
+                hp = build_regs<JB, CAP>(rv, m, cur.sh, tab, bst, scan_tmp, opaque_tid());
             }
             // the S slice in sub-slices of CAP (one, unless a heavy partition
             // gives the task more): the first half of the first sub-slice was
@@ -436,8 +451,8 @@ __global__ __launch_bounds__(JB, 4) void k_join(JoinArgs a) {
                 uint64_t sb[SB];
                 load_part<JB, SA, SB>(sb, a.S + cur.s0 + so, sub, opaque_tid());
                 uint64_t c = 0, x = 0, wpos = 0;
-                probe_count<JB, CAP, 0, SA>(sa, sub, tab, bst, lg, c, x, opaque_tid());
-                probe_count<JB, CAP, SA, SB>(sb, sub, tab, bst, lg, c, x, opaque_tid());
+                probe_count<JB, CAP, 0, SA>(sa, sub, tab, bst, hp, c, x, opaque_tid());
+                probe_count<JB, CAP, SA, SB>(sb, sub, tab, bst, hp, c, x, opaque_tid());
                 if (PAIRS) {
                     uint32_t total_c;
                     const uint32_t pre = block_exclusive_scan<JB>((uint32_t)c, scan_tmp, &total_c);
@@ -451,9 +466,9 @@ __global__ __launch_bounds__(JB, 4) void k_join(JoinArgs a) {
                     // slice's matches are produced (low 32 bits)
                     const uint32_t tnow =
                         a.records == 2 ? (uint32_t)__builtin_amdgcn_s_memrealtime() : 0u;
-                    probe_part<JB, CAP, true, 0, SA>(sa, sub, tab, bst, lg, dc, dx, a.pairs, wpos,
+                    probe_part<JB, CAP, true, 0, SA>(sa, sub, tab, bst, hp, dc, dx, a.pairs, wpos,
                                                     a.pairs_cap, opaque_tid(), a.records, tnow);
-                    probe_part<JB, CAP, true, SA, SB>(sb, sub, tab, bst, lg, dc, dx, a.pairs,
+                    probe_part<JB, CAP, true, SA, SB>(sb, sub, tab, bst, hp, dc, dx, a.pairs,
                                                      wpos, a.pairs_cap, opaque_tid(), a.records,
                                                      tnow);
                 }
@@ -550,6 +565,8 @@ int run_join(const JoinDesc &d, hipStream_t st, const Marks *mk, int m0) {
     a.pairs = d.pairs;
     a.pairs_cap = d.pairs_cap;
     a.records = d.records;
+    a.hash_shift = d.hash_shift;
+    a.seg_bits = d.seg_bits;
     a.pair_cursor = d.pair_cursor;
     a.task_cnt = d.task_cnt;
     a.task_base = d.task_base;

@@ -201,6 +201,7 @@ int enqueue_ranges(BatchPlan *p, const aj_stream_tuple_t *dR, uint64_t nR,
     j.part_seg_base = w.seg_part_base;
     j.nseg = nwin;
     j.hash_shift = 0;
+    j.seg_bits = W_<uint32_t>(p, L.bits);  // one pass of bits[w] radix bits per window
     j.cap = p->cap;
     j.split = p->split;
     j.seg_count = cnt;
