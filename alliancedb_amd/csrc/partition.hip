@@ -322,14 +322,27 @@ __global__ __launch_bounds__(PBLOCK) void k_scatter(PassArgs a) {
 //             consecutive addresses within a digit's run; the next tile's loads
 //             are issued before this step, so they are in flight meanwhile
 constexpr int PIPE_MAX_BITS = 10;
+// Spare counters and slots for lanes without a tuple, one per lane mod 32: a
+// single spare would take the atomics and stores of up to 64 lanes of a wave
+// on one LDS address (serialised), which in a partly filled tile costs more
+// than the rest of the tile's LDS work.
+constexpr int NSPARE = 32;
+
+// LDS of the pipelined scatter: PTILE + NSPARE tuples, then F + NSPARE
+// counters, F write bases and F cursors.
+constexpr size_t pipe_lds_bytes(int pblock, uint32_t F) {
+    return sizeof(uint64_t) * ((size_t)pblock * PITEMS + NSPARE) +
+           sizeof(uint32_t) * (3 * (size_t)F + NSPARE);
+}
 
 // Tile loop of the pipelined scatter over input tuples [begin, end): digit
-// (key >> shift) & (F-1); `cur` holds the F output cursors and `cnt` (F + 1
-// counters, the last one a spare) is zero (both visible to the whole
-// work-group) on entry; `buf` holds PTILE + 1 tuples, the last one a spare.
+// (key >> shift) & (F-1); `cur` holds the F output cursors and `cnt` (F
+// counters, then NSPARE spares) is zero (both visible to the whole
+// work-group) on entry; `buf` holds PTILE tuples, then NSPARE spare slots.
 // Every phase is branch-free over a lane's items: a tuple outside the range
-// counts on the spare counter and is placed in the spare slot, so the LDS
-// reads of a phase are issued in batches (WB items) and waited for once.
+// counts on its lane's spare counter and is placed in its lane's spare slot,
+// so the LDS reads of a phase are issued in batches (WB items) and waited for
+// once.
 template <int STRIDE, int PBLOCK>
 __device__ __forceinline__ void scatter_tiles(const void *in, uint64_t begin, uint64_t end,
                                               uint64_t *out, int shift, uint32_t F,
@@ -350,7 +363,8 @@ __device__ __forceinline__ void scatter_tiles(const void *in, uint64_t begin, ui
 #pragma unroll
         for (int j = 0; j < PITEMS; ++j) {
             const uint32_t d = (key_of(t[j]) >> shift) & mask;
-            const uint32_t r = atomicAdd(&cnt[v[j] ? d : F], 1u) & 0xffffu;
+            const uint32_t r =
+                atomicAdd(&cnt[v[j] ? d : F + (threadIdx.x & (NSPARE - 1))], 1u) & 0xffffu;
             if (j & 1)
                 rank2[j / 2] |= r << 16;
             else
@@ -391,7 +405,7 @@ __device__ __forceinline__ void scatter_tiles(const void *in, uint64_t begin, ui
 #pragma unroll
             for (int j = 0; j < WB; ++j)
                 buf[v[j0 + j] ? st[j] + ((rank2[(j0 + j) / 2] >> (16 * ((j0 + j) & 1))) & 0xffffu)
-                              : (uint32_t)PTILE] = t[j0 + j];
+                              : (uint32_t)PTILE + (threadIdx.x & (NSPARE - 1))] = t[j0 + j];
         }
         __syncthreads();
         const uint32_t len = *s_len;
@@ -430,9 +444,9 @@ __global__ __launch_bounds__(PBLOCK, 4) void k_scatter_pipe(PassArgs a) {
     ChunkLoc L;
     if (!locate_chunk(a, blockIdx.x, L)) return;
     const uint32_t F = 1u << a.bits[L.seg];
-    uint64_t *buf = smem;                                            // PTILE + 1 tuples
-    uint32_t *cnt = reinterpret_cast<uint32_t *>(buf + PTILE + 1);  // F + 1: counts, tile starts
-    uint32_t *dst = cnt + F + 1;                                     // F: cursor - tile start
+    uint64_t *buf = smem;                                    // PTILE + NSPARE tuples
+    uint32_t *cnt = reinterpret_cast<uint32_t *>(buf + PTILE + NSPARE);  // F + NSPARE counters
+    uint32_t *dst = cnt + F + NSPARE;                        // F: cursor - tile start
     uint32_t *cur = dst + F;                                         // F: global write cursors
     // S offsets follow every R tuple that lies in a segment
     const uint32_t rel_base = L.rel ? (uint32_t)a.obase0[a.nseg] : 0u;
@@ -441,7 +455,7 @@ __global__ __launch_bounds__(PBLOCK, 4) void k_scatter_pipe(PassArgs a) {
         cur[d] = a.hist_scan[hb + d * L.nchunk + L.c] - rel_base;
         cnt[d] = 0;
     }
-    if (threadIdx.x == 0) cnt[F] = 0;
+    if (threadIdx.x < NSPARE) cnt[F + threadIdx.x] = 0;
     __syncthreads();
     scatter_tiles<STRIDE, PBLOCK>(L.rel ? a.in1 : a.in0, L.begin, L.end, L.rel ? a.out1 : a.out0,
                                   a.shift, F, buf, cnt, dst, cur, &s_len);
@@ -675,15 +689,15 @@ __global__ __launch_bounds__(PBLOCK, 4) void k_scatter_p2(P2Args p) {
     const uint64_t s0 = p.span[2 * (uint64_t)u], end = p.span[2 * (uint64_t)u + 1];
     const int rel = (int)(s0 >> 63);
     const uint64_t begin = s0 & ~(1ull << 63);
-    uint64_t *buf = smem;                                            // PTILE + 1 tuples
-    uint32_t *cnt = reinterpret_cast<uint32_t *>(buf + PTILE + 1);  // F2 + 1 counters
-    uint32_t *dst = cnt + F2 + 1;
+    uint64_t *buf = smem;                                    // PTILE + NSPARE tuples
+    uint32_t *cnt = reinterpret_cast<uint32_t *>(buf + PTILE + NSPARE);  // F2 + NSPARE counters
+    uint32_t *dst = cnt + F2 + NSPARE;
     uint32_t *cur = dst + F2;
     for (uint32_t d = threadIdx.x; d < F2; d += PBLOCK) {
         cur[d] = p.G[((uint64_t)u << p.b2) + d];
         cnt[d] = 0;
     }
-    if (threadIdx.x == 0) cnt[F2] = 0;
+    if (threadIdx.x < NSPARE) cnt[F2 + threadIdx.x] = 0;
     __syncthreads();
     scatter_tiles<8, PBLOCK>(rel ? (const void *)p.in1 : (const void *)p.in0, begin, end,
                              rel ? p.out1 : p.out0, p.b1, F2, buf, cnt, dst, cur, &s_len);
@@ -751,8 +765,7 @@ static int launch_pass_kernels(const PassDesc &d, const PassWork &w, const PassA
         sizeof(uint64_t) * PBLOCK * PITEMS + sizeof(uint32_t) * (2 * F + 1);
     AJ_TRY(lds_limit_set((const void *)k_hist<STRIDE, PBLOCK>, lds_hist));
     AJ_TRY(lds_limit_set((const void *)k_scatter<STRIDE, PBLOCK>, lds_scatter));
-    const size_t lds_pipe =
-        sizeof(uint64_t) * (PBLOCK * PITEMS + 1) + sizeof(uint32_t) * (3 * F + 1);
+    const size_t lds_pipe = pipe_lds_bytes(PBLOCK, F);
     if (d.max_bits <= PIPE_MAX_BITS)
         AJ_TRY(lds_limit_set((const void *)k_scatter_pipe<STRIDE, PBLOCK>, lds_pipe));
     if (mk) mk->mark(m0);
@@ -787,8 +800,7 @@ static int launch_pass_kernels(const PassDesc &d, const PassWork &w, const PassA
             // writes to every pass-1 partition at once; longer runs write
             // faster)
             constexpr int BIG = 1024;
-            const size_t lds_big =
-                sizeof(uint64_t) * (BIG * PITEMS + 1) + sizeof(uint32_t) * (3 * F + 1);
+            const size_t lds_big = pipe_lds_bytes(BIG, F);
             AJ_TRY(lds_limit_set((const void *)k_scatter_pipe<8, BIG>, lds_big));
             hipLaunchKernelGGL((k_scatter_pipe<8, BIG>), dim3((unsigned)d.max_chunks), dim3(BIG),
                                lds_big, st, a);
@@ -898,8 +910,7 @@ int run_partition_pass(const PassDesc &d, const PassWork &w, hipStream_t st, con
 
 template <int PBLOCK>
 static int launch_scatter_p2(const P2Args &p, uint64_t max_units, hipStream_t st) {
-    const size_t lds =
-        sizeof(uint64_t) * (PBLOCK * PITEMS + 1) + sizeof(uint32_t) * (3 * (1u << p.b2) + 1);
+    const size_t lds = pipe_lds_bytes(PBLOCK, 1u << p.b2);
     AJ_TRY(lds_limit_set((const void *)k_scatter_p2<PBLOCK>, lds));
     hipLaunchKernelGGL((k_scatter_p2<PBLOCK>), dim3((unsigned)max_units), dim3(PBLOCK), lds, st, p);
     return AJ_OK;
@@ -944,8 +955,7 @@ int run_pass2_2d(const P2Desc &d, hipStream_t st, const Marks *mk, int m0) {
         mk->mark(m0 + 1);
         mk->mark(m0 + 2);
     }
-    // 1024 lanes (one work-group per CU at 128 VGPRs) beat two of 512 lanes:
-    // 1147 vs 1196 us at 2^27 x 2^27 with ~24 units per CU
+    // 1024 lanes: one work-group per CU at 128 VGPRs, 16384-tuple tiles
     AJ_TRY(d.block == 512 ? launch_scatter_p2<1024>(p, d.max_units, st)
                           : launch_scatter_p2<256>(p, d.max_units, st));
     if (mk) mk->mark(m0 + 3);
