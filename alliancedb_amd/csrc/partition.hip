@@ -794,15 +794,15 @@ static int launch_pass_kernels(const PassDesc &d, const PassWork &w, const PassA
     }
     if (mk) mk->mark(m0 + 2);
     if (d.max_chunks) {
-        if (STRIDE == 8 && PBLOCK == 512 && d.hist2d && d.max_bits <= PIPE_MAX_BITS) {
-            // pass 1 of a two-pass join: 1024-lane work-groups with tiles of
-            // 16384 tuples, whose runs per digit are twice as long (the pass
-            // writes to every pass-1 partition at once; longer runs write
-            // faster)
+        if (PBLOCK == 512 && (d.hist2d || STRIDE == 12) && d.max_bits <= PIPE_MAX_BITS) {
+            // pass 1 of a two-pass join, and window batches (12-byte tuples):
+            // 1024-lane work-groups with tiles of 16384 tuples, whose runs per
+            // digit are twice as long (the pass writes to every partition at
+            // once; longer runs write faster)
             constexpr int BIG = 1024;
             const size_t lds_big = pipe_lds_bytes(BIG, F);
-            AJ_TRY(lds_limit_set((const void *)k_scatter_pipe<8, BIG>, lds_big));
-            hipLaunchKernelGGL((k_scatter_pipe<8, BIG>), dim3((unsigned)d.max_chunks), dim3(BIG),
+            AJ_TRY(lds_limit_set((const void *)k_scatter_pipe<STRIDE, BIG>, lds_big));
+            hipLaunchKernelGGL((k_scatter_pipe<STRIDE, BIG>), dim3((unsigned)d.max_chunks), dim3(BIG),
                                lds_big, st, a);
         } else if (d.max_bits <= PIPE_MAX_BITS)
             hipLaunchKernelGGL((k_scatter_pipe<STRIDE, PBLOCK>), dim3((unsigned)d.max_chunks),

@@ -34,7 +34,7 @@
 
 namespace aj {
 
-constexpr uint32_t WTILE = 8192;
+constexpr uint32_t WTILE = 16384;  // the window scatter's tile: chunks are whole tiles
 
 __global__ void k_window_bounds(const uint32_t *R, uint64_t nR, const uint32_t *S, uint64_t nS,
                                 const uint32_t *bounds, int nwin, uint64_t *rb, uint64_t *re,
