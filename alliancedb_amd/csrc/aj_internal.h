@@ -191,6 +191,8 @@ struct JoinDesc {
     size_t scan_tmp_bytes;
     int num_cu;
 };
+// Raises a kernel's dynamic-LDS limit on the current device (thread-safe).
+int lds_limit_set(const void *fn, size_t bytes);
 size_t join_scan_tmp_bytes(uint64_t max_parts);
 // S tuples per join task (0 = 8 LDS rounds of `cap`).  A task probes its
 // slice in register-resident sub-slices of `cap` tuples.

@@ -525,16 +525,22 @@ uint64_t join_max_tasks(uint64_t max_parts, uint64_t max_S, uint32_t split) {
 template <int JB, int CAP, bool PAIRS>
 static int launch_join(const JoinArgs &a, int num_cu, hipStream_t st) {
     const size_t lds = join_lds_bytes(CAP);
-    static bool attr = false;
-    if (!attr) {
-        AJ_HIP(hipFuncSetAttribute((const void *)k_join<JB, CAP, PAIRS>,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        attr = true;
+    AJ_TRY(lds_limit_set((const void *)k_join<JB, CAP, PAIRS>, lds));
+    // resident work-groups per CU, per device (LDS and registers decide it)
+    static std::mutex mu;
+    static int per_cu_dev[64] = {};
+    int dev = 0;
+    AJ_HIP(hipGetDevice(&dev));
+    int per_cu;
+    {
+        std::lock_guard<std::mutex> g(mu);
+        per_cu = dev >= 0 && dev < 64 ? per_cu_dev[dev] : 0;
     }
-    static int per_cu = 0;
     if (!per_cu) {
         AJ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_join<JB, CAP, PAIRS>, JB, lds));
         if (per_cu < 1) per_cu = 1;
+        std::lock_guard<std::mutex> g(mu);
+        if (dev >= 0 && dev < 64) per_cu_dev[dev] = per_cu;
     }
     hipLaunchKernelGGL((k_join<JB, CAP, PAIRS>), dim3(num_cu * per_cu), dim3(JB), lds, st, a);
     return AJ_OK;

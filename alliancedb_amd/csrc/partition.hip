@@ -740,19 +740,28 @@ size_t pass_scan_tmp_bytes(uint64_t max_hist) {
     return bytes;
 }
 
-// Raises a kernel's dynamic-LDS limit once to the largest size requested so far
-// (callers hold the context lock).
-static int lds_limit_set(const void *fn, size_t bytes) {
-    static const void *fns[24] = {};
-    static size_t set[24] = {};
-    int i = 0;
-    while (i < 24 && fns[i] && fns[i] != fn) ++i;
-    if (i == 24) i = 23;
-    if (!fns[i]) fns[i] = fn;
-    if (bytes > set[i]) {
-        AJ_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
-        set[i] = bytes;
-    }
+// Raises a kernel's dynamic-LDS limit on the current device to the largest size
+// requested so far there; contexts on several devices or threads share the record.
+int lds_limit_set(const void *fn, size_t bytes) {
+    struct Rec {
+        int dev;
+        const void *fn;
+        size_t bytes;
+    };
+    static std::mutex mu;
+    static std::vector<Rec> recs;
+    int dev = 0;
+    AJ_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> g(mu);
+    Rec *r = nullptr;
+    for (Rec &x : recs)
+        if (x.dev == dev && x.fn == fn) r = &x;
+    if (r && bytes <= r->bytes) return AJ_OK;
+    AJ_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    if (r)
+        r->bytes = bytes;
+    else
+        recs.push_back({dev, fn, bytes});
     return AJ_OK;
 }
 
