@@ -1,4 +1,5 @@
-"""GPU: the headline workload at full size, |R| = |S| = 2^27 8-byte tuples.
+"""GPU: the headline workload at full size, |R| = |S| = 2^27 8-byte tuples, and the
+skewed foreign-key workload at full size (|R| = 2^24, |S| = 2^28 Zipf).
 
 R keys are a seeded permutation of 1..2^27 (payload = row id) and S keys are
 uniform over 1..2^27 (payload = row id), so every S tuple matches exactly the R
@@ -54,3 +55,23 @@ def test_full_size_pair_list(aj, full, algo):
     hi = (got.pairs >> np.uint64(32)).astype(np.uint32)
     assert (np.bincount(lo, minlength=N) == 1).all()  # every S payload exactly once
     assert np.array_equal(hi, want_hi[lo])
+
+
+@pytest.mark.parametrize("theta", [0.0, 0.5, 1.0, 1.5])
+def test_full_size_skewed_foreign_key(aj, theta):
+    """|R| = 2^24 unique keys, |S| = 2^28 keys Zipf(theta) over R's keys: every S tuple
+    matches exactly the R tuple holding its key, so count = |S| and the checksum is
+    sum over S of (R payload of s.key) << 32 | s.payload (mod 2^64)."""
+    nR, nS = 1 << 24, 1 << 28
+    print(f"case full size skewed foreign key theta={theta}", flush=True)
+    R = aj.gen_unique(nR, SEED_R)
+    S = aj.gen_zipf(nS, nR, theta, SEED_S)
+    inv = np.zeros(nR + 1, np.uint64)
+    inv[R["key"]] = R["payloadID"].astype(np.uint64)
+    hi_sum = int(inv[S["key"]].sum(dtype=np.uint64))
+    del inv
+    lo_sum = int(S["payloadID"].astype(np.uint64).sum(dtype=np.uint64))
+    want = (nS, ((hi_sum << 32) + lo_sum) % (1 << 64))
+    for algo in ("PRO", "NPO"):
+        got = getattr(aj, algo)(R, S, 8)
+        assert (got.count, got.checksum) == want, algo
