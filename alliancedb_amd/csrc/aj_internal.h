@@ -161,7 +161,7 @@ struct P2Desc {
     uint64_t max_units;          // grid bound: (nR + nS) / unit + 2 * 2^b1
     int block;
 };
-constexpr int P2D_MAX_BITS = 14;  // b1 + b2 for the 2-D histogram path (64 KB of LDS)
+constexpr int P2D_MAX_BITS = 15;  // b1 + b2 for the 2-D histogram path (128 KB of LDS)
 int run_pass2_2d(const P2Desc &d, hipStream_t st, const Marks *marks = nullptr, int m0 = 0);
 
 // Join of partition pairs (R part_off, S part_off) into per-segment results.

@@ -462,7 +462,7 @@ __global__ __launch_bounds__(PBLOCK, 4) void k_scatter_pipe(PassArgs a) {
 }
 
 // ---------------------------------------------------------------- pass 2 from a 2-D histogram
-// For the one-segment PRO with two passes of b1 and b2 bits (F1*F2 <= 2^14):
+// For the one-segment PRO with two passes of b1 and b2 bits (F1*F2 <= 2^15):
 //   k_hist2d     pass-1 histogram kernel that also counts (d1, d2) pairs of
 //                its chunk in LDS and stores them in H, laid out
 //                [relation][d1][chunk][d2]; the 1-D pass-1 histogram is the

@@ -166,8 +166,9 @@ static ProPlan plan_pro(uint64_t nR, uint64_t nS, const ProConfig &c, int num_cu
     p.o_scal = L.add<uint64_t>(8);
     p.p2d = p.b2 > 0 && p.b1 + p.b2 <= P2D_MAX_BITS && c.p2d;
     if (p.p2d) {
-        // pass-2 units of about 40 per CU (at least one tile, at most a pass-1 chunk)
-        p.unit = std::max<uint64_t>(PART_TILE, std::min<uint64_t>(p.chunk1, (nR + nS) / ((uint64_t)num_cu * 40)));
+        // pass-2 units of about 40 per CU (at least one 16384-tuple tile of the
+        // pass-2 scatter, at most a pass-1 chunk)
+        p.unit = std::max<uint64_t>(2 * PART_TILE, std::min<uint64_t>(p.chunk1, (nR + nS) / ((uint64_t)num_cu * 40)));
         p.max_units = (nR + nS) / p.unit + 2 * F1 + 1;
         p.o_tot = L.add<uint32_t>(p.max_units * F2);
         p.o_ub = L.add<uint32_t>(2 * F1 + 2);
