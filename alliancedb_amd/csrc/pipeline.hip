@@ -88,7 +88,9 @@ ProConfig choose_pro_config(uint64_t nR, uint64_t nS, const aj_join_opts_t *o) {
         c.bits1 = bits;
         c.bits2 = 0;
     } else if (passes == 2) {
-        c.bits1 = (bits + 1) / 2;
+        // an odd bit goes to pass 2: pass 1 writes into every pass-1 partition
+        // at once, pass 2 only into one pass-1 partition's range
+        c.bits1 = bits / 2;
         c.bits2 = bits - c.bits1;
     }
     (void)nS;

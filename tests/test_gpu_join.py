@@ -60,8 +60,8 @@ def test_empty_relations(aj, algo):
         assert (got.count, got.checksum) == (0, 0)
 
 
-@pytest.mark.parametrize("bits,passes", [(0, 1), (4, 1), (8, 1), (13, 1), (8, 2), (14, 2),
-                                         (18, 2)])
+@pytest.mark.parametrize("bits,passes", [(0, 1), (4, 1), (8, 1), (13, 1), (8, 2), (13, 2),
+                                         (14, 2), (15, 2), (16, 2), (18, 2)])
 @pytest.mark.parametrize("cap", [4096, 8192, 12288])
 def test_pro_geometry_invariance(aj, oracle, bits, passes, cap):
     """Results do not depend on radix bits, pass count or LDS round size."""
