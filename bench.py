@@ -123,6 +123,30 @@ def cpu_baseline(aj, log2n: int):
                       f"2 passes (reference defaults), median of 3 (temporaries included)"}
 
 
+def cpu_reference_pro(aj, log2n: int):
+    """The reference's own single-thread radix join (RJ_st, compiled from its sources
+    into oracle/_ref: the same radix_cluster / bucket_chaining_join code as its
+    multithreaded PRO, which cannot run off its authors' machine) on a bounded sample
+    of the same workload; reported beside the multithreaded port, not as the target."""
+    import oracle
+    if not oracle.ref_available():
+        return None
+    n = 1 << log2n
+    R = aj.gen_unique(n, 12345)
+    S = aj.gen_uniform(n, 1, n, 54321)
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        assert oracle.ref_join("RJ_st", R, S)[0] == n
+        times.append(time.perf_counter() - t0)
+    sec = sorted(times)[1]
+    return {"value": round(2 * n / sec / 1e6, 2), "unit": "M tuples/s", "cores": 1,
+            "kind": "reference",
+            "sample": f"reference RJ_st (parallel_radix_join.cpp:1719-1844, one thread) "
+                      f"|R|=|S|=2^{log2n}, same key distribution, median of 3 "
+                      f"(input copies included)"}
+
+
 def measured_traffic(kernel: str, workload: str):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 traffic summary of
     this workload, or None when none was measured for it."""
@@ -258,6 +282,10 @@ def run_pro(aj, args):
     if not args.no_cpu:
         line["cpu_baseline"] = (cpu_baseline_shj(aj, args.cpu_log2n) if algo == aj.AJ_ALGO_SHJ
                                 else cpu_baseline(aj, args.cpu_log2n))
+        if algo == aj.AJ_ALGO_PRO:
+            ref = cpu_reference_pro(aj, min(args.cpu_log2n, 22))
+            if ref:
+                line["cpu_reference"] = ref
     return line
 
 
