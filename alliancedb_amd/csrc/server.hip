@@ -377,6 +377,10 @@ static int server_init(Server *s) {
     memset(&bo, 0, sizeof(bo));
     bo.cap = s->o.cap;
     bo.no_graph = 1;
+    // 64K-tuple chunks: a served window (median 256K per side) spreads over
+    // several work-groups, so a batch of a few windows still fills the GPU (p99
+    // at half load 2.9 -> 1.6 ms against 256K chunks, same saturated rate)
+    bo.chunk = 65536;
     const uint64_t W = (uint64_t)s->o.max_batch_windows;
     for (int i = 0; i < s->o.slots; ++i) {
         std::unique_ptr<SSlot> sl(new SSlot());
