@@ -350,7 +350,8 @@ __device__ __forceinline__ void scatter_tiles(const void *in, uint64_t begin, ui
                                               uint32_t *cur, uint32_t *s_len) {
     constexpr int PTILE = PBLOCK * PITEMS;
     constexpr int MAXPER = (1 << PIPE_MAX_BITS) / 64;
-    constexpr int WB = 4;  // items per batch of LDS reads in the place / write phases
+    constexpr int WB = 8;  // items per batch of LDS reads in the place / write phases
+    static_assert(PITEMS % WB == 0, "a tile's items split into whole batches");
     const uint32_t mask = F - 1;
     uint64_t base = tile_start(begin, STRIDE);
     if (base >= end) return;
