@@ -309,3 +309,28 @@ def test_autotuner_picks_a_geometry_with_identical_results(aj, oracle):
             assert (out.count, out.checksum) == want
     finally:
         aj.tune_clear()
+
+
+@pytest.mark.parametrize("bits,passes", [(8, 1), (14, 2), (15, 2), (18, 2)])
+@pytest.mark.parametrize("block,chunk", [(256, 0), (512, 8192), (512, 65536), (256, 16384)])
+def test_pro_block_and_chunk_invariance(aj, oracle, bits, passes, block, chunk):
+    """Results do not depend on the partition work-group size (256 or 512 lanes: the
+    256-lane scatters, 2-D histogram and pass 2) or the pass-1 chunk (8192-tuple chunks
+    are half a 16384-tuple tile, so every tile is partly filled)."""
+    n = 1 << 18
+    R, S = aj.gen_uniform(n, 1, n // 2, 13), aj.gen_uniform(n + 777, 1, n // 2, 14)
+    want = oracle.aggregate(R, S)
+    dR, dS = aj.DeviceArray.from_host(R), aj.DeviceArray.from_host(S)
+    print(f"case bits={bits} passes={passes} block={block} chunk={chunk}", flush=True)
+    aj.set_option("part_block", block)
+    aj.set_option("part_chunk", chunk)
+    try:
+        with aj.Context(0) as ctx:
+            out = ctx.join(dR.ptr, len(R), dS.ptr, len(S), aj.AJ_ALGO_PRO, radix_bits=bits,
+                           passes=passes)
+    finally:
+        aj.set_option("part_block", 0)
+        aj.set_option("part_chunk", 0)
+        dR.free()
+        dS.free()
+    assert (out.count, out.checksum) == want
