@@ -334,3 +334,25 @@ def test_pro_block_and_chunk_invariance(aj, oracle, bits, passes, block, chunk):
         dR.free()
         dS.free()
     assert (out.count, out.checksum) == want
+
+
+@pytest.mark.parametrize("bits,passes", [(14, 2), (15, 2)])
+@pytest.mark.parametrize("option,value", [("part_hist2d", 0), ("kernel_timing", 1), ("tuned", 0)])
+def test_pro_option_invariance(aj, oracle, bits, passes, option, value):
+    """Results do not depend on the remaining switches: the generic pass 2 instead of the
+    2-D-histogram one, per-phase kernel timing, and ignoring the tuned-geometry table."""
+    n = 1 << 18
+    R, S = aj.gen_unique(n, 15), aj.gen_zipf(n, n, 1.0, 16)
+    want = oracle.aggregate(R, S)
+    dR, dS = aj.DeviceArray.from_host(R), aj.DeviceArray.from_host(S)
+    print(f"case bits={bits} passes={passes} {option}={value}", flush=True)
+    old = aj.get_option(option)
+    aj.set_option(option, value)
+    try:
+        with aj.Context(0) as ctx:
+            out = ctx.join(dR.ptr, n, dS.ptr, n, aj.AJ_ALGO_PRO, radix_bits=bits, passes=passes)
+    finally:
+        aj.set_option(option, old)
+        dR.free()
+        dS.free()
+    assert (out.count, out.checksum) == want
