@@ -104,3 +104,25 @@ def test_batch_window_equals_single_join(aj):
                                           t["payload"][lo:hi].view(np.int32)))
             single = aj.PRO(parts[0], parts[1], 1)
             assert (single.count, single.checksum) == (int(got_c[w]), int(got_x[w]))
+
+
+@pytest.mark.parametrize("cap,target,chunk", [(4096, 2048, 16384), (12288, 8192, 65536),
+                                              (8704, 0, 16384), (8192, 4096, 0)])
+def test_batch_geometry_invariance(aj, oracle, cap, target, chunk):
+    """Per-window results do not depend on the plan's LDS round size, its per-partition
+    target (which sets each window's radix bits) or its chunk size (16K-tuple chunks split
+    larger windows over several work-groups and leave partly filled tiles)."""
+    nwin = 48
+    with aj.Context(0) as ctx:
+        sizes, out = _streams(aj, ctx, nwin, 1 << 9, 1 << 17, seed=7)
+        bounds, host, want = _expected(aj, oracle, out, nwin)
+        cnt, chk = aj.DeviceArray(8 * nwin), aj.DeviceArray(8 * nwin)
+        (dR, nR, bR), (dS, nS, _) = out["R"], out["S"]
+        print(f"case cap={cap} target={target} chunk={chunk} nR={nR} nS={nS}", flush=True)
+        with aj.BatchPlan(ctx, nR, nS, nwin, int(sizes.max() * 1.1), cap=cap, target=target,
+                          chunk=chunk, graph=False) as plan:
+            plan.run(dR.ptr, nR, dS.ptr, nS, bR.ptr, nwin, cnt.ptr, chk.ptr)
+            aj.device_sync()
+            got = list(zip(cnt.to_host(np.uint64, nwin).tolist(),
+                           chk.to_host(np.uint64, nwin).tolist()))
+        assert got == [(int(c), int(x)) for c, x in want]
