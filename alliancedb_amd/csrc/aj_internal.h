@@ -82,7 +82,8 @@ bool kernel_timing_enabled();
 // Eager symmetric hash join (shj.hip): steps of `batch` tuples per side.
 uint64_t shj_steps(uint64_t nR, uint64_t nS, uint64_t batch);
 int shj_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, uint64_t nS,
-               uint64_t batch, uint64_t *progress, aj_join_out_t *out, hipStream_t st);
+               uint64_t batch, uint64_t *progress, uint64_t *pairs, uint64_t pcap, int records,
+               aj_join_out_t *out, hipStream_t st);
 // Writes the device's constant-rate clock to *slot (stream-ordered).
 int device_clock_stamp(uint64_t *slot, hipStream_t st);
 

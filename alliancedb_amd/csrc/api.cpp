@@ -212,12 +212,8 @@ static int join_dev_locked(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint
     }
     int rc;
     if (algo == AJ_ALGO_SHJ) {
-        if (pairs) {
-            set_error(AJ_ERR_ARG, "SHJ returns counts only (no pairs or records)");
-            return AJ_ERR_ARG;
-        }
         rc = shj_device(ctx, dR, nR, dS, nS, shj_batch(nR, nS, opts), opts ? opts->progress : nullptr,
-                        out, st);
+                        pairs, pcap, records, out, st);
     } else if (algo == AJ_ALGO_NPO) {
         rc = npo_device(ctx, dR, nR, dS, nS, pairs, pcap, records, out, st);
     } else if (algo == AJ_ALGO_PRO) {
@@ -401,9 +397,9 @@ static result_t *dropin(int algo, relation_t *relR, relation_t *relS, int nthrea
         aj_join_opts_t o;
         memset(&o, 0, sizeof(o));
         o.algo = algo;
-        if (algo == AJ_ALGO_SHJ && (mat || timing)) {
-            rc = set_error(AJ_ERR_ARG, "SHJ drop-ins return counts only (options materialize "
-                                       "and timing must be 0)");
+        if (algo == AJ_ALGO_SHJ && timing) {
+            rc = set_error(AJ_ERR_ARG, "SHJ drop-ins have no timed records (option timing must "
+                                       "be 0)");
             break;
         }
         g_force_timing = timing;  // per-phase device times of the counting run

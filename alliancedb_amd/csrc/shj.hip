@@ -84,6 +84,83 @@ __global__ __launch_bounds__(SBLOCK) void k_shj_step(const uint64_t *in, uint64_
     }
 }
 
+// The materialising form of k_shj_step: the same inserts, probes, per-block
+// counts and checksums, and every match also written to out: mode 1 the pair
+// (R payload << 32 | S payload), mode 2 the probing tuple <key, payloadID> (the
+// record SHJJoiner::join's probe appends, common_functions.cpp:251-257).  A
+// wave counts its lanes' matches, reserves one range with a single atomic on
+// *cursor, then walks its probe chains again to write; positions at or past
+// cap are counted but not written (the caller reports the shortfall).  The
+// loop runs a work-group-uniform number of rounds so the wave operations see
+// every lane.
+__global__ __launch_bounds__(SBLOCK) void k_shj_step_out(
+    const uint64_t *in, uint64_t lo, uint64_t hi, uint64_t *tab, uint64_t mask,
+    uint64_t *own_sent, const uint64_t *other, uint64_t omask, const uint64_t *other_sent,
+    int is_r, uint64_t *sum_part, uint64_t *step_cnt, int mode, uint64_t *out, uint64_t cap,
+    uint64_t *cursor) {
+    __shared__ uint64_t red[SBLOCK / 64];
+    const uint64_t nsent = *other_sent;  // not written during this kernel
+    uint64_t cnt = 0, sum = 0, sent = 0;
+    for (uint64_t base = lo + blockIdx.x * (uint64_t)SBLOCK; base < hi;
+         base += (uint64_t)gridDim.x * SBLOCK) {
+        const uint64_t i = base + threadIdx.x;
+        const bool valid = i < hi;
+        const uint64_t v = valid ? in[i] : 0;
+        const uint32_t k = key_of(v);
+        uint64_t c = 0;
+        if (valid) {
+            for (uint64_t h = fmix32(k) & omask;; h = (h + 1) & omask) {
+                const uint64_t t = other[h];
+                if (t == EMPTY_SLOT) break;
+                c += key_of(t) == k;
+            }
+            if (k == 0xFFFFFFFFu) c += nsent;  // all-ones tuples of the other side
+        }
+        const uint64_t incl = wave_inclusive_scan(c);
+        const uint64_t total = __shfl(incl, 63);
+        uint64_t wbase = 0;
+        if (lane_id() == 63 && total)
+            wbase = atomicAdd((unsigned long long *)cursor, (unsigned long long)total);
+        uint64_t pos = __shfl(wbase, 63) + incl - c;
+        if (c) {
+            auto emit = [&](uint64_t t) {
+                const uint64_t pv = is_r ? pair_value(v, t) : pair_value(t, v);
+                sum += pv;
+                if (pos < cap) out[pos] = mode == 2 ? v : pv;
+                ++pos;
+            };
+            for (uint64_t h = fmix32(k) & omask;; h = (h + 1) & omask) {
+                const uint64_t t = other[h];
+                if (t == EMPTY_SLOT) break;
+                if (key_of(t) == k) emit(t);
+            }
+            if (k == 0xFFFFFFFFu)
+                for (uint64_t n = 0; n < nsent; ++n) emit(EMPTY_SLOT);
+            cnt += c;
+        }
+        if (!valid) continue;
+        if (v == EMPTY_SLOT) {
+            ++sent;
+            continue;
+        }
+        uint64_t h = fmix32(k) & mask;
+        // terminates: fewer tuples than slots
+        while (atomicCAS((unsigned long long *)&tab[h], (unsigned long long)EMPTY_SLOT,
+                         (unsigned long long)v) != EMPTY_SLOT)
+            h = (h + 1) & mask;
+    }
+    sent = block_sum<SBLOCK>(sent, red);
+    cnt = block_sum<SBLOCK>(cnt, red);
+    sum = block_sum<SBLOCK>(sum, red);
+    if (threadIdx.x == 0) {
+        if (sent) atomicAdd((unsigned long long *)own_sent, (unsigned long long)sent);
+        if (cnt) {
+            step_cnt[blockIdx.x] += cnt;
+            sum_part[blockIdx.x] += sum;
+        }
+    }
+}
+
 // Work-group k: rows[k] = matches of step k, the sum of the per-block row
 // step_cnt[k * G + b] (coalesced reads across the work-group).
 __global__ __launch_bounds__(SBLOCK) void k_shj_rows(const uint64_t *step_cnt, unsigned G,
@@ -144,11 +221,17 @@ uint64_t shj_steps(uint64_t nR, uint64_t nS, uint64_t batch) {
 }
 
 int shj_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, uint64_t nS,
-               uint64_t batch, uint64_t *progress, aj_join_out_t *out, hipStream_t st) {
+               uint64_t batch, uint64_t *progress, uint64_t *pairs, uint64_t pcap, int records,
+               aj_join_out_t *out, hipStream_t st) {
     if (batch == 0) {
         set_error(AJ_ERR_ARG, "SHJ batch must be >= 1");
         return AJ_ERR_ARG;
     }
+    if (pairs && records == 2) {
+        set_error(AJ_ERR_ARG, "SHJ has no timed records (records must be 0 or 1)");
+        return AJ_ERR_ARG;
+    }
+    const int mode = pairs ? (records ? 2 : 1) : 0;
     const uint64_t capR = table_cap(nR), capS = table_cap(nS);
     const uint64_t steps = shj_steps(nR, nS, batch);
     if (steps > (1ull << 20)) {
@@ -161,7 +244,8 @@ int shj_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
     AJ_TRY(ctx->table2.ensure(capS * 8));
     AJ_TRY(ctx->acc.ensure(8 * 8));
     AJ_TRY(ctx->steps.ensure(8 * (steps * G + G + steps)));
-    uint64_t *acc = ctx->acc.as<uint64_t>();  // [0] count [1] sum [2] R sentinels [3] S sentinels
+    // [0] count [1] sum [2] R sentinels [3] S sentinels [4] output cursor
+    uint64_t *acc = ctx->acc.as<uint64_t>();
     uint64_t *tr = ctx->table.as<uint64_t>(), *ts = ctx->table2.as<uint64_t>();
     uint64_t *scnt = ctx->steps.as<uint64_t>();  // [steps][G] per-block step matches
     uint64_t *sum_part = scnt + steps * G;       // [G] per-block checksums
@@ -175,15 +259,27 @@ int shj_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
         const uint64_t lo = k * batch;
         if (lo < nR) {
             const uint64_t hi = lo + batch < nR ? lo + batch : nR;
-            hipLaunchKernelGGL(k_shj_step, dim3(G), dim3(SBLOCK), 0, st, dR, lo, hi, tr, capR - 1,
-                               acc + 2, (const uint64_t *)ts, capS - 1,
-                               (const uint64_t *)(acc + 3), 1, sum_part, scnt + k * G);
+            if (mode)
+                hipLaunchKernelGGL(k_shj_step_out, dim3(G), dim3(SBLOCK), 0, st, dR, lo, hi, tr,
+                                   capR - 1, acc + 2, (const uint64_t *)ts, capS - 1,
+                                   (const uint64_t *)(acc + 3), 1, sum_part, scnt + k * G, mode,
+                                   pairs, pcap, acc + 4);
+            else
+                hipLaunchKernelGGL(k_shj_step, dim3(G), dim3(SBLOCK), 0, st, dR, lo, hi, tr,
+                                   capR - 1, acc + 2, (const uint64_t *)ts, capS - 1,
+                                   (const uint64_t *)(acc + 3), 1, sum_part, scnt + k * G);
         }
         if (lo < nS) {
             const uint64_t hi = lo + batch < nS ? lo + batch : nS;
-            hipLaunchKernelGGL(k_shj_step, dim3(G), dim3(SBLOCK), 0, st, dS, lo, hi, ts, capS - 1,
-                               acc + 3, (const uint64_t *)tr, capR - 1,
-                               (const uint64_t *)(acc + 2), 0, sum_part, scnt + k * G);
+            if (mode)
+                hipLaunchKernelGGL(k_shj_step_out, dim3(G), dim3(SBLOCK), 0, st, dS, lo, hi, ts,
+                                   capS - 1, acc + 3, (const uint64_t *)tr, capR - 1,
+                                   (const uint64_t *)(acc + 2), 0, sum_part, scnt + k * G, mode,
+                                   pairs, pcap, acc + 4);
+            else
+                hipLaunchKernelGGL(k_shj_step, dim3(G), dim3(SBLOCK), 0, st, dS, lo, hi, ts,
+                                   capS - 1, acc + 3, (const uint64_t *)tr, capR - 1,
+                                   (const uint64_t *)(acc + 2), 0, sum_part, scnt + k * G);
         }
     }
     hipLaunchKernelGGL(k_shj_rows, dim3((unsigned)steps), dim3(SBLOCK), 0, st,
@@ -192,7 +288,7 @@ int shj_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
                        (const uint64_t *)sum_part, prog, acc);
     AJ_HIP(hipGetLastError());
     AJ_HIP(hipEventRecord(ctx->ev1, st));
-    uint64_t host[2];
+    uint64_t host[5];
     AJ_HIP(hipMemcpyAsync(host, acc, sizeof(host), hipMemcpyDeviceToHost, st));
     if (progress)
         AJ_HIP(hipMemcpyAsync(progress, prog, 8 * steps, hipMemcpyDefault, st));
@@ -201,7 +297,7 @@ int shj_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
     AJ_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     out->count = host[0];
     out->checksum = host[1];
-    out->npairs = 0;
+    out->npairs = mode ? host[4] : 0;
     out->device_ms = ms;
     return AJ_OK;
 }

@@ -93,7 +93,11 @@ typedef struct aj_result_t {
  * (onlinejoins.h:94, onlinejoins.cpp:132-162): the join-matrix and
  * join-biclique/ContRand layouts only decide which CPU thread sees which tuple,
  * and all three return the same matches.  nthreads is recorded only.  Count
- * and checksum as PRO; options materialize and timing must be 0.
+ * and checksum as PRO.  Option materialize 1 returns the pairs, 2 the
+ * reference's records: per match the probing (later-arriving) tuple's
+ * <key, payloadID>, as SHJJoiner::join's probe appends them
+ * (common_functions.cpp:251-257), all in result list entry 0.  Option timing
+ * must be 0.
  */
 result_t *SHJ_JM_NP(relation_t *relR, relation_t *relS, int nthreads);
 result_t *SHJ_JBCR_NP(relation_t *relR, relation_t *relS, int nthreads);
@@ -244,9 +248,11 @@ typedef struct aj_join_opts_t {
     uint64_t *pairs;   /* device buffer for pairs, NULL = count/checksum only */
     uint64_t pairs_capacity;
     int records;       /* 1: write the reference's result records <S.key, S.payloadID>
-                          (the matching S tuple, one per match) into pairs instead of
+                          (the matching S tuple, one per match; for SHJ the probing,
+                          later-arriving tuple) into pairs instead of
                           (R.payloadID << 32 | S.payloadID); 2: timed records
-                          (device clock << 32 | S.payloadID); count and checksum unchanged */
+                          (device clock << 32 | S.payloadID; not SHJ); count and
+                          checksum unchanged */
     uint64_t batch;    /* SHJ: tuples per side per eager step, 0 = max(65536, n / 256) */
     uint64_t *progress; /* SHJ (nullable, host or device): matches after each step,
                           ceil(max(|R|, |S|) / batch) entries */

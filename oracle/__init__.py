@@ -303,6 +303,53 @@ def ref_shj(R, S) -> int:
     return int(c.value)
 
 
+def ref_shj_records(R, S):
+    """The reference's own single-thread shj with its materialised records: per match
+    the probing (later-arriving) tuple's key << 32 | payloadID, sorted; and the count."""
+    R, S = _c(R), _c(S)
+    cnt, _ = aggregate(R, S)
+    buf = np.empty(max(cnt, 1), np.uint64)
+    f = ref().ref_shj_records
+    f.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64,
+                  ctypes.POINTER(ctypes.c_int64), ctypes.c_void_p, ctypes.c_uint64,
+                  ctypes.POINTER(ctypes.c_uint64)]
+    f.restype = ctypes.c_int
+    c, n = ctypes.c_int64(), ctypes.c_uint64()
+    if f(_p(R), len(R), _p(S), len(S), ctypes.byref(c), buf.ctypes.data, cnt, ctypes.byref(n)):
+        raise ValueError("the reference's SHJ needs |R|, |S| >= 2")
+    return int(c.value), np.sort(buf[:n.value])
+
+
+def shj_records(R, S, batch: int) -> np.ndarray:
+    """Sorted SHJ records under the step rule: arrivals in steps of `batch` tuples per
+    side, R's part of a step before S's; a pair (r_i, s_j) of equal keys is found by
+    the probe of its later tuple, which is r_i when floor(i/batch) > floor(j/batch) and
+    s_j otherwise, and the record is that tuple (key << 32 | payloadID).  For batch = 1
+    this is the reference's alternating R[0], S[0], R[1], ... order
+    (localjoiner.cpp:44-53).  Written out pair by pair: for small inputs."""
+    R, S = _c(R), _c(S)
+    rk = R["key"].astype(np.int64)
+    sk = S["key"].astype(np.int64)
+    rv = (R["key"].astype(np.uint32).astype(np.uint64) << np.uint64(32)) | \
+        R["payloadID"].astype(np.uint32).astype(np.uint64)
+    sv = (S["key"].astype(np.uint32).astype(np.uint64) << np.uint64(32)) | \
+        S["payloadID"].astype(np.uint32).astype(np.uint64)
+    ro, so = np.argsort(rk, kind="stable"), np.argsort(sk, kind="stable")
+    keys, r_start, r_cnt = np.unique(rk[ro], return_index=True, return_counts=True)
+    out = []
+    for key, a, na in zip(keys, r_start, r_cnt):
+        lo, hi = np.searchsorted(sk[so], [key, key + 1])
+        if hi == lo:
+            continue
+        ri = ro[a:a + na]
+        sj = so[lo:hi]
+        r_later = (ri[:, None] // batch) > (sj[None, :] // batch)
+        out.append(np.where(r_later, rv[ri][:, None], sv[sj][None, :]).ravel())
+    if not out:
+        return np.zeros(0, np.uint64)
+    return np.sort(np.concatenate(out))
+
+
 def shj_progress(R, S, batch: int) -> np.ndarray:
     """Matches found after each eager step of `batch` tuples per side (R part
     first): a pair (r_i, s_j) belongs to step max(i // batch, j // batch) --

@@ -128,6 +128,47 @@ int ref_shj(const void *R, uint64_t nR, const void *S, uint64_t nS, int64_t *cou
     return 0;
 }
 
+// The reference's own shj() with its records: every match appends the probing
+// (later-arriving) tuple's <key, payloadID> to the chained buffer
+// (common_functions.cpp:251-257, built with JOIN_RESULT_MATERIALIZE); list
+// (nullable) receives them as (key << 32 | payloadID).  Returns 0, -1 on bad
+// arguments, -3 if list is too small.
+int ref_shj_records(const void *R, uint64_t nR, const void *S, uint64_t nS, int64_t *count,
+                    uint64_t *list, uint64_t cap, uint64_t *nlist) {
+    if (!count || nR < 2 || nS < 2) return -1;  // the tables need >= 1 bucket
+    relation_t r, s;
+    r.tuples = (tuple_t *)malloc(nR * sizeof(tuple_t));
+    s.tuples = (tuple_t *)malloc(nS * sizeof(tuple_t));
+    memcpy(r.tuples, R, nR * sizeof(tuple_t));
+    memcpy(s.tuples, S, nS * sizeof(tuple_t));
+    r.num_tuples = nR;
+    s.num_tuples = nS;
+    r.payload = s.payload = nullptr;
+    chainedtuplebuffer_t *cb = chainedtuplebuffer_init();
+    {
+        SHJJoiner j = shj(0, &r, &s, cb);
+        *count = j.matches;
+    }
+    int rc = 0;
+    uint64_t n = 0;
+    uint32_t len = cb->writepos;  // newest buffer first; older buffers are full
+    for (tuplebuffer_t *b = cb->buf; b; b = b->next, len = CHAINEDBUFF_NUMTUPLESPERBUF) {
+        for (uint32_t i = 0; i < len; i++) {
+            if (list) {
+                if (n >= cap) { rc = -3; break; }
+                list[n] = ((uint64_t)(uint32_t)b->tuples[i].key << 32) |
+                          (uint32_t)b->tuples[i].payloadID;
+            }
+            n++;
+        }
+    }
+    if (nlist) *nlist = n;
+    chainedtuplebuffer_free(cb);
+    free(r.tuples);
+    free(s.tuples);
+    return rc;
+}
+
 // The reference's own breakdown_global of the lazy joins (timer/t_timer.cpp:
 // 113-140) for the given phase cycles, written to path.  (merge / dump_timing,
 // the progressive records, compile to nothing here: the reference's

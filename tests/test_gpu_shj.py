@@ -1,5 +1,6 @@
 """GPU: the eager symmetric hash join (SHJ_JM_NP / SHJ_JBCR_NP / SHJ_st and
-algo AJ_ALGO_SHJ of the device API) against the oracle.
+algo AJ_ALGO_SHJ of the device API) against the oracle, counts, progress curves,
+pairs and the reference's per-match records.
 
 Bit-exact bar: count and checksum equal the equi-join's (closed form by key
 aggregation, and the reference's own single-thread SHJ count), and the
@@ -71,13 +72,89 @@ def test_shj_progress(aj, oracle, nR, nS, batch):
     assert np.array_equal(prog, want)
 
 
-def test_shj_rejects_materialisation(aj):
-    R = aj.gen_unique(100, 1)
+def _shj_inputs(aj, kind):
+    if kind == "dups":
+        return aj.gen_uniform(3000, 1, 300, 31), aj.gen_uniform(2500, 1, 300, 32)
+    R = aj.as_tuples(np.full(1200, -1), np.full(1200, -1))  # all-ones tuples on both sides
+    R["key"][::3] = 5
+    S = aj.as_tuples(np.full(900, -1), np.full(900, -1))
+    S["key"][1::2] = 5
+    return R, S
+
+
+@pytest.mark.parametrize("kind", ["dups", "all_ones"])
+@pytest.mark.parametrize("batch", [1, 7, 1000])
+def test_shj_records_step_rule(aj, oracle, kind, batch):
+    """Device API with records: per match the probing (later) tuple, as the step rule
+    fixes it; at batch 1 that is the reference's own shj records."""
+    R, S = _shj_inputs(aj, kind)
+    want = oracle.shj_records(R, S, batch)
+    if batch == 1 and oracle.ref_available():
+        c, ref = oracle.ref_shj_records(R, S)
+        assert c == len(want) and np.array_equal(ref, want)
+    cnt = len(want)
+    print(f"case SHJ records {kind} batch={batch} matches={cnt}", flush=True)
+    with aj.Context(0) as ctx:
+        dR, dS = aj.DeviceArray.from_host(R), aj.DeviceArray.from_host(S)
+        buf = aj.DeviceArray(8 * cnt)
+        try:
+            out = ctx.join(dR.ptr, len(R), dS.ptr, len(S), algo=aj.AJ_ALGO_SHJ, batch=batch,
+                           pairs_ptr=buf.ptr, pairs_capacity=cnt, records=1)
+            got = np.sort(buf.to_host(np.uint64, cnt))
+        finally:
+            dR.free()
+            dS.free()
+            buf.free()
+    assert (out.count, out.checksum) == oracle.aggregate(R, S) and out.npairs == cnt
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("kind", ["dups", "all_ones"])
+def test_shj_pairs_and_dropin_results(aj, oracle, kind):
+    """Pairs through the device API and the drop-ins' materialize 1 (pairs) and 2 (the
+    reference's record lists); a too small pair buffer is an error."""
+    R, S = _shj_inputs(aj, kind)
+    pairs = oracle.pairs_sortmerge(R, S)
+    cnt = len(pairs)
+    with aj.Context(0) as ctx:
+        dR, dS = aj.DeviceArray.from_host(R), aj.DeviceArray.from_host(S)
+        buf = aj.DeviceArray(8 * cnt)
+        try:
+            out = ctx.join(dR.ptr, len(R), dS.ptr, len(S), algo=aj.AJ_ALGO_SHJ, batch=64,
+                           pairs_ptr=buf.ptr, pairs_capacity=cnt)
+            assert out.npairs == cnt
+            assert np.array_equal(np.sort(buf.to_host(np.uint64, cnt)), pairs)
+            with pytest.raises(aj.AjError) as e:
+                ctx.join(dR.ptr, len(R), dS.ptr, len(S), algo=aj.AJ_ALGO_SHJ, batch=64,
+                         pairs_ptr=buf.ptr, pairs_capacity=cnt // 2)
+            assert e.value.status == aj.AJ_ERR_CAPACITY
+        finally:
+            dR.free()
+            dS.free()
+            buf.free()
     aj.set_option("materialize", 1)
+    try:
+        got = aj.SHJ_JM_NP(R, S, 4)
+    finally:
+        aj.set_option("materialize", 0)
+    assert got.count == cnt and np.array_equal(np.sort(got.pairs), pairs)
+    aj.set_option("materialize", 2)
+    try:
+        got = aj.SHJ_st(R, S, 1)
+    finally:
+        aj.set_option("materialize", 0)
+    # the drop-in's step: max(65536, n / 256) tuples per side, one step here
+    assert got.count == cnt and got.pairs is None
+    assert np.array_equal(np.sort(got.records[0]), oracle.shj_records(R, S, 65536))
+
+
+def test_shj_rejects_timing(aj):
+    R = aj.gen_unique(100, 1)
+    aj.set_option("timing", 1)
     try:
         with pytest.raises(aj.AjError):
             aj.SHJ_st(R, R, 1)
     finally:
-        aj.set_option("materialize", 0)
+        aj.set_option("timing", 0)
     E = np.empty(0, dtype=aj.TUPLE_DTYPE)
     assert aj.SHJ_JM_NP(R, E, 1).count == 0

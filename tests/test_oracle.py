@@ -159,3 +159,20 @@ def test_shj_progress_closed_form(aj, oracle):
     S = aj.as_tuples(np.full(n, 3))
     want = np.array([((k + 1) * b) ** 2 for k in range(n // b)], np.uint64)
     assert np.array_equal(oracle.shj_progress(R, S, b), want)
+
+
+def test_shj_records_pinned(aj, oracle):
+    """The SHJ record restatement: at batch 1 the reference's own shj records (when the
+    reference build is present); in one step (batch >= |R|, |S|) every match is found by
+    the S tuple's probe, so each S tuple appears once per R tuple with its key."""
+    R = aj.gen_uniform(700, 1, 90, 41)
+    S = aj.gen_uniform(650, 1, 90, 42)
+    if oracle.ref_available():
+        c, ref = oracle.ref_shj_records(R, S)
+        assert np.array_equal(oracle.shj_records(R, S, 1), ref) and c == len(ref)
+    one = oracle.shj_records(R, S, 1 << 20)
+    cnt = np.bincount(R["key"], minlength=91)
+    sv = (S["key"].astype(np.uint64) << np.uint64(32)) | S["payloadID"].astype(np.uint64)
+    assert np.array_equal(one, np.sort(np.repeat(sv, cnt[S["key"]])))
+    # every record is one of the pair's two tuples and the count is the join's
+    assert len(oracle.shj_records(R, S, 7)) == oracle.aggregate(R, S)[0]
