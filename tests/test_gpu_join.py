@@ -252,6 +252,21 @@ def test_driver_persists_result(aj, oracle, tmp_path):
     assert np.array_equal(np.sort(_parse_tbl(out)), want)
 
 
+def test_driver_persists_shj_records(aj, oracle, tmp_path):
+    """ajbench -a SHJ_st --out writes the eager join's records: per match the probing
+    (later-arriving) tuple, under the drop-in's step of max(65536, n / 256) tuples."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(aj.LIB_PATH), "ajbench")
+    out = tmp_path / "Out.tbl"
+    r = subprocess.run([exe, "-a", "SHJ_st", "-r", "100000", "-s", "150000", "--out", str(out)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    R = aj.gen_unique(100000, 12345)
+    S = aj.gen_uniform(150000, 1, 100000, 54321)
+    assert np.array_equal(np.sort(_parse_tbl(out)), oracle.shj_records(R, S, 65536))
+
+
 @pytest.mark.parametrize("algo", ["PRO", "NPO"])
 def test_driver_runs_join(aj, oracle, algo):
     """The C++ driver joins generated relations and prints the reference's result line."""
