@@ -82,7 +82,7 @@ class aj_result_t(ctypes.Structure):
                 ("device_ms", ctypes.c_float), ("total_ms", ctypes.c_float),
                 ("ntimed", ctypes.c_uint64), ("timed", ctypes.POINTER(ctypes.c_uint64)),
                 ("clock_hz", ctypes.c_double), ("algo", ctypes.c_int),
-                ("phase_ms", ctypes.c_float * 12)]
+                ("phase_ms", ctypes.c_float * 12), ("ntimed_r", ctypes.c_uint64)]
 
 
 class aj_join_opts_t(ctypes.Structure):
@@ -101,7 +101,7 @@ PHASE_NAMES = ["plan1", "hist1", "scan1", "scatter1", "plan2", "hist2", "scan2",
 class aj_join_out_t(ctypes.Structure):
     _fields_ = [("count", ctypes.c_uint64), ("checksum", ctypes.c_uint64),
                 ("npairs", ctypes.c_uint64), ("device_ms", ctypes.c_float),
-                ("phase_ms", ctypes.c_float * AJ_NPHASES)]
+                ("phase_ms", ctypes.c_float * AJ_NPHASES), ("nrecords_r", ctypes.c_uint64)]
 
     def phases(self) -> dict:
         return {n: float(self.phase_ms[i]) for i, n in enumerate(PHASE_NAMES)}
@@ -282,6 +282,8 @@ class JoinResult:
     timed: Optional[np.ndarray] = None
     clock_hz: float = 0.0
     phase_ms: Optional[list] = None
+    # SHJ: the last timed_r timed records are R tuples' probes (R.payloadID)
+    timed_r: int = 0
 
 
 def _read_chain(cb: aj_chainedtuplebuffer_t, n: int) -> np.ndarray:
@@ -333,7 +335,7 @@ def _dropin(fn, R: np.ndarray, S: np.ndarray, nthreads: int,
             timed = np.ctypeslib.as_array(ext.timed, shape=(ext.ntimed,)).copy()
         return JoinResult(int(ext.base.totalresults), int(ext.checksum), pairs,
                           float(ext.device_ms), float(ext.total_ms), records, chains, timed,
-                          float(ext.clock_hz), list(ext.phase_ms))
+                          float(ext.clock_hz), list(ext.phase_ms), int(ext.ntimed_r))
     finally:
         lib.aj_free_result(res)
 

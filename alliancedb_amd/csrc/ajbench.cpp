@@ -273,11 +273,15 @@ int main(int argc, char **argv) {
                (long long)res->totalresults, (unsigned long long)x->checksum,
                (unsigned long long)x->npairs, x->device_ms, x->total_ms, ms);
         if (p.timing_dir && rep + 1 == p.repeats) {
-            const bool pro = strcmp(p.algo->name, "PRO") == 0;
+            // the lazy joins' result names (PRJ / NPJ); the eager joins use their
+            // algorithm name, as the reference's finishing does (param.algo_name)
+            const char *tname = strcmp(p.algo->name, "PRO") == 0   ? "PRJ"
+                                : strcmp(p.algo->name, "NPO") == 0 ? "NPJ"
+                                                                   : p.algo->name;
             aj_timing_summary_t sm;
             relR.payload = p.file_r ? &plR : nullptr;
             relS.payload = p.file_s ? &plS : nullptr;
-            if (aj_write_timing_files(res, &relR, &relS, p.timing_dir, pro ? "PRJ" : "NPJ",
+            if (aj_write_timing_files(res, &relR, &relS, p.timing_dir, tname,
                                       p.exp_id, p.window, p.gap, &sm) != AJ_OK) {
                 fprintf(stderr, "[ERROR] %s\n", aj_last_error());
                 aj_free_result(res);

@@ -335,7 +335,7 @@ static threadresult_t *build_resultlist(const uint64_t *dsrc, uint64_t n, int nt
 // Timed records: one per match, (device clock ticks since the join started << 32)
 // | S.payloadID, from a join run with records = 2 after a clock stamp.
 static int timed_run(Ctx *ctx, aj_join_opts_t o, uint64_t nR, uint64_t nS, uint64_t count,
-                     std::vector<uint64_t> *timed, double *clock_hz) {
+                     std::vector<uint64_t> *timed, double *clock_hz, uint64_t *ntimed_r) {
     AJ_TRY(ctx->pairs.ensure((count + 1) * 8));
     uint64_t *stamp = ctx->pairs.as<uint64_t>() + count;
     AJ_TRY(device_clock_stamp(stamp, ctx->stream));
@@ -345,8 +345,9 @@ static int timed_run(Ctx *ctx, aj_join_opts_t o, uint64_t nR, uint64_t nS, uint6
     aj_join_out_t out;
     AJ_TRY(join_dev_locked(ctx, ctx->inR.as<uint64_t>(), nR, ctx->inS.as<uint64_t>(), nS, &o,
                            &out, ctx->stream));
-    if (out.count != count || out.npairs != count)
+    if (out.count != count || out.npairs != count || out.nrecords_r > count)
         return set_error(AJ_ERR_STATE, "timed join disagrees with the counting join");
+    *ntimed_r = out.nrecords_r;  // SHJ: the last ones are R tuples' probes
     timed->resize(count + 1);
     if (hipMemcpyAsync(timed->data(), ctx->pairs.p, (count + 1) * 8, hipMemcpyDeviceToHost,
                        ctx->stream) ||
@@ -382,6 +383,7 @@ static result_t *dropin(int algo, relation_t *relR, relation_t *relS, int nthrea
     uint64_t npairs = 0;
     std::vector<uint64_t> host_pairs, timed;
     double clock_hz = 0;
+    uint64_t ntimed_r = 0;
     threadresult_t *rlist = nullptr;
     int rc = AJ_OK;
     do {
@@ -397,11 +399,6 @@ static result_t *dropin(int algo, relation_t *relR, relation_t *relS, int nthrea
         aj_join_opts_t o;
         memset(&o, 0, sizeof(o));
         o.algo = algo;
-        if (algo == AJ_ALGO_SHJ && timing) {
-            rc = set_error(AJ_ERR_ARG, "SHJ drop-ins have no timed records (option timing must "
-                                       "be 0)");
-            break;
-        }
         g_force_timing = timing;  // per-phase device times of the counting run
         rc = join_dev_locked(ctx, ctx->inR.as<uint64_t>(), nR, ctx->inS.as<uint64_t>(), nS, &o,
                              &out, ctx->stream);
@@ -437,7 +434,7 @@ static result_t *dropin(int algo, relation_t *relR, relation_t *relS, int nthrea
                 }
             }
         }
-        if (timing) rc = timed_run(ctx, o, nR, nS, out.count, &timed, &clock_hz);
+        if (timing) rc = timed_run(ctx, o, nR, nS, out.count, &timed, &clock_hz, &ntimed_r);
     } while (0);
     if (rc) {
         free_resultlist(rlist, nt);
@@ -469,6 +466,7 @@ static result_t *dropin(int algo, relation_t *relR, relation_t *relS, int nthrea
     res->timed = ntimed ? reinterpret_cast<uint64_t *>(res + 1) + npairs : nullptr;
     if (ntimed) memcpy(res->timed, timed.data(), ntimed * 8);
     res->clock_hz = clock_hz;
+    res->ntimed_r = ntimed_r;
     res->algo = algo;
     if (timing) memcpy(res->phase_ms, out.phase_ms, sizeof(res->phase_ms));
     res->total_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0)

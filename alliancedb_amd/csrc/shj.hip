@@ -87,10 +87,14 @@ __global__ __launch_bounds__(SBLOCK) void k_shj_step(const uint64_t *in, uint64_
 // The materialising form of k_shj_step: the same inserts, probes, per-block
 // counts and checksums, and every match also written to out: mode 1 the pair
 // (R payload << 32 | S payload), mode 2 the probing tuple <key, payloadID> (the
-// record SHJJoiner::join's probe appends, common_functions.cpp:251-257).  A
-// wave counts its lanes' matches, reserves one range with a single atomic on
-// *cursor, then walks its probe chains again to write; positions at or past
-// cap are counted but not written (the caller reports the shortfall).  The
+// record SHJJoiner::join's probe appends, common_functions.cpp:251-257), mode 3
+// a timed record (device clock << 32 | the probing tuple's payloadID, the
+// sample END_PROGRESSIVE_MEASURE takes there).  A wave counts its lanes'
+// matches, reserves one range with a single atomic on *cursor, then walks its
+// probe chains again to write; positions at or past cap are counted but not
+// written (the caller reports the shortfall).  In mode 3 an R tuple's records
+// fill out from the end backwards (its cursor counts them), so the caller can
+// tell the two probe sides apart.  The
 // loop runs a work-group-uniform number of rounds so the wave operations see
 // every lane.
 __global__ __launch_bounds__(SBLOCK) void k_shj_step_out(
@@ -122,11 +126,17 @@ __global__ __launch_bounds__(SBLOCK) void k_shj_step_out(
         if (lane_id() == 63 && total)
             wbase = atomicAdd((unsigned long long *)cursor, (unsigned long long)total);
         uint64_t pos = __shfl(wbase, 63) + incl - c;
+        const uint32_t tnow = mode == 3 ? (uint32_t)__builtin_amdgcn_s_memrealtime() : 0u;
         if (c) {
             auto emit = [&](uint64_t t) {
                 const uint64_t pv = is_r ? pair_value(v, t) : pair_value(t, v);
                 sum += pv;
-                if (pos < cap) out[pos] = mode == 2 ? v : pv;
+                if (pos < cap) {
+                    if (mode == 3)
+                        out[is_r ? cap - 1 - pos : pos] = ((uint64_t)tnow << 32) | payload_of(v);
+                    else
+                        out[pos] = mode == 2 ? v : pv;
+                }
                 ++pos;
             };
             for (uint64_t h = fmix32(k) & omask;; h = (h + 1) & omask) {
@@ -227,11 +237,8 @@ int shj_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
         set_error(AJ_ERR_ARG, "SHJ batch must be >= 1");
         return AJ_ERR_ARG;
     }
-    if (pairs && records == 2) {
-        set_error(AJ_ERR_ARG, "SHJ has no timed records (records must be 0 or 1)");
-        return AJ_ERR_ARG;
-    }
-    const int mode = pairs ? (records ? 2 : 1) : 0;
+    // 1 pairs, 2 the reference's records, 3 timed records (R probes from the end)
+    const int mode = pairs ? (records == 2 ? 3 : records ? 2 : 1) : 0;
     const uint64_t capR = table_cap(nR), capS = table_cap(nS);
     const uint64_t steps = shj_steps(nR, nS, batch);
     if (steps > (1ull << 20)) {
@@ -245,6 +252,7 @@ int shj_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
     AJ_TRY(ctx->acc.ensure(8 * 8));
     AJ_TRY(ctx->steps.ensure(8 * (steps * G + G + steps)));
     // [0] count [1] sum [2] R sentinels [3] S sentinels [4] output cursor
+    // [5] R-probe output cursor (timed records)
     uint64_t *acc = ctx->acc.as<uint64_t>();
     uint64_t *tr = ctx->table.as<uint64_t>(), *ts = ctx->table2.as<uint64_t>();
     uint64_t *scnt = ctx->steps.as<uint64_t>();  // [steps][G] per-block step matches
@@ -263,7 +271,7 @@ int shj_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
                 hipLaunchKernelGGL(k_shj_step_out, dim3(G), dim3(SBLOCK), 0, st, dR, lo, hi, tr,
                                    capR - 1, acc + 2, (const uint64_t *)ts, capS - 1,
                                    (const uint64_t *)(acc + 3), 1, sum_part, scnt + k * G, mode,
-                                   pairs, pcap, acc + 4);
+                                   pairs, pcap, mode == 3 ? acc + 5 : acc + 4);
             else
                 hipLaunchKernelGGL(k_shj_step, dim3(G), dim3(SBLOCK), 0, st, dR, lo, hi, tr,
                                    capR - 1, acc + 2, (const uint64_t *)ts, capS - 1,
@@ -288,7 +296,7 @@ int shj_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
                        (const uint64_t *)sum_part, prog, acc);
     AJ_HIP(hipGetLastError());
     AJ_HIP(hipEventRecord(ctx->ev1, st));
-    uint64_t host[5];
+    uint64_t host[6];
     AJ_HIP(hipMemcpyAsync(host, acc, sizeof(host), hipMemcpyDeviceToHost, st));
     if (progress)
         AJ_HIP(hipMemcpyAsync(progress, prog, 8 * steps, hipMemcpyDefault, st));
@@ -297,7 +305,8 @@ int shj_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
     AJ_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     out->count = host[0];
     out->checksum = host[1];
-    out->npairs = mode ? host[4] : 0;
+    out->npairs = mode ? host[4] + host[5] : 0;
+    out->nrecords_r = mode == 3 ? host[5] : 0;
     out->device_ms = ms;
     return AJ_OK;
 }

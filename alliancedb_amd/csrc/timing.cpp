@@ -78,26 +78,43 @@ extern "C" int aj_write_timing_files(const result_t *r, const relation_t *relR,
         return set_error(AJ_ERR_STATE, "result carries no timed records (option timing 1)");
     if (r->totalresults > 0 && !(x->clock_hz > 0))
         return set_error(AJ_ERR_STATE, "result carries no device clock rate");
+    if (x->ntimed_r > x->ntimed) return set_error(AJ_ERR_STATE, "bad count of R-side records");
     const uint64_t *sts = relS->payload ? relS->payload->ts : nullptr;
+    const uint64_t *rts = relR->payload ? relR->payload->ts : nullptr;
 
-    // production order: device time, then S.payloadID; every gap-th sampled
-    std::vector<uint64_t> rec(x->timed, x->timed + x->ntimed);
+    // production order: device time, then side (S before R), then payloadID;
+    // every gap-th sampled.  The last ntimed_r records are R tuples' probes
+    // (SHJ); merge treats the R and S samples alike but against their own
+    // side's timestamps and with their own sample index (t_timer.cpp:412-434).
+    struct Rec {
+        uint64_t ticks;
+        int r;
+        uint32_t id;
+        bool operator<(const Rec &o) const {
+            return ticks != o.ticks ? ticks < o.ticks : r != o.r ? r < o.r : id < o.id;
+        }
+    };
+    std::vector<Rec> rec(x->ntimed);
+    const uint64_t first_r = x->ntimed - x->ntimed_r;
+    for (uint64_t k = 0; k < x->ntimed; ++k)
+        rec[k] = {x->timed[k] >> 32, k >= first_r, (uint32_t)x->timed[k]};
     std::sort(rec.begin(), rec.end());
     std::vector<double> times, lat, gaps;
     const uint64_t wait_cycles = (uint64_t)(lastTS * 2.1 * 1E6);
     const double cycles_per_tick = 2.1e9 / x->clock_hz;  // 21 exactly for a 100 MHz clock
-    int64_t idx = 0;
-    for (uint64_t k = (uint64_t)gap - 1; k < rec.size(); k += (uint64_t)gap, ++idx) {
-        const uint64_t ticks = rec[k] >> 32;
-        const uint32_t sid = (uint32_t)rec[k];
-        if (sts && sid >= relS->num_tuples)
-            return set_error(AJ_ERR_ARG, "S.payloadID %u outside relS's %llu timestamps", sid,
-                             (unsigned long long)relS->num_tuples);
-        const uint64_t cycles = (uint64_t)((double)ticks * cycles_per_tick);
+    int64_t idx[2] = {0, 0};
+    for (uint64_t k = (uint64_t)gap - 1; k < rec.size(); k += (uint64_t)gap) {
+        const Rec &e = rec[k];
+        const relation_t *rel = e.r ? relR : relS;
+        const uint64_t *ts = e.r ? rts : sts;
+        if (ts && e.id >= rel->num_tuples)
+            return set_error(AJ_ERR_ARG, "%s.payloadID %u outside the relation's %llu timestamps",
+                             e.r ? "R" : "S", e.id, (unsigned long long)rel->num_tuples);
+        const uint64_t cycles = (uint64_t)((double)e.ticks * cycles_per_tick);
         times.push_back(cycles / (2.1 * 1E6));
-        const int64_t latency = (int64_t)(cycles - (sts ? sts[sid] : 0) + wait_cycles);
+        const int64_t latency = (int64_t)(cycles - (ts ? ts[e.id] : 0) + wait_cycles);
         lat.push_back(latency / (2.1 * 1E6));
-        gaps.push_back((double)((int32_t)sid - (int32_t)idx));
+        gaps.push_back((double)((int32_t)e.id - (int32_t)idx[e.r]++));
     }
     std::sort(times.begin(), times.end());
     std::sort(lat.begin(), lat.end());
@@ -125,6 +142,8 @@ extern "C" int aj_write_timing_files(const result_t *r, const relation_t *relR,
         if (x->algo == AJ_ALGO_NPO) {
             build = cyc(ph[AJ_PH_NPO_BUILD]);
             join = cyc(ph[AJ_PH_NPO_PROBE]);
+        } else if (x->algo == AJ_ALGO_SHJ) {
+            join = cyc(x->device_ms);  // an eager join builds and probes in one phase
         } else {
             float p = 0;
             for (int i = AJ_PH_PLAN1; i <= AJ_PH_SCATTER2; ++i) p += ph[i];

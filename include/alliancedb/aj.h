@@ -78,8 +78,10 @@ typedef struct aj_result_t {
     uint64_t ntimed;   /* timed records, one per match */
     uint64_t *timed;   /* (device clock ticks since the join started << 32) | S.payloadID */
     double clock_hz;   /* tick rate of timed (the device's constant-rate clock) */
-    int algo;          /* AJ_ALGO_PRO or AJ_ALGO_NPO */
+    int algo;          /* AJ_ALGO_PRO, AJ_ALGO_NPO or AJ_ALGO_SHJ */
     float phase_ms[12]; /* per-phase device times, indexed by enum aj_phase */
+    uint64_t ntimed_r; /* SHJ: the last ntimed_r timed records are R tuples' probes
+                          (R.payloadID in the low word); 0 for PRO and NPO */
 } aj_result_t;
 
 /*
@@ -97,7 +99,9 @@ typedef struct aj_result_t {
  * reference's records: per match the probing (later-arriving) tuple's
  * <key, payloadID>, as SHJJoiner::join's probe appends them
  * (common_functions.cpp:251-257), all in result list entry 0.  Option timing
- * must be 0.
+ * 1 gives timed records of both probe sides (aj_result_t.ntimed_r), which
+ * aj_write_timing_files turns into the reference's files as its merge does
+ * (R probes against R's timestamps, S probes against S's).
  */
 result_t *SHJ_JM_NP(relation_t *relR, relation_t *relS, int nthreads);
 result_t *SHJ_JBCR_NP(relation_t *relR, relation_t *relS, int nthreads);
@@ -251,7 +255,8 @@ typedef struct aj_join_opts_t {
                           (the matching S tuple, one per match; for SHJ the probing,
                           later-arriving tuple) into pairs instead of
                           (R.payloadID << 32 | S.payloadID); 2: timed records
-                          (device clock << 32 | S.payloadID; not SHJ); count and
+                          (device clock << 32 | S.payloadID; SHJ: of the probing
+                          tuple, R probes from the end, out.nrecords_r); count and
                           checksum unchanged */
     uint64_t batch;    /* SHJ: tuples per side per eager step, 0 = max(65536, n / 256) */
     uint64_t *progress; /* SHJ (nullable, host or device): matches after each step,
@@ -282,6 +287,8 @@ typedef struct aj_join_out_t {
     uint64_t npairs;
     float device_ms;
     float phase_ms[AJ_NPHASES];
+    uint64_t nrecords_r; /* SHJ timed records (records = 2): R tuples' probes, written
+                            from the end of the pair buffer backwards */
 } aj_join_out_t;
 
 /*

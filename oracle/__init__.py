@@ -268,19 +268,25 @@ def ref_breakdown_file(path: str, nR: int, nS: int, last_ts: int, overall: int, 
         raise RuntimeError("ref_breakdown_file failed")
 
 
-def timing_records(cycles, sids, s_ts, last_ts: int):
-    """Restatement of the reference's merge + sortRecords for S-side samples
-    (timer/t_timer.cpp:401-452): per sample, the match time in ms, the latency
-    (unsigned cycle arithmetic, then signed) in ms and the disorder gap
-    S.payloadID - sample index; each list sorted ascending."""
+def timing_records(cycles, sids, s_ts, last_ts: int, sides=None, r_ts=None):
+    """Restatement of the reference's merge + sortRecords (timer/t_timer.cpp:
+    401-452): per sample, the match time in ms, the latency (unsigned cycle
+    arithmetic, then signed) in ms against the sample's own side's timestamps,
+    and the disorder gap payloadID - index of the sample among its side's samples;
+    each list sorted ascending.  sides[i] = 1 marks an R-side sample (an R tuple's
+    probe in the eager joins), default all S."""
     wait = int(last_ts * 2.1 * 1e6)
     times, lat, gaps = [], [], []
+    idx = [0, 0]
     for i, (c, s) in enumerate(zip(cycles, sids)):
         c, s = int(c), int(s)
+        r = int(sides[i]) if sides is not None else 0
+        ts = r_ts if r else s_ts
         times.append(c / (2.1 * 1e6))
-        u = (c - (int(s_ts[s]) if s_ts is not None else 0) + wait) % 2**64
+        u = (c - (int(ts[s]) if ts is not None else 0) + wait) % 2**64
         lat.append((u - 2**64 if u >= 2**63 else u) / (2.1 * 1e6))
-        gaps.append(float(s - i))
+        gaps.append(float(s - idx[r]))
+        idx[r] += 1
     return sorted(times), sorted(lat), sorted(gaps)
 
 
@@ -320,34 +326,37 @@ def ref_shj_records(R, S):
     return int(c.value), np.sort(buf[:n.value])
 
 
-def shj_records(R, S, batch: int) -> np.ndarray:
-    """Sorted SHJ records under the step rule: arrivals in steps of `batch` tuples per
-    side, R's part of a step before S's; a pair (r_i, s_j) of equal keys is found by
-    the probe of its later tuple, which is r_i when floor(i/batch) > floor(j/batch) and
-    s_j otherwise, and the record is that tuple (key << 32 | payloadID).  For batch = 1
-    this is the reference's alternating R[0], S[0], R[1], ... order
-    (localjoiner.cpp:44-53).  Written out pair by pair: for small inputs."""
+def shj_records_by_side(R, S, batch: int):
+    """Sorted SHJ records under the step rule, split by the probing side: arrivals in
+    steps of `batch` tuples per side, R's part of a step before S's; a pair (r_i, s_j)
+    of equal keys is found by the probe of its later tuple, which is r_i when
+    floor(i/batch) > floor(j/batch) and s_j otherwise, and the record is that tuple
+    (key << 32 | payloadID).  For batch = 1 this is the reference's alternating
+    R[0], S[0], R[1], ... order (localjoiner.cpp:44-53).  Returns (records of R
+    tuples' probes, records of S tuples' probes); every pair is expanded, so the
+    inputs must be small enough for their pair count."""
     R, S = _c(R), _c(S)
-    rk = R["key"].astype(np.int64)
-    sk = S["key"].astype(np.int64)
+    rk, sk = R["key"].astype(np.int64), S["key"].astype(np.int64)
     rv = (R["key"].astype(np.uint32).astype(np.uint64) << np.uint64(32)) | \
         R["payloadID"].astype(np.uint32).astype(np.uint64)
     sv = (S["key"].astype(np.uint32).astype(np.uint64) << np.uint64(32)) | \
         S["payloadID"].astype(np.uint32).astype(np.uint64)
-    ro, so = np.argsort(rk, kind="stable"), np.argsort(sk, kind="stable")
-    keys, r_start, r_cnt = np.unique(rk[ro], return_index=True, return_counts=True)
-    out = []
-    for key, a, na in zip(keys, r_start, r_cnt):
-        lo, hi = np.searchsorted(sk[so], [key, key + 1])
-        if hi == lo:
-            continue
-        ri = ro[a:a + na]
-        sj = so[lo:hi]
-        r_later = (ri[:, None] // batch) > (sj[None, :] // batch)
-        out.append(np.where(r_later, rv[ri][:, None], sv[sj][None, :]).ravel())
-    if not out:
-        return np.zeros(0, np.uint64)
-    return np.sort(np.concatenate(out))
+    ro = np.argsort(rk, kind="stable")
+    rks = rk[ro]
+    lo = np.searchsorted(rks, sk, side="left")
+    cnt = np.searchsorted(rks, sk, side="right") - lo
+    total = int(cnt.sum())
+    sj = np.repeat(np.arange(len(S), dtype=np.int64), cnt)
+    offs = np.arange(total, dtype=np.int64) - np.repeat(np.cumsum(cnt) - cnt, cnt)
+    ri = ro[np.repeat(lo, cnt) + offs]
+    r_later = (ri // batch) > (sj // batch)
+    return np.sort(rv[ri[r_later]]), np.sort(sv[sj[~r_later]])
+
+
+def shj_records(R, S, batch: int) -> np.ndarray:
+    """All SHJ records under the step rule (shj_records_by_side), sorted."""
+    r, s = shj_records_by_side(R, S, batch)
+    return np.sort(np.concatenate([r, s]))
 
 
 def shj_progress(R, S, batch: int) -> np.ndarray:

@@ -148,13 +148,44 @@ def test_shj_pairs_and_dropin_results(aj, oracle, kind):
     assert np.array_equal(np.sort(got.records[0]), oracle.shj_records(R, S, 65536))
 
 
-def test_shj_rejects_timing(aj):
+def test_shj_empty_side(aj):
     R = aj.gen_unique(100, 1)
-    aj.set_option("timing", 1)
-    try:
-        with pytest.raises(aj.AjError):
-            aj.SHJ_st(R, R, 1)
-    finally:
-        aj.set_option("timing", 0)
     E = np.empty(0, dtype=aj.TUPLE_DTYPE)
     assert aj.SHJ_JM_NP(R, E, 1).count == 0
+
+
+def test_shj_timed_records_both_sides(aj, oracle, tmp_path):
+    """Option timing = 1 on the eager join: one timed record per match, the probing
+    tuple's payloadID; the last timed_r of them are R tuples' probes.  Under the
+    drop-in's step (65536 tuples per side, four steps here) both sides probe, and each
+    side's payloadIDs are the ones the step rule assigns to it.  The driver writes the
+    progressive files under the algorithm's name."""
+    import os
+    import subprocess
+    R = aj.gen_unique(200000, 12345)
+    S = aj.gen_uniform(220000, 1, 200000, 54321)
+    aj.set_option("timing", 1)
+    try:
+        print("case SHJ timed records", flush=True)
+        got = aj.SHJ_st(R, S, 1)
+    finally:
+        aj.set_option("timing", 0)
+    r_rec, s_rec = oracle.shj_records_by_side(R, S, 65536)
+    lo = lambda v: np.sort(v & np.uint64(0xFFFFFFFF))  # noqa: E731
+    assert got.count == len(r_rec) + len(s_rec) == len(got.timed)
+    assert 0 < got.timed_r == len(r_rec) < got.count
+    assert np.array_equal(lo(got.timed[:got.count - got.timed_r]), lo(s_rec))
+    assert np.array_equal(lo(got.timed[got.count - got.timed_r:]), lo(r_rec))
+    ms = (got.timed >> np.uint64(32)).astype(np.float64) / got.clock_hz * 1e3
+    assert ms.min() >= 0 and ms.max() < 1000
+    exe = os.path.join(os.path.dirname(aj.LIB_PATH), "ajbench")
+    d = tmp_path / "exp"
+    r = subprocess.run([exe, "-a", "SHJ_JM_NP", "-r", "200000", "-s", "220000", "--timing",
+                        str(d), "--exp-id", "5"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    res = d / "results"
+    assert (res / "records" / "SHJ_JM_NP_5.txt").read_text().split() == ["420000",
+                                                                       str(got.count)]
+    for sub in ("timestamps", "latency", "gaps"):
+        v = np.array((res / sub / "SHJ_JM_NP_5.txt").read_text().split(), float)
+        assert len(v) == got.count and np.all(np.diff(v) >= 0), sub

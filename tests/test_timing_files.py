@@ -126,3 +126,45 @@ def test_timing_files_closed_form(aj, tmp_path):
     bd = [float(x) for x in rd("breakdown")]
     assert bd[0] == pytest.approx(10 * 2.1e6 / 5) and bd[1:6] == [0, 0, 0, 0, 0]
     assert summ.samples == 3 and summ.progress_ms[4] == 11.0 and summ.latency95_ms == 10.0
+
+
+@pytest.mark.parametrize("gap,last_ts", [(1, 0), (4, 500)])
+def test_timing_files_both_probe_sides(aj, oracle, tmp_path, gap, last_ts):
+    """Eager-join results: the last ntimed_r records are R tuples' probes.  Samples are
+    taken in production order over both sides, then each is measured against its own
+    side's timestamps and counted in its own side's gap index, as merge does for the
+    reference's recordR / recordS lists (t_timer.cpp:412-434)."""
+    rng = np.random.default_rng(gap + 11)
+    nR, nS, ns, nr = 4000, 6000, 9000, 5000
+    s_ticks = rng.integers(0, 2_000_000, ns).astype(np.uint64)
+    r_ticks = rng.integers(0, 2_000_000, nr).astype(np.uint64)
+    r_ticks[:30] = s_ticks[0]  # ties across the sides: S first
+    sids = rng.integers(0, nS, ns).astype(np.uint64)
+    rids = rng.integers(0, nR, nr).astype(np.uint64)
+    s_ts = np.sort(rng.integers(0, int(last_ts * 2.1e6) + 1, nS)).astype(np.uint64)
+    r_ts = np.sort(rng.integers(0, int(last_ts * 2.1e6) + 1, nR)).astype(np.uint64)
+    timed = np.concatenate([(s_ticks << np.uint64(32)) | sids, (r_ticks << np.uint64(32)) | rids])
+    keep = []
+    res = _result(keep, ns + nr, timed, 2.1e9, 2, [0.0] * 12, 1.5)
+    res.ntimed_r = nr
+    pr = _lib.relation_payload_t(r_ts.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), nR)
+    ps = _lib.relation_payload_t(s_ts.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), nS)
+    relR = _lib.relation_t(None, nR, ctypes.addressof(pr))
+    relS = _lib.relation_t(None, nS, ctypes.addressof(ps))
+    ours = tmp_path / "ours"
+    _lib.check(_lib.lib.aj_write_timing_files(
+        ctypes.cast(ctypes.pointer(res), ctypes.POINTER(_lib.result_t)), ctypes.byref(relR),
+        ctypes.byref(relS), str(ours).encode(), b"SHJ", 1, last_ts, gap, None))
+    # production order: ticks, then side (S before R), then payloadID
+    t = np.concatenate([s_ticks, r_ticks])
+    side = np.concatenate([np.zeros(ns, np.int64), np.ones(nr, np.int64)])
+    pid = np.concatenate([sids, rids]).astype(np.int64)
+    order = np.lexsort((pid, side, t))[gap - 1::gap]
+    times, lat, gaps = oracle.timing_records(t[order], pid[order], s_ts, last_ts,
+                                             sides=side[order], r_ts=r_ts)
+    rd = lambda d: (ours / "results" / d / "SHJ_1.txt").read_text()  # noqa: E731
+    assert rd("timestamps") == oracle.timing_text(times, last_ts)
+    assert rd("latency") == oracle.timing_text(lat, last_ts)
+    assert rd("gaps") == oracle.timing_text(gaps, last_ts)
+    # a plain PRO-style result (no R-side records) is unchanged by the side handling
+    assert rd("records") == f"{nR + nS}\n{ns + nr}\n"
