@@ -168,3 +168,34 @@ def test_timing_files_both_probe_sides(aj, oracle, tmp_path, gap, last_ts):
     assert rd("gaps") == oracle.timing_text(gaps, last_ts)
     # a plain PRO-style result (no R-side records) is unchanged by the side handling
     assert rd("records") == f"{nR + nS}\n{ns + nr}\n"
+
+
+def test_timing_files_both_sides_by_hand(aj, tmp_path):
+    """Four matches worked out by hand: S probes (payloadIDs 2, 0) at 2.1e6 and 6.3e6
+    cycles, R probes (payloadIDs 1, 0) at 4.2e6 and 8.4e6; S timestamps [0, 0, 1.05e6],
+    R timestamps [2.1e6, 0]; lastTS 0, every match sampled.  Times are cycles / 2.1e6 ms;
+    latency subtracts the probing tuple's own side's timestamp; the gap is its
+    payloadID minus its index among its side's samples."""
+    c = 2_100_000  # cycles per ms; clock_hz 2.1e9 gives one cycle per tick
+    s = [(1 * c, 2), (3 * c, 0)]
+    r = [(2 * c, 1), (4 * c, 0)]
+    timed = np.array([(t << 32) | i for t, i in s + r], np.uint64)
+    keep = []
+    res = _result(keep, 4, timed, 2.1e9, 2, [0.0] * 12, 1.0)
+    res.ntimed_r = 2
+    s_ts = np.array([0, 0, c // 2], np.uint64)
+    r_ts = np.array([c, 0], np.uint64)
+    pr = _lib.relation_payload_t(r_ts.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), 2)
+    ps = _lib.relation_payload_t(s_ts.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), 3)
+    relR = _lib.relation_t(None, 2, ctypes.addressof(pr))
+    relS = _lib.relation_t(None, 3, ctypes.addressof(ps))
+    _lib.check(_lib.lib.aj_write_timing_files(
+        ctypes.cast(ctypes.pointer(res), ctypes.POINTER(_lib.result_t)), ctypes.byref(relR),
+        ctypes.byref(relS), str(tmp_path).encode(), b"SHJ_st", 0, 0, 1, None))
+    rd = lambda d: [float(x) for x in (tmp_path / "results" / d / "SHJ_st_0.txt").read_text().split()]  # noqa: E731,E501
+    assert rd("timestamps") == [1.0, 2.0, 3.0, 4.0]
+    # S: 1 - 0.5 (ts of S row 2), 3 - 0 ; R: 2 - 0 (R row 1), 4 - 1 (R row 0)
+    assert rd("latency") == [0.5, 2.0, 3.0, 3.0]
+    # S: 2 - 0, 0 - 1 ; R: 1 - 0, 0 - 1
+    assert rd("gaps") == [-1.0, -1.0, 1.0, 2.0]
+    assert rd("records") == [5.0, 4.0]
