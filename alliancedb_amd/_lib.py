@@ -14,7 +14,9 @@ from typing import Optional
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libaj.so")
+# AJ_LIB names another build of the same library (the host-ASan build of the
+# CPU suite); by default the in-tree libaj.so
+LIB_PATH = os.environ.get("AJ_LIB") or os.path.join(_HERE, "libaj.so")
 
 # tuple_t of hashing/utils/types.h:67-71: {int32 payloadID @0, int32 key @4}
 TUPLE_DTYPE = np.dtype([("payloadID", "<i4"), ("key", "<i4")])
@@ -168,6 +170,8 @@ def _load() -> ctypes.CDLL:
         "SHJ_JBCR_NP": (c.POINTER(result_t), [c.POINTER(relation_t), c.POINTER(relation_t),
                                               c.c_int]),
         "SHJ_st": (c.POINTER(result_t), [c.POINTER(relation_t), c.POINTER(relation_t), c.c_int]),
+        "PRO_CPU": (c.POINTER(result_t), [c.POINTER(relation_t), c.POINTER(relation_t), c.c_int]),
+        "NPO_CPU": (c.POINTER(result_t), [c.POINTER(relation_t), c.POINTER(relation_t), c.c_int]),
         "aj_result_ext": (c.POINTER(aj_result_t), [c.POINTER(result_t)]),
         "aj_free_result": (None, [c.POINTER(result_t)]),
         "aj_write_timing_files": (c.c_int, [c.POINTER(result_t), c.POINTER(relation_t),
@@ -385,6 +389,16 @@ def SHJ_JBCR_NP(R: np.ndarray, S: np.ndarray, nthreads: int = 1) -> JoinResult:
 def SHJ_st(R: np.ndarray, S: np.ndarray, nthreads: int = 1) -> JoinResult:
     """Drop-in eager symmetric hash join (single-thread name); see aj.h."""
     return _dropin(lib.SHJ_st, R, S, nthreads)
+
+
+def PRO_CPU(R: np.ndarray, S: np.ndarray, nthreads: int = 0) -> JoinResult:
+    """The multithreaded CPU PRO baseline (host threads only; see aj.h)."""
+    return _dropin(lib.PRO_CPU, R, S, nthreads)
+
+
+def NPO_CPU(R: np.ndarray, S: np.ndarray, nthreads: int = 0) -> JoinResult:
+    """The multithreaded CPU NPO baseline (host threads only; see aj.h)."""
+    return _dropin(lib.NPO_CPU, R, S, nthreads)
 
 
 class Context:

@@ -3,7 +3,7 @@
 // benchmark(), hashing/main.cpp:256-296, 444-705; hashing/benchmark.cpp:164-257).
 //
 //   ajbench -a PRO|NPO|SHJ_st|SHJ_JM_NP|SHJ_JBCR_NP [-r |R|] [-s |S|] [-x r_seed] [-y s_seed]
-//           [-d key distribution: 0 unique/uniform, 2 zipf] [-z zipf theta]
+//           [-d key distribution: 0 unique/uniform, 2 zipf] [-z zipf theta] [--fk-zipf]
 //           [-w window -e step -l interval] [-n nthreads] [-m] [--repeats n]
 //           [--pairs] [--out FILE] [-R fileR -J keyR -L tsR] [-S fileS -K keyS -M tsS]
 //           [--timing EXP_DIR [--exp-id n] [--gap n]]
@@ -14,9 +14,13 @@
 //
 // Sizes: with -w/-e/-l, |R| = |S| = (window / interval) * step as in the
 // reference (benchmark.cpp:179-180); otherwise -r/-s (default 128M as in the
-// reference's help text, main.cpp:403-404).  R keys are a seeded permutation of
-// 1..|R|; S keys are uniform on 1..|R| (-d 0) or Zipf(theta) over R's keys
-// (-d 2).  Prints "Results = N" like benchmark.cpp:249, plus the checksum and
+// reference's help text, main.cpp:403-404).  -d 0: R keys are a seeded
+// permutation of 1..|R|, S keys uniform on 1..|R|.  -d 2, as in the reference
+// (createRelation -> create_relation_zipf(rel, n, n, skew), benchmark.cpp:103,
+// reached for R at :188 and for S at :204): the keys of each relation are drawn
+// Zipf(theta) over 1..its own size.  --fk-zipf keeps R a permutation and draws
+// S Zipf(theta) over R's keys (a skewed foreign-key join).  Prints "Results = N"
+// like benchmark.cpp:249, plus the checksum and
 // the device and end-to-end times.  -m materialises the reference's result
 // records (<S.key, S.payloadID> per match in chained per-thread buffers, as a
 // JOIN_RESULT_MATERIALIZE build does); --out FILE also writes them as the
@@ -72,6 +76,8 @@ const algo_t algos[] = {
     {"SHJ_st", SHJ_st},
     {"SHJ_JM_NP", SHJ_JM_NP},
     {"SHJ_JBCR_NP", SHJ_JBCR_NP},
+    {"PRO_CPU", PRO_CPU},  // the multithreaded CPU baseline (host threads: -n)
+    {"NPO_CPU", NPO_CPU},
     {nullptr, nullptr},
 };
 
@@ -90,10 +96,12 @@ struct params {
     int rkey = 0, skey = 0, rts = 0, sts = 0;
     const char *timing_dir = nullptr;
     int exp_id = 0, gap = 1;
+    bool fk_zipf = false;  // R unique, S Zipf over R's keys
 };
 
 void usage(const char *prog) {
-    printf("Usage: %s -a <PRO|NPO|SHJ_st|SHJ_JM_NP|SHJ_JBCR_NP> [-r R] [-s S] [-x seed] [-y seed] [-d 0|2] [-z theta]\n"
+    printf("Usage: %s -a <PRO|NPO|SHJ_st|SHJ_JM_NP|SHJ_JBCR_NP|PRO_CPU|NPO_CPU> [-r R] [-s S] [-x seed] [-y seed]\n"
+           "          [-d 0|2] [-z theta] [--fk-zipf]\n"
            "          [-w window -e step -l interval] [-n nthreads] [-m] [--repeats n]\n"
            "          [--pairs] [--out FILE] [-R fileR -J keyR -L tsR] [-S fileS -K keyS -M tsS]\n"
            "          [--timing EXP_DIR [--exp-id n] [--gap n]]\n"
@@ -106,7 +114,7 @@ void usage(const char *prog) {
 
 enum { OPT_LOAD = 1000, OPT_PEAK, OPT_SECONDS, OPT_STREAMS, OPT_BUDGET, OPT_SLOTS, OPT_BWIN,
        OPT_CLIENTS, OPT_VERIFY, OPT_MEDIAN, OPT_SIGMA, OPT_CAPT, OPT_DOMAIN, OPT_BTUP,
-       OPT_PAIRS, OPT_OUT, OPT_REPEATS, OPT_TIMING, OPT_EXPID, OPT_GAP };
+       OPT_PAIRS, OPT_OUT, OPT_REPEATS, OPT_TIMING, OPT_EXPID, OPT_GAP, OPT_FKZIPF };
 
 }  // namespace
 
@@ -147,6 +155,7 @@ int main(int argc, char **argv) {
                                 {"timing", required_argument, 0, OPT_TIMING},
                                 {"exp-id", required_argument, 0, OPT_EXPID},
                                 {"gap", required_argument, 0, OPT_GAP},
+                                {"fk-zipf", no_argument, 0, OPT_FKZIPF},
                                 {"loadfileR", required_argument, 0, 'R'},
                                 {"loadfileS", required_argument, 0, 'S'},
                                 {"rkey", required_argument, 0, 'J'},
@@ -204,6 +213,7 @@ int main(int argc, char **argv) {
             case OPT_TIMING: p.timing_dir = optarg; break;
             case OPT_EXPID: p.exp_id = atoi(optarg); break;
             case OPT_GAP: p.gap = atoi(optarg); break;
+            case OPT_FKZIPF: p.fk_zipf = true; break;
             case 'R': p.file_r = optarg; break;
             case 'S': p.file_s = optarg; break;
             case 'J': p.rkey = atoi(optarg); break;
@@ -240,15 +250,20 @@ int main(int argc, char **argv) {
     relation_payload_t plR = {nullptr, 0}, plS = {nullptr, 0};
     const uint32_t parts = (uint32_t)(p.nthreads > 0 ? p.nthreads : 1);
     int rc = AJ_OK;
+    const bool zipf_both = p.key_distribution == 2 && !p.fk_zipf;
     if (p.file_r)
         rc = aj_load_relation(&relR, &plR, p.rkey, p.rts, p.file_r, p.r_size, parts);
+    else if (zipf_both)
+        rc = aj_gen_zipf(R.data(), p.r_size, (uint32_t)(p.r_size ? p.r_size : 1), p.skew, p.r_seed);
     else
         rc = aj_gen_unique(R.data(), p.r_size, p.r_seed);
     if (rc == AJ_OK && p.file_s) {
         rc = aj_load_relation(&relS, &plS, p.skey, p.sts, p.file_s, p.s_size, parts);
     } else if (rc == AJ_OK) {
-        if (p.key_distribution == 2)
-            rc = aj_gen_zipf(S.data(), p.s_size, (uint32_t)p.r_size, p.skew, p.s_seed);
+        if (zipf_both)
+            rc = aj_gen_zipf(S.data(), p.s_size, (uint32_t)(p.s_size ? p.s_size : 1), p.skew, p.s_seed);
+        else if (p.key_distribution == 2)
+            rc = aj_gen_zipf(S.data(), p.s_size, (uint32_t)(p.r_size ? p.r_size : 1), p.skew, p.s_seed);
         else
             rc = aj_gen_uniform(S.data(), p.s_size, 1, (uint32_t)(p.r_size ? p.r_size : 1), p.s_seed);
     }

@@ -95,7 +95,7 @@ void DevBuf::release() {
 // ---------------------------------------------------------------- options
 static std::atomic<int64_t> opt_materialize{0}, opt_radix_bits{0}, opt_passes{0}, opt_device{0},
     opt_cap{0}, opt_split{0}, opt_kernel_timing{0}, opt_part_block{0}, opt_part_chunk{0},
-    opt_part_hist2d{1}, opt_tuned{1}, opt_timing{0};
+    opt_part_hist2d{1}, opt_tuned{1}, opt_timing{0}, opt_shj_batch{0};
 
 static thread_local bool g_force_timing = false;  // phase timing for one drop-in call
 bool kernel_timing_enabled() { return g_force_timing || opt_kernel_timing.load() != 0; }
@@ -399,6 +399,7 @@ static result_t *dropin(int algo, relation_t *relR, relation_t *relS, int nthrea
         aj_join_opts_t o;
         memset(&o, 0, sizeof(o));
         o.algo = algo;
+        if (algo == AJ_ALGO_SHJ) o.batch = (uint64_t)opt_shj_batch.load();  // 0 = default
         g_force_timing = timing;  // per-phase device times of the counting run
         rc = join_dev_locked(ctx, ctx->inR.as<uint64_t>(), nR, ctx->inS.as<uint64_t>(), nS, &o,
                              &out, ctx->stream);
@@ -598,7 +599,8 @@ int aj_set_option(const char *name, int64_t value) {
                {"part_chunk", &opt_part_chunk, 0, 1ll << 24},
                {"part_hist2d", &opt_part_hist2d, 0, 1},
                {"tuned", &opt_tuned, 0, 1},
-               {"timing", &opt_timing, 0, 1}};
+               {"timing", &opt_timing, 0, 1},
+               {"shj_batch", &opt_shj_batch, 0, 1ll << 31}};
     for (auto &e : tab) {
         if (strcmp(e.n, name) == 0) {
             if (value < e.lo || value > e.hi) {
@@ -644,6 +646,7 @@ int64_t aj_get_option(const char *name) {
     if (!strcmp(name, "part_hist2d")) return opt_part_hist2d.load();
     if (!strcmp(name, "tuned")) return opt_tuned.load();
     if (!strcmp(name, "timing")) return opt_timing.load();
+    if (!strcmp(name, "shj_batch")) return opt_shj_batch.load();
     if (!strcmp(name, "device_budget")) return (int64_t)g_dev_budget.load();
     if (!strcmp(name, "device_used")) return (int64_t)g_dev_used.load();
     return -1;

@@ -335,6 +335,12 @@ static void completer_main(Server *s) {
                 set_error(AJ_ERR_HIP, "batch failed: %s", hipGetErrorString(e));
                 status = AJ_ERR_HIP;
             }
+        } else {
+            // a launch that failed part-way may have queued the ranges copy and
+            // some kernels: they still read window memory and h_ranges, so wait
+            // for them before the windows' memory and the slot are reused
+            (void)hipStreamSynchronize(sl->st);
+            (void)hipGetLastError();
         }
         const double t = s->now_us();
         lk.lock();

@@ -107,6 +107,25 @@ result_t *SHJ_JM_NP(relation_t *relR, relation_t *relS, int nthreads);
 result_t *SHJ_JBCR_NP(relation_t *relR, relation_t *relS, int nthreads);
 result_t *SHJ_st(relation_t *relR, relation_t *relS, int nthreads);
 
+/*
+ * The multithreaded CPU joins with the same signature and result contract:
+ * the baseline a driver selects by name beside the GPU joins (the reference's
+ * algos[] registry, hashing/main.cpp:306-338, selects PRO and NPO the same
+ * way).  PRO_CPU restates the reference's PRO (parallel_radix_join.cpp:
+ * 981-1473): pass 1 with per-thread histograms, pass 2 per pass-1 partition,
+ * a bucket-chaining join per partition pair, radix bits and passes from the
+ * options "radix_bits" / "passes" (0 = the reference's defaults, 8 bits in 2
+ * passes, prj_params.h:16, 21).  NPO_CPU restates NPO
+ * (no_partitioning_join.cpp:156-476): one chained table of next_pow2(|R| / 2)
+ * buckets indexed by key & (buckets - 1), built and probed by all threads.
+ * They run on nthreads host threads (<= 0: the host's hardware threads), never
+ * modify their inputs, return count and checksum (aj_result_t.total_ms = wall
+ * time including thread start-up and temporaries), and ignore the
+ * materialize and timing options.  The GPU entry points never call them.
+ */
+result_t *PRO_CPU(relation_t *relR, relation_t *relS, int nthreads);
+result_t *NPO_CPU(relation_t *relR, relation_t *relS, int nthreads);
+
 /* Returns the extension of a result from PRO/NPO, or NULL if r is NULL. */
 const aj_result_t *aj_result_ext(const result_t *r);
 
@@ -158,6 +177,16 @@ int aj_write_result_file(const result_t *r, const char *path);
  *   "timing"       0/1   drop-in results also carry timed records and per-phase
  *                        device times for aj_write_timing_files (default 0)
  *   "device_budget" bytes of device memory the library may hold (0 = unlimited)
+ *   "shj_batch"    >=0   tuples per side per eager step of the SHJ drop-ins
+ *                        (0 = max(65536, max(|R|, |S|) / 256)).  The reference's
+ *                        shj alternates R[i], S[i] one tuple at a time
+ *                        (localjoiner.cpp:43-52); 1 gives exactly its order, so
+ *                        the records (materialize 2) and the timed records'
+ *                        R/S split are the reference's.  With a larger step a
+ *                        pair (r_i, s_j) of one step is recorded by its S tuple
+ *                        (the R part of a step is inserted first), where the
+ *                        reference records the later of the two; count and
+ *                        checksum never change.  At most 2^20 steps per call.
  */
 int aj_set_option(const char *name, int64_t value);
 int64_t aj_get_option(const char *name);

@@ -14,16 +14,34 @@
  * Read as a little-endian uint64, a tuple is (key << 32) | payloadID.
  * Keys and payloads are compared and combined as 32-bit bit patterns; the
  * signed declaration only matters to callers, never to join results.
+ *
+ * Next to the reference's own headers: the shared types (intkey_t through
+ * result_t) are declared only when the reference's hashing/utils/types.h has
+ * not been seen (its include guard is TYPES_H, types.h:11-12), so a
+ * translation unit that includes the reference's headers first gets the
+ * reference's declarations.  When this header comes first in C++ code built
+ * with the reference's hashing/ directory on the include path (detected by
+ * its utils/types.h and joins/parallel_radix_join.h), it includes the
+ * reference's utils/types.h itself, so a later include of it is a no-op.
+ * Define AJ_NO_REFERENCE_TYPES to turn that detection off.
  */
 #ifndef ALLIANCEDB_TYPES_H
 #define ALLIANCEDB_TYPES_H
 
 #include <stdint.h>
 
+#if defined(__cplusplus) && !defined(TYPES_H) && !defined(AJ_NO_REFERENCE_TYPES) && \
+    defined(__has_include)
+#if __has_include("utils/types.h") && __has_include("joins/parallel_radix_join.h")
+#include "utils/types.h"
+#endif
+#endif
+
 #ifdef __cplusplus
 extern "C" {
 #endif
 
+#ifndef TYPES_H
 typedef int32_t intkey_t;
 typedef int32_t value_t;
 
@@ -54,6 +72,7 @@ typedef struct result_t {
     threadresult_t *resultlist; /* NULL unless option "materialize" is 2 (see aj.h) */
     int nthreads;
 } result_t;
+#endif /* TYPES_H */
 
 /*
  * Chained result buffers, layout-identical to the reference's

@@ -17,7 +17,8 @@ from typing import Optional
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-ORACLE_PATH = os.path.join(_HERE, "liboracle.so")
+# AJ_ORACLE_LIB names another build of the oracle (the ASan build of the CPU suite)
+ORACLE_PATH = os.environ.get("AJ_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 REF_PATH = os.path.join(_HERE, "_ref", "libref.so")
 
 TUPLE_DTYPE = np.dtype([("payloadID", "<i4"), ("key", "<i4")])

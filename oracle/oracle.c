@@ -306,7 +306,7 @@ int oj_pro(const tuple_t *R, uint64_t nR, const tuple_t *S, uint64_t nS, int nth
         if (args[i].pb.oom) rc = -2;
         if (pairs && rc == 0) {
             if (np + args[i].pb.n > pairs_cap) rc = -3;
-            else memcpy(pairs + np, args[i].pb.v, args[i].pb.n * sizeof(uint64_t));
+            else if (args[i].pb.n) memcpy(pairs + np, args[i].pb.v, args[i].pb.n * sizeof(uint64_t));
             np += args[i].pb.n;
         }
         free(args[i].pb.v);
@@ -485,7 +485,7 @@ int oj_npo(const tuple_t *R, uint64_t nR, const tuple_t *S, uint64_t nS, int nth
         if (args[i].oom || args[i].pb.oom) rc = -2;
         if (pairs && rc == 0) {
             if (np + args[i].pb.n > pairs_cap) rc = -3;
-            else memcpy(pairs + np, args[i].pb.v, args[i].pb.n * sizeof(uint64_t));
+            else if (args[i].pb.n) memcpy(pairs + np, args[i].pb.v, args[i].pb.n * sizeof(uint64_t));
             np += args[i].pb.n;
         }
         free(args[i].pb.v);

@@ -267,20 +267,31 @@ def test_driver_persists_shj_records(aj, oracle, tmp_path):
     assert np.array_equal(np.sort(_parse_tbl(out)), oracle.shj_records(R, S, 65536))
 
 
+@pytest.mark.parametrize("mode", ["zipf_both", "fk_zipf"])
 @pytest.mark.parametrize("algo", ["PRO", "NPO"])
-def test_driver_runs_join(aj, oracle, algo):
-    """The C++ driver joins generated relations and prints the reference's result line."""
+def test_driver_runs_join(aj, oracle, algo, mode):
+    """The C++ driver joins generated relations and prints the reference's result line.
+
+    -d 2 draws both relations' keys Zipf over their own sizes, as the reference's
+    createRelation does for R and for S (benchmark.cpp:103, 188, 204); --fk-zipf
+    keeps R unique and draws S over R's keys."""
     import os
     import re
     import subprocess
     exe = os.path.join(os.path.dirname(aj.LIB_PATH), "ajbench")
+    extra = ["--fk-zipf"] if mode == "fk_zipf" else []
+    print(f"case driver {algo} {mode}", flush=True)
     r = subprocess.run([exe, "-a", algo, "-r", "200000", "-s", "300000", "-d", "2", "-z", "1.0",
-                        "--repeats", "2"], capture_output=True, text=True, timeout=120)
+                        "--repeats", "2", *extra], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     got = re.findall(r"Results = (\d+)\. checksum = (\d+)", r.stdout)
     assert len(got) == 2
-    R = aj.gen_unique(200000, 12345)
-    S = aj.gen_zipf(300000, 200000, 1.0, 54321)
+    if mode == "fk_zipf":
+        R = aj.gen_unique(200000, 12345)
+        S = aj.gen_zipf(300000, 200000, 1.0, 54321)
+    else:
+        R = aj.gen_zipf(200000, 200000, 1.0, 12345)
+        S = aj.gen_zipf(300000, 300000, 1.0, 54321)
     want = oracle.aggregate(R, S)
     for c, x in got:
         assert (int(c), int(x)) == want
@@ -371,3 +382,48 @@ def test_pro_option_invariance(aj, oracle, bits, passes, option, value):
         dR.free()
         dS.free()
     assert (out.count, out.checksum) == want
+
+
+def test_device_budget_failure_returns_null_and_nomem(aj):
+    """A join whose device memory does not fit the option device_budget fails cleanly:
+    the drop-in returns NULL with AJ_ERR_NOMEM (the reference exit()s when an
+    allocation fails, parallel_radix_join.cpp:58-65, common_functions.cpp:83-88), the
+    device API returns the status, and after the budget is lifted the same process
+    joins again.  Runs in a fresh process: the default context keeps the buffers
+    earlier tests grew."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = r"""
+import sys
+sys.path.insert(0, %r)
+import alliancedb_amd as aj
+n = 1 << 22
+R = aj.gen_unique(n, 1)
+S = aj.gen_uniform(n, 1, n, 2)
+d = aj.DeviceArray.from_host(R)  # 32 MiB held before the budget is set
+aj.set_option("device_budget", 16 << 20)
+for fn in (aj.PRO, aj.NPO):
+    try:
+        fn(R, S, 1)
+        print("NOFAIL", flush=True)
+    except aj.AjError as e:
+        print("STATUS", e.status, flush=True)
+with aj.Context(0) as ctx:
+    try:
+        ctx.join(d.ptr, n, d.ptr, n, aj.AJ_ALGO_NPO)
+        print("NOFAIL", flush=True)
+    except aj.AjError as e:
+        print("STATUS", e.status, flush=True)
+aj.set_option("device_budget", 0)
+d.free()
+r = aj.PRO(R, S, 1)
+print("OK", r.count, flush=True)
+""" % root
+    print("case device_budget 16 MiB, 2^22 tuples per side", flush=True)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = r.stdout.split("\n")
+    assert lines[:3] == [f"STATUS {aj.AJ_ERR_NOMEM}"] * 3, r.stdout
+    assert lines[3] == f"OK {1 << 22}"

@@ -148,6 +148,35 @@ def test_shj_pairs_and_dropin_results(aj, oracle, kind):
     assert np.array_equal(np.sort(got.records[0]), oracle.shj_records(R, S, 65536))
 
 
+@pytest.mark.parametrize("kind", ["dups", "all_ones"])
+def test_shj_dropin_step_one_gives_reference_records(aj, oracle, kind):
+    """Option shj_batch = 1: the drop-in SHJ_st alternates R[i], S[i] as the reference's
+    shj does (localjoiner.cpp:43-52), so its record lists (materialize 2) are the
+    reference's own records, and the timed records split into R and S probes as the
+    reference's recordR / recordS lists do."""
+    R, S = _shj_inputs(aj, kind)
+    want = oracle.shj_records(R, S, 1)
+    if oracle.ref_available():
+        c, ref = oracle.ref_shj_records(R, S)
+        assert c == len(want) and np.array_equal(ref, want)
+    by_side = oracle.shj_records_by_side(R, S, 1)
+    print(f"case SHJ drop-in shj_batch=1 {kind} matches={len(want)}", flush=True)
+    aj.set_option("shj_batch", 1)
+    aj.set_option("materialize", 2)
+    try:
+        got = aj.SHJ_st(R, S, 1)
+        aj.set_option("materialize", 0)
+        aj.set_option("timing", 1)
+        timed = aj.SHJ_JM_NP(R, S, 2)
+    finally:
+        aj.set_option("materialize", 0)
+        aj.set_option("timing", 0)
+        aj.set_option("shj_batch", 0)
+    assert (got.count, got.checksum) == oracle.aggregate(R, S)
+    assert np.array_equal(np.sort(got.records[0]), want)
+    assert timed.count == len(want) and timed.timed_r == len(by_side[0])
+
+
 def test_shj_empty_side(aj):
     R = aj.gen_unique(100, 1)
     E = np.empty(0, dtype=aj.TUPLE_DTYPE)
