@@ -5,7 +5,8 @@ include/alliancedb/aj.h; this package is its Python binding.
 """
 from ._lib import (  # noqa: F401
     AJ_ALGO_NPO, AJ_ALGO_PRO, AJ_ALGO_SHJ, AJ_ERR_ARG, AJ_ERR_CAPACITY, AJ_ERR_HIP, AJ_ERR_NODEV,
-    AJ_ERR_NOMEM, AJ_ERR_STATE, AJ_OK, LIB_PATH, NPO, NPO_CPU, PRO, PRO_CPU, SHJ_JBCR_NP,
+    AJ_ERR_NOMEM, AJ_ERR_STATE, AJ_OK, AJ_OUT_FALLBACK, AJ_OUT_STAGED, LIB_PATH, NPO, NPO_CPU,
+    PRO, PRO_CPU, SHJ_JBCR_NP,
     SHJ_JM_NP, SHJ_st,
     STREAM_DTYPE, TUPLE_DTYPE,
     AjError, BatchPlan, Context, DeviceArray, JoinResult, Server, as_tuples, check,

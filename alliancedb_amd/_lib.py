@@ -31,6 +31,9 @@ AJ_ERR_CAPACITY = -4
 AJ_ERR_STATE = -5
 AJ_ERR_NODEV = -6
 AJ_ALGO_PRO = 0
+# aj_join_out_t.flags (enum aj_out_flags)
+AJ_OUT_STAGED = 1
+AJ_OUT_FALLBACK = 2
 AJ_ALGO_NPO = 1
 AJ_ALGO_SHJ = 2
 
@@ -103,7 +106,8 @@ PHASE_NAMES = ["plan1", "hist1", "scan1", "scatter1", "plan2", "hist2", "scan2",
 class aj_join_out_t(ctypes.Structure):
     _fields_ = [("count", ctypes.c_uint64), ("checksum", ctypes.c_uint64),
                 ("npairs", ctypes.c_uint64), ("device_ms", ctypes.c_float),
-                ("phase_ms", ctypes.c_float * AJ_NPHASES), ("nrecords_r", ctypes.c_uint64)]
+                ("phase_ms", ctypes.c_float * AJ_NPHASES), ("nrecords_r", ctypes.c_uint64),
+                ("flags", ctypes.c_uint32)]
 
     def phases(self) -> dict:
         return {n: float(self.phase_ms[i]) for i, n in enumerate(PHASE_NAMES)}

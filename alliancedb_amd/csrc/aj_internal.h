@@ -58,6 +58,11 @@ struct ProConfig {
     int chunk = 0;        // tuples per partition work-group (0 = derived from sizes)
     int block = 512;      // lanes per partition work-group (256 or 512)
     bool p2d = true;      // two passes: pass 2 from the 2-D pass-1 histogram when it fits
+    bool staged = true;   // two passes of <= 8 bits each: staged partitioning first
+    int stage_sets = 0;   // staged: cursor sets (0 = default)
+    uint64_t stage_chunk = 0;  // staged: tuples per super-chunk (0 = default)
+    int stage_block = 0;  // staged: lanes per work-group (0 = default)
+    int stage_cache = -1; // staged: nontemporal access mode (-1 = default)
 };
 ProConfig choose_pro_config(uint64_t nR, uint64_t nS, const aj_join_opts_t *opts);
 // the autotuner's entry for the size bucket of (nR, nS), if any (tune.cpp)
@@ -75,8 +80,10 @@ struct Ctx {
     DevBuf hist, scan_tmp, plan, seg, tasks, acc, pairs, table, misc;
     DevBuf hist2d;              // per-chunk (d1, d2) counts of a two-pass PRO
     DevBuf table2, steps;       // SHJ: S-side table, per-step match counts
+    DevBuf stage;               // staged PRO: staging regions
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t evk[AJ_NPHASES + 1] = {};  // per-kernel timing (option kernel_timing)
+    std::vector<hipEvent_t> evs;          // per-launch timing of the staged passes
 };
 bool kernel_timing_enabled();
 // Eager symmetric hash join (shj.hip): steps of `batch` tuples per side.
@@ -165,6 +172,45 @@ struct P2Desc {
 constexpr int P2D_MAX_BITS = 15;  // b1 + b2 for the 2-D histogram path (128 KB of LDS)
 int run_pass2_2d(const P2Desc &d, hipStream_t st, const Marks *marks = nullptr, int m0 = 0);
 
+// Two-pass one-segment PRO partitioning streamed in super-chunks through a
+// staging buffer, output space reserved by atomics (staged.hip).
+struct StagedGeom {
+    int b1 = 0, b2 = 0;       // radix bits of pass 1 and pass 2 (each 1..8)
+    int nset = 1;             // staging cursor sets (work-group mod nset)
+    int block = 1024;         // lanes per work-group (512 or 1024)
+    int cache_mode = 9;       // nontemporal accesses: 1 input loads, 2 staging stores,
+                              // 4 staging loads, 8 final stores
+    uint64_t chunk = 0;       // tuples per super-chunk (a multiple of the tile)
+    uint64_t tiles_per_wg = 0;
+    uint32_t grid1 = 0;       // pass-1 work-groups of a full super-chunk
+    uint32_t units = 1;       // pass-2 work-groups per staging region
+    uint32_t scap = 0;        // tuples per staging region (nset * 2^b1 regions)
+    uint32_t fcap[2] = {0, 0};  // tuples per final partition of R and S
+    uint64_t stage_tuples = 0;  // staging buffer size (regions + a tile of trash)
+    uint64_t out_tuples[2] = {0, 0};
+};
+struct StagedDesc {
+    StagedGeom g;
+    const uint64_t *in[2];
+    uint64_t n[2];
+    uint64_t *stage;          // g.stage_tuples
+    uint64_t *out[2];         // g.out_tuples[r]: partition p at p * g.fcap[r]
+    uint32_t *scur;           // 2 * nset * 2^b1 staging cursors
+    uint32_t *fcur;           // 2 * 2^(b1+b2) final cursors (R, then S)
+    uint32_t *flag;           // set when a partition overflowed
+};
+// chunk 0 = 2^25 tuples
+int staged_geometry(StagedGeom *g, uint64_t nR, uint64_t nS, int b1, int b2, uint64_t chunk,
+                    int nset, int block, int num_cu);
+// events recorded by run_staged when timing: before and after every pass-1
+// launch (pass 2 runs between one super-chunk's "after" and the next "before")
+int staged_events(const StagedDesc &d);
+int run_staged(const StagedDesc &d, hipStream_t st, std::vector<hipEvent_t> *evs = nullptr);
+uint64_t staged_max_tasks(const StagedGeom &g, uint32_t split);
+// join task records and task_base[0] / task_base[max_parts] from the final cursors
+int staged_tasks(const StagedDesc &d, uint32_t split, uint4 *task_info, uint32_t *task_base,
+                 uint64_t max_parts, uint64_t max_tasks, hipStream_t st);
+
 // Join of partition pairs (R part_off, S part_off) into per-segment results.
 struct JoinDesc {
     const uint64_t *R, *S;          // partitioned tuples
@@ -191,6 +237,8 @@ struct JoinDesc {
     void *scan_tmp;
     size_t scan_tmp_bytes;
     int num_cu;
+    bool tasks_ready = false;       // task_info and task_base[0], task_base[max_parts]
+                                    // filled by the caller (no task planning)
 };
 // Raises a kernel's dynamic-LDS limit on the current device (thread-safe).
 int lds_limit_set(const void *fn, size_t bytes);

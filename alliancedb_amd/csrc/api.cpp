@@ -95,7 +95,8 @@ void DevBuf::release() {
 // ---------------------------------------------------------------- options
 static std::atomic<int64_t> opt_materialize{0}, opt_radix_bits{0}, opt_passes{0}, opt_device{0},
     opt_cap{0}, opt_split{0}, opt_kernel_timing{0}, opt_part_block{0}, opt_part_chunk{0},
-    opt_part_hist2d{1}, opt_tuned{1}, opt_timing{0}, opt_shj_batch{0};
+    opt_part_hist2d{1}, opt_tuned{1}, opt_timing{0}, opt_shj_batch{0}, opt_staged{1},
+    opt_stage_sets{0}, opt_stage_chunk{0}, opt_stage_block{0}, opt_stage_cache{-1};
 
 static thread_local bool g_force_timing = false;  // phase timing for one drop-in call
 bool kernel_timing_enabled() { return g_force_timing || opt_kernel_timing.load() != 0; }
@@ -127,8 +128,10 @@ static void ctx_fini(Ctx *c) {
     (void)hipSetDevice(c->device);
     DevBuf *bufs[] = {&c->inR, &c->inS, &c->p1R, &c->p1S, &c->p2R, &c->p2S, &c->hist,
                       &c->scan_tmp, &c->plan, &c->seg, &c->tasks, &c->acc, &c->pairs,
-                      &c->table, &c->misc, &c->hist2d, &c->table2, &c->steps};
+                      &c->table, &c->misc, &c->hist2d, &c->table2, &c->steps, &c->stage};
     for (DevBuf *b : bufs) b->release();
+    for (hipEvent_t e : c->evs) (void)hipEventDestroy(e);
+    c->evs.clear();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     for (hipEvent_t &e : c->evk)
@@ -179,6 +182,11 @@ static ProConfig config_from(uint64_t nR, uint64_t nS, const aj_join_opts_t *o) 
     if (opt_part_block.load()) c.block = (int)opt_part_block.load();
     if (opt_part_chunk.load()) c.chunk = (int)opt_part_chunk.load();
     c.p2d = opt_part_hist2d.load() != 0;
+    c.staged = opt_staged.load() != 0;
+    c.stage_sets = (int)opt_stage_sets.load();
+    c.stage_chunk = (uint64_t)opt_stage_chunk.load();
+    c.stage_block = (int)opt_stage_block.load();
+    c.stage_cache = (int)opt_stage_cache.load();
     return c;
 }
 
@@ -600,7 +608,12 @@ int aj_set_option(const char *name, int64_t value) {
                {"part_hist2d", &opt_part_hist2d, 0, 1},
                {"tuned", &opt_tuned, 0, 1},
                {"timing", &opt_timing, 0, 1},
-               {"shj_batch", &opt_shj_batch, 0, 1ll << 31}};
+               {"shj_batch", &opt_shj_batch, 0, 1ll << 31},
+               {"staged", &opt_staged, 0, 1},
+               {"stage_sets", &opt_stage_sets, 0, 64},
+               {"stage_chunk", &opt_stage_chunk, 0, 1ll << 31},
+               {"stage_block", &opt_stage_block, 0, 1024},
+               {"stage_cache", &opt_stage_cache, -1, 15}};
     for (auto &e : tab) {
         if (strcmp(e.n, name) == 0) {
             if (value < e.lo || value > e.hi) {
@@ -610,6 +623,10 @@ int aj_set_option(const char *name, int64_t value) {
             }
             if (e.v == &opt_part_block && value && value != 256 && value != 512) {
                 set_error(AJ_ERR_ARG, "option part_block must be 0, 256 or 512");
+                return AJ_ERR_ARG;
+            }
+            if (e.v == &opt_stage_block && value && value != 512 && value != 1024) {
+                set_error(AJ_ERR_ARG, "option stage_block must be 0, 512 or 1024");
                 return AJ_ERR_ARG;
             }
             if (e.v == &opt_part_chunk && value % 8192) {
@@ -647,6 +664,11 @@ int64_t aj_get_option(const char *name) {
     if (!strcmp(name, "tuned")) return opt_tuned.load();
     if (!strcmp(name, "timing")) return opt_timing.load();
     if (!strcmp(name, "shj_batch")) return opt_shj_batch.load();
+    if (!strcmp(name, "staged")) return opt_staged.load();
+    if (!strcmp(name, "stage_sets")) return opt_stage_sets.load();
+    if (!strcmp(name, "stage_chunk")) return opt_stage_chunk.load();
+    if (!strcmp(name, "stage_block")) return opt_stage_block.load();
+    if (!strcmp(name, "stage_cache")) return opt_stage_cache.load();
     if (!strcmp(name, "device_budget")) return (int64_t)g_dev_budget.load();
     if (!strcmp(name, "device_used")) return (int64_t)g_dev_used.load();
     return -1;

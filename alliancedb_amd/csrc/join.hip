@@ -583,17 +583,19 @@ int run_join(const JoinDesc &d, hipStream_t st, const Marks *mk, int m0) {
         set_error(AJ_ERR_ARG, "join workspace without task results");
         return AJ_ERR_ARG;
     }
-    const unsigned g = (unsigned)std::min<uint64_t>((d.max_parts + 256) / 256, 4096);
-    hipLaunchKernelGGL(k_task_count, dim3(g), dim3(256), 0, st, a);
-    size_t tmp = d.scan_tmp_bytes;
-    hipError_t e = hipcub::DeviceScan::ExclusiveSum(d.scan_tmp, tmp, d.task_cnt, d.task_base,
-                                                    (int)(d.max_parts + 1), st);
-    if (e != hipSuccess) {
-        set_error(AJ_ERR_HIP, "task scan failed: %s", hipGetErrorString(e));
-        return AJ_ERR_HIP;
+    if (!d.tasks_ready) {
+        const unsigned g = (unsigned)std::min<uint64_t>((d.max_parts + 256) / 256, 4096);
+        hipLaunchKernelGGL(k_task_count, dim3(g), dim3(256), 0, st, a);
+        size_t tmp = d.scan_tmp_bytes;
+        hipError_t e = hipcub::DeviceScan::ExclusiveSum(d.scan_tmp, tmp, d.task_cnt, d.task_base,
+                                                        (int)(d.max_parts + 1), st);
+        if (e != hipSuccess) {
+            set_error(AJ_ERR_HIP, "task scan failed: %s", hipGetErrorString(e));
+            return AJ_ERR_HIP;
+        }
+        const unsigned gt = (unsigned)std::min<uint64_t>((d.max_tasks + 255) / 256, 16384);
+        hipLaunchKernelGGL(k_task_fill, dim3(gt), dim3(256), 0, st, a);
     }
-    const unsigned gt = (unsigned)std::min<uint64_t>((d.max_tasks + 255) / 256, 16384);
-    hipLaunchKernelGGL(k_task_fill, dim3(gt), dim3(256), 0, st, a);
     if (mk) mk->mark(m0);
     const bool pairs = d.pairs != nullptr;
     int rc;

@@ -29,10 +29,10 @@
 
 #include "aj_internal.h"
 #include "device_common.h"
+#include "tiles.h"
 
 namespace aj {
 
-constexpr int PITEMS = 16;                // tuples per lane per tile
 constexpr int PTILE_MAX = 512 * PITEMS;   // largest tile (512-lane work-groups)
 constexpr int MAX_PASS_BITS = 13;
 
@@ -168,67 +168,6 @@ __device__ __forceinline__ bool locate_chunk(const PassArgs &a, uint64_t g, Chun
     return true;
 }
 
-// Tile of PTILE tuples starting at `tb` (tb even for 8-byte tuples): element
-// j of this thread is tuple idx[j]; v[j] says whether it lies in [lo, hi).
-// 8-byte tuples are read two per lane with 16-byte loads (consecutive lanes on
-// consecutive 16 bytes); a load never reaches past `hi`, so nothing outside
-// the segment's array is touched.  12-byte stream tuples are read one per lane.
-template <int PBLOCK, int NI = PITEMS>
-__device__ __forceinline__ void load_tile(const void *in, int stride, uint64_t tb, uint64_t lo,
-                                          uint64_t hi, uint64_t (&t)[NI],
-                                          bool (&v)[NI]) {
-    if (stride == 8) {
-        typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
-        const uint64_t *p = static_cast<const uint64_t *>(in);
-        if (tb >= lo && tb + PBLOCK * NI <= hi) {  // whole tile inside: unconditional 16-byte loads
-#pragma unroll
-            for (int j = 0; j < NI / 2; ++j) {
-                const u64x2 q = __builtin_nontemporal_load(
-                    reinterpret_cast<const u64x2 *>(p + tb + 2 * ((uint64_t)j * PBLOCK + threadIdx.x)));
-                t[2 * j] = q.x;
-                t[2 * j + 1] = q.y;
-                v[2 * j] = v[2 * j + 1] = true;
-            }
-            return;
-        }
-        // the lane index behind an empty asm statement: the per-item offsets
-        // below are recomputed here instead of being hoisted out of the tile
-        // loop and kept live (or spilled) across it
-        uint32_t lane = threadIdx.x;
-        asm volatile("" : "+v"(lane));
-#pragma unroll
-        for (int j = 0; j < NI / 2; ++j) {
-            const uint64_t i0 = tb + 2 * ((uint64_t)j * PBLOCK + lane);
-            uint64_t a = 0, b = 0;
-            if (i0 + 1 < hi) {
-                typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
-                const u64x2 q = __builtin_nontemporal_load(reinterpret_cast<const u64x2 *>(p + i0));
-                a = q.x;
-                b = q.y;
-            } else if (i0 < hi) {
-                a = p[i0];
-            }
-            t[2 * j] = a;
-            t[2 * j + 1] = b;
-            v[2 * j] = i0 >= lo && i0 < hi;
-            v[2 * j + 1] = i0 + 1 >= lo && i0 + 1 < hi;
-        }
-    } else {
-#pragma unroll
-        for (int j = 0; j < NI; ++j) {
-            const uint64_t i = tb + (uint64_t)j * PBLOCK + threadIdx.x;
-            v[j] = i >= lo && i < hi;
-            t[j] = v[j] ? load_tuple(in, stride, i) : 0;
-        }
-    }
-}
-
-// first tile start of a chunk: 8-byte tuples start on an even index so that
-// 16-byte loads are aligned (the element before `begin` is masked out)
-__device__ __forceinline__ uint64_t tile_start(uint64_t begin, int stride) {
-    return stride == 8 ? (begin & ~1ull) : begin;
-}
-
 // ---------------------------------------------------------------- k_hist
 template <int STRIDE, int PBLOCK>
 __global__ __launch_bounds__(PBLOCK) void k_hist(PassArgs a) {
@@ -322,12 +261,6 @@ __global__ __launch_bounds__(PBLOCK) void k_scatter(PassArgs a) {
 //             consecutive addresses within a digit's run; the next tile's loads
 //             are issued before this step, so they are in flight meanwhile
 constexpr int PIPE_MAX_BITS = 10;
-// Spare counters and slots for lanes without a tuple, one per lane mod 32: a
-// single spare would take the atomics and stores of up to 64 lanes of a wave
-// on one LDS address (serialised), which in a partly filled tile costs more
-// than the rest of the tile's LDS work.
-constexpr int NSPARE = 32;
-
 // LDS of the pipelined scatter: PTILE + NSPARE tuples, then F + NSPARE
 // counters, F write bases and F cursors.
 constexpr size_t pipe_lds_bytes(int pblock, uint32_t F) {

@@ -211,9 +211,170 @@ int device_clock_stamp(uint64_t *slot, hipStream_t st) {
     return AJ_OK;
 }
 
+__global__ void k_staged_setup(uint32_t *psb, uint32_t parts, uint64_t *scalars, int nscalars) {
+    if (threadIdx.x == 0) {
+        psb[0] = 0;
+        psb[1] = parts;
+    }
+    for (int i = threadIdx.x; i < nscalars; i += blockDim.x) scalars[i] = 0;
+}
+
+// Staged partitioning (staged.hip) is taken for two passes of at most 8 bits
+// each on inputs of at least 2^20 tuples in total.
+static bool staged_applies(uint64_t nR, uint64_t nS, const ProConfig &c) {
+    return c.staged && c.bits1 >= 1 && c.bits2 >= 1 && c.bits1 <= 8 && c.bits2 <= 8 &&
+           nR + nS >= (1ull << 20);
+}
+
+static float event_ms(hipEvent_t a, hipEvent_t b) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+        (void)hipGetLastError();
+        ms = 0;
+    }
+    return ms;
+}
+
+// PRO with staged partitioning.  *overflow is set (and out left unset) when a
+// partition did not fit its capacity; the caller then joins with the
+// histogram-based passes.
+static int pro_staged(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS,
+                      uint64_t nS, const ProConfig &cfg, uint64_t *pairs, uint64_t pairs_cap,
+                      int records, aj_join_out_t *out, hipStream_t st, bool *overflow) {
+    *overflow = false;
+    StagedDesc sd;
+    const int sets = cfg.stage_sets ? cfg.stage_sets : 4;
+    const int block = cfg.stage_block ? cfg.stage_block : 1024;
+    AJ_TRY(staged_geometry(&sd.g, nR, nS, cfg.bits1, cfg.bits2, cfg.stage_chunk, sets, block,
+                           ctx->num_cu));
+    if (cfg.stage_cache >= 0) sd.g.cache_mode = cfg.stage_cache;
+    const StagedGeom &g = sd.g;
+    const uint32_t F1 = 1u << g.b1, F = F1 << g.b2;
+    const uint32_t split = (uint32_t)cfg.split;
+    const uint64_t max_tasks = staged_max_tasks(g, split);
+    Layout L;
+    const size_t o_scur = L.add<uint32_t>(2ull * g.nset * F1);
+    const size_t o_fcur = L.add<uint32_t>(2ull * F);
+    const size_t o_tbase = L.add<uint32_t>((uint64_t)F + 1);
+    const size_t o_tinfo = L.add<uint4>(2 * max_tasks);
+    const size_t o_tres = L.add<ulonglong2>(max_tasks);
+    const size_t o_psb = L.add<uint32_t>(2);
+    const size_t o_scal = L.add<uint64_t>(8);
+    AJ_TRY(ctx->plan.ensure(L.off));
+    AJ_TRY(ctx->stage.ensure(g.stage_tuples * 8));
+    AJ_TRY(ctx->p2R.ensure(g.out_tuples[0] * 8));
+    AJ_TRY(ctx->p2S.ensure(g.out_tuples[1] * 8));
+    const DevBuf &B = ctx->plan;
+    uint64_t *scal = at<uint64_t>(B, o_scal);
+    uint32_t *psb = at<uint32_t>(B, o_psb);
+    sd.in[0] = dR;
+    sd.in[1] = dS;
+    sd.n[0] = nR;
+    sd.n[1] = nS;
+    sd.stage = ctx->stage.as<uint64_t>();
+    sd.out[0] = ctx->p2R.as<uint64_t>();
+    sd.out[1] = ctx->p2S.as<uint64_t>();
+    sd.scur = at<uint32_t>(B, o_scur);
+    sd.fcur = at<uint32_t>(B, o_fcur);
+    sd.flag = reinterpret_cast<uint32_t *>(scal + 4);
+    const bool timing = kernel_timing_enabled();
+    std::vector<hipEvent_t> *evs = nullptr;
+    Marks mk;
+    const Marks *mkp = nullptr;
+    if (timing) {
+        AJ_TRY(ensure_phase_events(ctx));
+        const size_t need = (size_t)staged_events(sd);
+        while (ctx->evs.size() < need) {
+            hipEvent_t e;
+            AJ_HIP(hipEventCreate(&e));
+            ctx->evs.push_back(e);
+        }
+        evs = &ctx->evs;
+        mk.ev = ctx->evk;
+        mk.st = st;
+        mkp = &mk;
+    }
+    AJ_HIP(hipEventRecord(ctx->ev0, st));
+    mk.mark(0);
+    hipLaunchKernelGGL(k_staged_setup, dim3(1), dim3(64), 0, st, psb, F, scal, 8);
+    AJ_TRY(run_staged(sd, st, evs));
+    for (int i = 1; i < 9; ++i) mk.mark(i);
+    AJ_TRY(staged_tasks(sd, split, at<uint4>(B, o_tinfo), at<uint32_t>(B, o_tbase), F, max_tasks,
+                        st));
+    JoinDesc j;
+    j.R = sd.out[0];
+    j.S = sd.out[1];
+    j.roff = j.soff = nullptr;
+    j.nparts_dev = nullptr;
+    j.max_parts = F;
+    j.part_seg_base = psb;
+    j.nseg = 1;
+    j.hash_shift = g.b1 + g.b2;
+    j.cap = cfg.cap;
+    j.split = split;
+    j.seg_count = scal + 0;
+    j.seg_sum = scal + 1;
+    j.pairs = pairs;
+    j.pairs_cap = pairs_cap;
+    j.records = records;
+    j.pair_cursor = scal + 2;
+    j.task_cnt = nullptr;
+    j.task_base = at<uint32_t>(B, o_tbase);
+    j.task_next = reinterpret_cast<uint32_t *>(scal + 3);
+    j.task_info = at<uint4>(B, o_tinfo);
+    j.task_res = at<ulonglong2>(B, o_tres);
+    j.max_tasks = max_tasks;
+    j.scan_tmp = nullptr;
+    j.scan_tmp_bytes = 0;
+    j.num_cu = ctx->num_cu;
+    j.tasks_ready = true;
+    AJ_TRY(run_join(j, st, mkp, 9));
+    AJ_HIP(hipEventRecord(ctx->ev1, st));
+    uint64_t host[5];
+    AJ_HIP(hipMemcpyAsync(host, scal, sizeof(host), hipMemcpyDeviceToHost, st));
+    AJ_HIP(hipStreamSynchronize(st));
+    const float ms = event_ms(ctx->ev0, ctx->ev1);
+    if ((uint32_t)host[4]) {
+        *overflow = true;
+        out->device_ms = ms;  // the attempt's time, added to the fallback's
+        return AJ_OK;
+    }
+    out->count = host[0];
+    out->checksum = host[1];
+    out->npairs = pairs ? host[2] : 0;
+    out->device_ms = ms;
+    out->flags = AJ_OUT_STAGED;
+    if (timing) {
+        read_phases(ctx, out, AJ_PH_PLAN1, AJ_PH_JOIN + 1);
+        const int ne = staged_events(sd);
+        float s1 = 0, s2 = 0;
+        for (int e = 0; e + 1 < ne; ++e) {
+            const float t = event_ms(ctx->evs[e], ctx->evs[e + 1]);
+            if (e & 1)
+                s2 += t;
+            else
+                s1 += t;
+        }
+        out->phase_ms[AJ_PH_SCATTER1] = s1;
+        out->phase_ms[AJ_PH_SCATTER2] = s2;
+        out->phase_ms[AJ_PH_PLAN1] = std::max(0.0f, out->phase_ms[AJ_PH_PLAN1] - s1 - s2);
+    }
+    return AJ_OK;
+}
+
 int pro_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, uint64_t nS,
                const ProConfig &cfg, uint64_t *pairs, uint64_t pairs_cap, int records,
                aj_join_out_t *out, hipStream_t st) {
+    float attempt_ms = 0;
+    if (staged_applies(nR, nS, cfg)) {
+        bool overflow = false;
+        AJ_TRY(pro_staged(ctx, dR, nR, dS, nS, cfg, pairs, pairs_cap, records, out, st,
+                          &overflow));
+        if (!overflow) return AJ_OK;
+        attempt_ms = out->device_ms;
+        memset(out, 0, sizeof(*out));
+        out->flags = AJ_OUT_FALLBACK;
+    }
     ProPlan p = plan_pro(nR, nS, cfg, ctx->num_cu);
     AJ_TRY(ctx->plan.ensure(p.bytes));
     AJ_TRY(ctx->scan_tmp.ensure(p.scan_bytes));
@@ -394,7 +555,7 @@ int pro_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
     out->count = host[0];
     out->checksum = host[1];
     out->npairs = pairs ? host[2] : 0;
-    out->device_ms = ms;
+    out->device_ms = ms + attempt_ms;
     if (mkp) read_phases(ctx, out, AJ_PH_PLAN1, AJ_PH_JOIN + 1);
     return AJ_OK;
 }

@@ -1,0 +1,447 @@
+// staged.hip -- two-pass PRO partitioning streamed through a staging buffer
+// the size of the memory-side cache, with output space reserved by atomics.
+//
+// The reference partitions in two passes (parallel_radix_partition for pass 1,
+// parallel_radix_join.cpp:739-806, then radix_cluster per pass-1 partition,
+// :573-614), each pass preceded by a histogram of its input that fixes every
+// thread's output offsets (:762-796).  Here both relations are cut into
+// super-chunks of `chunk` tuples, and each super-chunk goes through both
+// passes before the next one starts:
+//
+//   k_stage1  pass 1 of one super-chunk: work-groups take runs of consecutive
+//             tiles, counting-sort each tile by the pass-1 digit in LDS, and
+//             reserve room for each digit's run with one atomic add on that
+//             digit's staging cursor; the runs land in the staging regions
+//             (set, d1), where set = work-group mod nset spreads the atomics of
+//             one digit over nset cursors.  The staging buffer is reused by
+//             every super-chunk, so its lines stay in the memory-side cache
+//             between this kernel's writes and k_stage2's reads.
+//   k_stage2  pass 2 of the same super-chunk: each staging region is split
+//             into units; a unit's tiles are counting-sorted by the pass-2
+//             digit and each run reserves room in its final partition (d1, d2)
+//             with one atomic add on that partition's cursor.
+//   k_stage_tasks  join task records straight from the final cursors.
+//
+// No histogram is read: partitions are given fixed capacities (the expected
+// size plus max(1/8, 8 sigma) of it plus 64) and the cursors say how much of
+// each was filled.  A run that does not fit its region is written to a trash
+// area instead and sets the overflow flag; the caller then joins the input
+// again with the histogram-based passes, so the result never depends on the
+// capacities (they only decide which path computes it).  The order of the
+// runs within a partition depends on the order of the atomics; the join
+// results (count, checksum, pair multiset) do not.
+//
+// HBM traffic per tuple: 8 read (pass 1) + 8 written (pass 2); the staging
+// writes and reads go to the memory-side cache as far as it holds them.
+#include <algorithm>
+#include <cmath>
+
+#include "aj_internal.h"
+#include "device_common.h"
+#include "tiles.h"
+
+namespace aj {
+
+constexpr int STAGE_MAXF = 256;  // digits per pass (8 bits)
+
+// Where the runs of a tile go: region d starts at out index base + d * stride,
+// holds cap tuples and has filled *cur[d] of them.
+struct StageRes {
+    uint32_t *cur;
+    uint32_t base, stride, cap;
+    uint32_t trash;  // out index of a tile-sized area for runs that do not fit
+    uint32_t *flag;
+};
+
+// The pipelined tile loop of the partition scatter (four barriers per tile,
+// branch-free phases, the next tile's loads in flight during the write), with
+// the write bases taken from atomic reservations instead of running cursors.
+// Wave 0 scans the tile's F counters and issues one atomic add per non-empty
+// digit; the returned offsets are needed only for the write phase, so they are
+// turned into write bases after wave 0 has placed its own tuples.
+template <int PBLOCK, bool NT_LOAD, bool NT_STORE>
+__device__ __forceinline__ void stage_tiles(const uint64_t *in, uint64_t begin, uint64_t end,
+                                            uint64_t *out, int shift, uint32_t F, uint64_t *buf,
+                                            uint32_t *cnt, uint32_t *dst, uint32_t *s_len,
+                                            const StageRes &r) {
+    constexpr int PTILE = PBLOCK * PITEMS;
+    constexpr int MAXPER = STAGE_MAXF / 64;
+    constexpr int WB = 8;
+    static_assert(PITEMS % WB == 0, "a tile's items split into whole batches");
+    const uint32_t mask = F - 1;
+    uint64_t base = tile_start(begin, 8);
+    if (base >= end) return;
+    uint64_t t[PITEMS];
+    bool v[PITEMS];
+    load_tile<PBLOCK, PITEMS, NT_LOAD>(in, 8, base, begin, end, t, v);
+    for (;;) {
+        uint32_t rank2[PITEMS / 2];
+#pragma unroll
+        for (int j = 0; j < PITEMS; ++j) {
+            const uint32_t d = (key_of(t[j]) >> shift) & mask;
+            const uint32_t rk =
+                atomicAdd(&cnt[v[j] ? d : F + (threadIdx.x & (NSPARE - 1))], 1u) & 0xffffu;
+            if (j & 1)
+                rank2[j / 2] |= rk << 16;
+            else
+                rank2[j / 2] = rk;
+        }
+        __syncthreads();
+        uint32_t o[MAXPER], pk[MAXPER], ck[MAXPER];
+        const uint32_t lane = threadIdx.x;
+        const uint32_t per = (F + 63) >> 6;
+        const uint32_t b = lane * per;
+        if (threadIdx.x < 64) {
+            uint32_t sum = 0;
+#pragma unroll
+            for (int k = 0; k < MAXPER; ++k) {
+                ck[k] = ((uint32_t)k < per && b + k < F) ? cnt[b + k] : 0u;
+                sum += ck[k];
+            }
+            const uint32_t inc = wave_inclusive_scan(sum);
+            uint32_t pre = inc - sum;
+#pragma unroll
+            for (int k = 0; k < MAXPER; ++k) {
+                pk[k] = pre;
+                if ((uint32_t)k < per && b + k < F) cnt[b + k] = pre;
+                o[k] = ck[k] ? atomicAdd(&r.cur[b + k], ck[k]) : 0u;
+                pre += ck[k];
+            }
+            if (lane == 63) *s_len = inc;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j0 = 0; j0 < PITEMS; j0 += WB) {
+            uint32_t st[WB];
+#pragma unroll
+            for (int j = 0; j < WB; ++j) st[j] = cnt[(key_of(t[j0 + j]) >> shift) & mask];
+#pragma unroll
+            for (int j = 0; j < WB; ++j)
+                buf[v[j0 + j] ? st[j] + ((rank2[(j0 + j) / 2] >> (16 * ((j0 + j) & 1))) & 0xffffu)
+                              : (uint32_t)PTILE + (threadIdx.x & (NSPARE - 1))] = t[j0 + j];
+        }
+        if (threadIdx.x < 64) {
+#pragma unroll
+            for (int k = 0; k < MAXPER; ++k) {
+                if (ck[k]) {
+                    const bool fits = o[k] <= r.cap && ck[k] <= r.cap - o[k];
+                    if (!fits) atomicOr(r.flag, 1u);
+                    dst[b + k] = fits ? r.base + (b + k) * r.stride + o[k] - pk[k] : r.trash - pk[k];
+                }
+            }
+        }
+        __syncthreads();
+        const uint32_t len = *s_len;
+        const uint64_t nbase = base + PTILE;
+        const bool more = nbase < end;
+        if (more) load_tile<PBLOCK, PITEMS, NT_LOAD>(in, 8, nbase, begin, end, t, v);
+#pragma unroll
+        for (int k0 = 0; k0 < PITEMS; k0 += WB) {
+            uint64_t u[WB];
+            uint32_t w[WB];
+#pragma unroll
+            for (int k = 0; k < WB; ++k) {
+                const uint32_t i = threadIdx.x + (uint32_t)(k0 + k) * PBLOCK;
+                u[k] = buf[i < len ? i : (uint32_t)PTILE];
+            }
+#pragma unroll
+            for (int k = 0; k < WB; ++k) w[k] = dst[(key_of(u[k]) >> shift) & mask];
+#pragma unroll
+            for (int k = 0; k < WB; ++k) {
+                const uint32_t i = threadIdx.x + (uint32_t)(k0 + k) * PBLOCK;
+                if (i < len) {
+                    uint64_t *q = out + (uint64_t)(uint32_t)(w[k] + i);
+                    if constexpr (NT_STORE)
+                        __builtin_nontemporal_store(u[k], q);
+                    else
+                        *q = u[k];
+                }
+            }
+        }
+        for (uint32_t d = threadIdx.x; d < F; d += PBLOCK) cnt[d] = 0;
+        __syncthreads();
+        if (!more) break;
+        base = nbase;
+    }
+}
+
+constexpr size_t stage_lds_bytes(int pblock, uint32_t F) {
+    return sizeof(uint64_t) * ((size_t)pblock * PITEMS + NSPARE) +
+           sizeof(uint32_t) * (2 * (size_t)F + NSPARE);
+}
+
+struct Stage1Args {
+    const uint64_t *in;
+    uint64_t begin, end;    // rows of this super-chunk
+    uint64_t tiles_per_wg;  // consecutive tiles per work-group
+    int b1, nset;
+    uint64_t *stage;
+    uint32_t scap;          // tuples per staging region
+    uint32_t *scur;         // nset * F1 cursors of this super-chunk
+    uint32_t *flag;
+};
+
+// MODE bits: 1 = input loads nontemporal, 2 = staging stores nontemporal
+template <int PBLOCK, int MODE>
+__global__ __launch_bounds__(PBLOCK, 4) void k_stage1(Stage1Args a) {
+    constexpr int PTILE = PBLOCK * PITEMS;
+    extern __shared__ uint64_t smem[];
+    __shared__ uint32_t s_len;
+    const uint32_t F = 1u << a.b1;
+    const uint64_t b = a.begin + (uint64_t)blockIdx.x * a.tiles_per_wg * PTILE;
+    if (b >= a.end) return;
+    const uint64_t e = min(b + a.tiles_per_wg * PTILE, a.end);
+    uint64_t *buf = smem;                                                // PTILE + NSPARE
+    uint32_t *cnt = reinterpret_cast<uint32_t *>(buf + PTILE + NSPARE);  // F + NSPARE
+    uint32_t *dst = cnt + F + NSPARE;                                    // F
+    for (uint32_t d = threadIdx.x; d < F + NSPARE; d += PBLOCK) cnt[d] = 0;
+    __syncthreads();
+    const uint32_t set = blockIdx.x % (uint32_t)a.nset;
+    StageRes r;
+    r.cur = a.scur + set * F;
+    r.base = set * F * a.scap;
+    r.stride = a.scap;
+    r.cap = a.scap;
+    r.trash = (uint32_t)a.nset * F * a.scap;
+    r.flag = a.flag;
+    stage_tiles<PBLOCK, (MODE & 1) != 0, (MODE & 2) != 0>(a.in, b, e, a.stage, 0, F, buf, cnt, dst,
+                                                          &s_len, r);
+}
+
+struct Stage2Args {
+    const uint64_t *stage;
+    uint32_t scap;
+    const uint32_t *scur;  // cursors k_stage1 filled
+    uint32_t *scur_next;   // the other super-chunk's cursors: zeroed here
+    int b1, b2, nset;
+    uint32_t units;        // units per staging region
+    uint64_t *out;         // final partitions of this relation
+    uint32_t fcap;
+    uint32_t *fcur;        // F1 * F2 cursors of this relation
+    uint32_t *flag;
+};
+
+// MODE bits: 4 = staging loads nontemporal, 8 = final stores nontemporal
+template <int PBLOCK, int MODE>
+__global__ __launch_bounds__(PBLOCK, 4) void k_stage2(Stage2Args a) {
+    constexpr int PTILE = PBLOCK * PITEMS;
+    extern __shared__ uint64_t smem[];
+    __shared__ uint32_t s_len;
+    const uint32_t F1 = 1u << a.b1, F2 = 1u << a.b2;
+    const uint32_t nreg = (uint32_t)a.nset * F1;
+    if (blockIdx.x == 0)
+        for (uint32_t i = threadIdx.x; i < nreg; i += PBLOCK) a.scur_next[i] = 0;
+    const uint32_t region = blockIdx.x / a.units, j = blockIdx.x % a.units;
+    if (region >= nreg) return;
+    const uint32_t d1 = region & (F1 - 1);
+    const uint32_t n = min(a.scur[region], a.scap);
+    // units of whole tile pairs of 8-byte alignment: an even split point keeps
+    // the 16-byte loads aligned
+    const uint32_t per = ((n + a.units - 1) / a.units + 1) & ~1u;
+    const uint32_t ub = j * per, ue = min(ub + per, n);
+    if (ub >= ue) return;
+    uint64_t *buf = smem;
+    uint32_t *cnt = reinterpret_cast<uint32_t *>(buf + PTILE + NSPARE);
+    uint32_t *dst = cnt + F2 + NSPARE;
+    for (uint32_t d = threadIdx.x; d < F2 + NSPARE; d += PBLOCK) cnt[d] = 0;
+    __syncthreads();
+    StageRes r;
+    r.cur = a.fcur + d1 * F2;
+    r.base = d1 * F2 * a.fcap;
+    r.stride = a.fcap;
+    r.cap = a.fcap;
+    r.trash = F1 * F2 * a.fcap;
+    r.flag = a.flag;
+    stage_tiles<PBLOCK, (MODE & 4) != 0, (MODE & 8) != 0>(a.stage + (uint64_t)region * a.scap, ub,
+                                                          ue, a.out, a.b1, F2, buf, cnt, dst,
+                                                          &s_len, r);
+}
+
+// Join tasks of the staged layout: T tasks per final partition p (S slices of
+// `split`), task p * T + j = (R rows [p fcapR, + nR_p), S rows [p fcapS + j split,
+// + its slice)); a task with nothing to join has nR = nS = 0 and its S start
+// at the partition start (k_join then loads nothing it uses, and every address
+// it touches lies in the partition's region or the trash area after it).
+__global__ void k_stage_tasks(const uint32_t *fcur, uint32_t F, uint32_t fcapR, uint32_t fcapS,
+                              uint32_t split, uint32_t T, uint32_t sh, uint4 *task_info,
+                              uint32_t *task_base, uint64_t max_parts) {
+    const uint64_t total = (uint64_t)F * T;
+    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < total;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t p = (uint32_t)(t / T), j = (uint32_t)(t % T);
+        uint32_t nr = min(fcur[p], fcapR);
+        const uint32_t ns_all = min(fcur[F + p], fcapS);
+        const uint32_t s_off = j * split;
+        uint32_t ns = ns_all > s_off ? min(split, ns_all - s_off) : 0u;
+        if (!ns || !nr) ns = nr = 0;
+        const uint64_t r0 = (uint64_t)p * fcapR;
+        const uint64_t s0 = (uint64_t)p * fcapS + (ns ? s_off : 0u);
+        task_info[2 * t] = make_uint4((uint32_t)r0, (uint32_t)(r0 >> 32), nr, 0u);
+        task_info[2 * t + 1] = make_uint4((uint32_t)s0, (uint32_t)(s0 >> 32), ns, sh);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        task_base[0] = 0;
+        task_base[max_parts] = (uint32_t)total;
+    }
+}
+
+__global__ void k_stage_zero(uint32_t *p, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+// ---------------------------------------------------------------- host side
+static uint32_t region_cap(double expect) {
+    const double slack = std::max(expect / 8.0, 8.0 * std::sqrt(expect));
+    const uint64_t c = (uint64_t)std::ceil(expect + slack) + 64;
+    return (uint32_t)((c + 63) / 64 * 64);
+}
+
+int staged_geometry(StagedGeom *g, uint64_t nR, uint64_t nS, int b1, int b2, uint64_t chunk,
+                    int nset, int block, int num_cu) {
+    if (b1 < 1 || b2 < 1 || (1 << b1) > STAGE_MAXF || (1 << b2) > STAGE_MAXF ||
+        (block != 512 && block != 1024) || nset < 1 || nset > 64 || nR == 0 || nS == 0) {
+        set_error(AJ_ERR_ARG, "staged partitioning: bad geometry (%d + %d bits, block %d, %d sets)",
+                  b1, b2, block, nset);
+        return AJ_ERR_ARG;
+    }
+    const uint64_t ptile = (uint64_t)block * PITEMS;
+    g->b1 = b1;
+    g->b2 = b2;
+    g->nset = nset;
+    g->block = block;
+    const uint64_t nmax = std::max(nR, nS);
+    if (!chunk) chunk = 1ull << 25;
+    chunk = std::max<uint64_t>(ptile, chunk / ptile * ptile);
+    g->chunk = std::min<uint64_t>(chunk, (nmax + ptile - 1) / ptile * ptile);
+    const uint64_t tiles = g->chunk / ptile;
+    const int per_cu = block == 1024 ? 1 : 2;
+    const uint64_t g1 = std::min<uint64_t>(tiles, (uint64_t)num_cu * per_cu);
+    g->tiles_per_wg = (tiles + g1 - 1) / g1;
+    g->grid1 = (uint32_t)((tiles + g->tiles_per_wg - 1) / g->tiles_per_wg);
+    // the largest share of a super-chunk one cursor set can receive
+    const uint64_t wg_per_set = (g->grid1 + nset - 1) / nset;
+    const double set_tuples = (double)std::min<uint64_t>(g->chunk, wg_per_set * g->tiles_per_wg * ptile);
+    const uint32_t F1 = 1u << b1, F = F1 << b2;
+    g->scap = region_cap(set_tuples / F1);
+    g->fcap[0] = region_cap((double)nR / F);
+    g->fcap[1] = region_cap((double)nS / F);
+    const uint32_t nreg = (uint32_t)nset * F1;
+    g->units = (uint32_t)std::max<uint64_t>(1, ((uint64_t)num_cu * per_cu * 2 + nreg - 1) / nreg);
+    g->stage_tuples = (uint64_t)nreg * g->scap + ptile;
+    for (int r = 0; r < 2; ++r) g->out_tuples[r] = (uint64_t)F * g->fcap[r] + ptile;
+    if (g->stage_tuples >= (1ull << 32) || g->out_tuples[0] >= (1ull << 32) ||
+        g->out_tuples[1] >= (1ull << 32)) {
+        set_error(AJ_ERR_ARG, "staged partitioning: regions exceed 32-bit offsets");
+        return AJ_ERR_ARG;
+    }
+    return AJ_OK;
+}
+
+template <int PBLOCK, int MODE>
+static int launch_staged(const StagedDesc &d, hipStream_t st, std::vector<hipEvent_t> *evs) {
+    const StagedGeom &g = d.g;
+    const uint32_t F1 = 1u << g.b1, F2 = 1u << g.b2;
+    const size_t lds1 = stage_lds_bytes(PBLOCK, F1), lds2 = stage_lds_bytes(PBLOCK, F2);
+    AJ_TRY(lds_limit_set((const void *)k_stage1<PBLOCK, MODE>, lds1));
+    AJ_TRY(lds_limit_set((const void *)k_stage2<PBLOCK, MODE>, lds2));
+    const uint32_t nreg = (uint32_t)g.nset * F1;
+    int k = 0, ev = 0;
+    for (int rel = 0; rel < 2; ++rel) {
+        for (uint64_t b = 0; b < d.n[rel]; b += g.chunk, ++k) {
+            uint32_t *cur = d.scur + (uint64_t)(k & 1) * nreg;
+            uint32_t *next = d.scur + (uint64_t)((k + 1) & 1) * nreg;
+            Stage1Args a1;
+            a1.in = d.in[rel];
+            a1.begin = b;
+            a1.end = std::min(b + g.chunk, d.n[rel]);
+            a1.tiles_per_wg = g.tiles_per_wg;
+            a1.b1 = g.b1;
+            a1.nset = g.nset;
+            a1.stage = d.stage;
+            a1.scap = g.scap;
+            a1.scur = cur;
+            a1.flag = d.flag;
+            const uint64_t tiles = (a1.end - a1.begin + PBLOCK * PITEMS - 1) / (PBLOCK * PITEMS);
+            const unsigned grid1 = (unsigned)((tiles + g.tiles_per_wg - 1) / g.tiles_per_wg);
+            if (evs) (void)hipEventRecord((*evs)[ev++], st);
+            hipLaunchKernelGGL((k_stage1<PBLOCK, MODE>), dim3(grid1), dim3(PBLOCK), lds1, st, a1);
+            if (evs) (void)hipEventRecord((*evs)[ev++], st);
+            Stage2Args a2;
+            a2.stage = d.stage;
+            a2.scap = g.scap;
+            a2.scur = cur;
+            a2.scur_next = next;
+            a2.b1 = g.b1;
+            a2.b2 = g.b2;
+            a2.nset = g.nset;
+            a2.units = g.units;
+            a2.out = d.out[rel];
+            a2.fcap = g.fcap[rel];
+            a2.fcur = d.fcur + (uint64_t)rel * (F1 * F2);
+            a2.flag = d.flag;
+            hipLaunchKernelGGL((k_stage2<PBLOCK, MODE>), dim3(nreg * g.units), dim3(PBLOCK), lds2, st,
+                               a2);
+        }
+    }
+    if (evs) (void)hipEventRecord((*evs)[ev++], st);
+    AJ_HIP(hipGetLastError());
+    return AJ_OK;
+}
+
+int staged_events(const StagedDesc &d) {
+    int k = 0;
+    for (int rel = 0; rel < 2; ++rel) k += (int)((d.n[rel] + d.g.chunk - 1) / d.g.chunk);
+    return 2 * k + 1;
+}
+
+int run_staged(const StagedDesc &d, hipStream_t st, std::vector<hipEvent_t> *evs) {
+    const StagedGeom &g = d.g;
+    const uint32_t F1 = 1u << g.b1, F = F1 << g.b2;
+    if (evs && (int)evs->size() < staged_events(d)) {
+        set_error(AJ_ERR_ARG, "staged partitioning: too few timing events");
+        return AJ_ERR_ARG;
+    }
+    // both cursor sets of the staging regions, the final cursors and the flag
+    const uint64_t nz = 2ull * g.nset * F1;
+    hipLaunchKernelGGL(k_stage_zero, dim3(64), dim3(256), 0, st, d.scur, nz);
+    hipLaunchKernelGGL(k_stage_zero, dim3(64), dim3(256), 0, st, d.fcur, 2ull * F);
+    hipLaunchKernelGGL(k_stage_zero, dim3(1), dim3(64), 0, st, d.flag, (uint64_t)1);
+    switch (g.block * 16 + g.cache_mode) {
+#define AJ_STAGED_CASE(B, M) \
+    case B * 16 + M: return launch_staged<B, M>(d, st, evs);
+        AJ_STAGED_CASE(1024, 0) AJ_STAGED_CASE(1024, 1) AJ_STAGED_CASE(1024, 5)
+        AJ_STAGED_CASE(1024, 9) AJ_STAGED_CASE(1024, 13) AJ_STAGED_CASE(1024, 15)
+        AJ_STAGED_CASE(512, 1) AJ_STAGED_CASE(512, 5) AJ_STAGED_CASE(512, 9)
+#undef AJ_STAGED_CASE
+        default:
+            set_error(AJ_ERR_ARG, "staged partitioning: no kernels for block %d, cache mode %d",
+                      g.block, g.cache_mode);
+            return AJ_ERR_ARG;
+    }
+}
+
+int staged_tasks(const StagedDesc &d, uint32_t split, uint4 *task_info, uint32_t *task_base,
+                 uint64_t max_parts, uint64_t max_tasks, hipStream_t st) {
+    const StagedGeom &g = d.g;
+    const uint32_t F = 1u << (g.b1 + g.b2);
+    const uint32_t T = (g.fcap[1] + split - 1) / split;
+    if ((uint64_t)F * T > max_tasks || max_parts != F) {
+        set_error(AJ_ERR_ARG, "staged partitioning: task workspace too small");
+        return AJ_ERR_ARG;
+    }
+    const unsigned grid = (unsigned)std::min<uint64_t>(((uint64_t)F * T + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_stage_tasks, dim3(grid), dim3(256), 0, st, d.fcur, F, g.fcap[0],
+                       g.fcap[1], split, T, (uint32_t)(g.b1 + g.b2), task_info, task_base,
+                       max_parts);
+    AJ_HIP(hipGetLastError());
+    return AJ_OK;
+}
+
+uint64_t staged_max_tasks(const StagedGeom &g, uint32_t split) {
+    const uint64_t F = 1ull << (g.b1 + g.b2);
+    return F * ((g.fcap[1] + split - 1) / split);
+}
+
+}  // namespace aj

@@ -177,6 +177,20 @@ int aj_write_result_file(const result_t *r, const char *path);
  *   "timing"       0/1   drop-in results also carry timed records and per-phase
  *                        device times for aj_write_timing_files (default 0)
  *   "device_budget" bytes of device memory the library may hold (0 = unlimited)
+ *   "staged"       0/1   two-pass PRO with at most 8 bits per pass: partition both
+ *                        passes super-chunk by super-chunk through a staging buffer
+ *                        with atomically reserved, over-allocated partitions (no
+ *                        histogram pass); an input whose partitions overflow is
+ *                        detected on the device and joined again with the
+ *                        histogram passes (default 1)
+ *   "stage_sets"   0..64 staged: cursor sets of the staging regions (0 = 4)
+ *   "stage_chunk"  >=0   staged: tuples per super-chunk (0 = 2^25)
+ *   "stage_block"  0, 512, 1024  staged: lanes per work-group (0 = 1024)
+ *   "stage_cache"  -1..15 staged: nontemporal accesses, a sum of 1 (input loads),
+ *                        2 (staging stores), 4 (staging loads), 8 (partition
+ *                        stores); -1 = the default (9); modes 0, 1, 5, 9, 13, 15
+ *                        have kernels for 1024 lanes, 1, 5, 9 for 512, others are
+ *                        AJ_ERR_ARG
  *   "shj_batch"    >=0   tuples per side per eager step of the SHJ drop-ins
  *                        (0 = max(65536, max(|R|, |S|) / 256)).  The reference's
  *                        shj alternates R[i], S[i] one tuple at a time
@@ -318,7 +332,12 @@ typedef struct aj_join_out_t {
     float phase_ms[AJ_NPHASES];
     uint64_t nrecords_r; /* SHJ timed records (records = 2): R tuples' probes, written
                             from the end of the pair buffer backwards */
+    uint32_t flags;      /* PRO: AJ_OUT_STAGED when the staged partitioning produced
+                            the result, AJ_OUT_FALLBACK when a staged attempt overflowed
+                            and the histogram passes joined again (device_ms covers both) */
 } aj_join_out_t;
+
+enum aj_out_flags { AJ_OUT_STAGED = 1, AJ_OUT_FALLBACK = 2 };
 
 /*
  * Join device-resident relations (8-byte tuple_t arrays in HBM) on `stream`
