@@ -40,16 +40,22 @@ PRO_BYTES_PER_TUPLE = 32  # histogram read 8 + scatter read 8 + write 8 + build/
 NPO_BYTES_PER_TUPLE = 16  # R read 8 + slot write 8 / S read 8 + slot read 8
 SHJ_BYTES_PER_TUPLE = 24  # read 8 + own-table slot write 8 + other-table slot read 8
 WINDOW_BYTES_PER_TUPLE = 40  # 12 hist read + 12 scatter read + 8 write + 8 build/probe read
-# algorithmic bytes per input tuple of each kernel (8-byte tuples).  With the
-# default two-pass geometry pass 1's histogram kernel (k_hist2d) also counts the
+# algorithmic bytes per input tuple of each kernel (8-byte tuples), credited for
+# the tuples that kernel reads: NPO's build reads R only, its probe S only.  With
+# the histogram passes pass 1's histogram kernel (k_hist2d) also counts the
 # pass-2 digits, and the "hist2" phase is only the pass-2 unit planning.
 KERNEL_BYTES = {"hist1": 8, "scatter1": 16, "hist2": 0, "scatter2": 16, "join": 8,
                 "npo_build": 16, "npo_probe": 16, "shj_step": 24}
+KERNEL_SIDE = {"npo_build": "R", "npo_probe": "S"}  # others: |R| + |S|
 KERNEL_NAMES = {"hist1": "k_hist2d (pass 1)", "scatter1": "k_scatter_pipe (pass 1)",
                 "hist2": "pass-2 planning (k_p2_units, k_h2_usum, k_h2_base)",
                 "scatter2": "k_scatter_p2 (pass 2)",
                 "join": "k_join", "npo_build": "k_npo_build", "npo_probe": "k_npo_probe",
                 "shj_step": "k_shj_step (all eager steps)"}
+# the staged partitioning (aj_join_out_t.flags & AJ_OUT_STAGED): one pass-1 and one
+# pass-2 launch per super-chunk of STAGE_CHUNK tuples (the library's default)
+STAGED_NAMES = {"scatter1": "k_stage1 (pass 1)", "scatter2": "k_stage2 (pass 2)"}
+STAGE_CHUNK = 1 << 25
 
 
 def parse():
@@ -73,9 +79,12 @@ def parse():
     ap.add_argument("--passes", type=int, default=0)
     ap.add_argument("--cap", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-staged", action="store_true",
+                    help="PRO: histogram-based partition passes instead of the staged ones")
     ap.add_argument("--no-tune", action="store_true",
                     help="pro/skew: use the size heuristic instead of autotuning the geometry")
-    ap.add_argument("--cpu-log2n", type=int, default=24)
+    ap.add_argument("--cpu-log2n", type=int, default=27,
+                    help="SHJ's CPU baseline sample and the cap of the reference's RJ_st sample")
     ap.add_argument("--calib", action="store_true",
                     help="also run aj_copy_kernel over 1 GiB (traffic-counter calibration)")
     return ap.parse_args()
@@ -103,24 +112,64 @@ def cpu_baseline_shj(aj, log2n: int):
                       f"(input copies included)"}
 
 
-def cpu_baseline(aj, log2n: int):
-    """The CPU oracle's PRO (clean-room restatement of the reference's
-    multithreaded PRO) on a bounded sample of the same workload."""
+def host_info():
+    """The host the CPU baselines run on: model, sockets, nproc and the CPUs this
+    process may use (the box's share)."""
+    model, sockets = "unknown", set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name") and model == "unknown":
+                    model = line.split(":", 1)[1].strip()
+                elif line.startswith("physical id"):
+                    sockets.add(line.split(":", 1)[1].strip())
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"model": model, "sockets": max(len(sockets), 1), "nproc": os.cpu_count(),
+            "usable_cpus": usable}
+
+
+def cpu_pro_timing(R, S, threads: int, reps: int):
+    """The CPU oracle's PRO (clean-room restatement of the reference's multithreaded
+    PRO, 8 radix bits in 2 passes: its defaults) on host relations: median of `reps`
+    runs after one warm-up, temporaries included."""
     import oracle
-    n = 1 << log2n
+    times = []
+    for i in range(reps + 1):
+        r = oracle.pro(R, S, nthreads=threads, radix_bits=8, passes=2)
+        assert r.count == len(S)
+        if i:
+            times.append(r.total_ms)
+    return float(np.median(times))
+
+
+def cpu_baseline(aj, R, S, host):
+    """configs[1]'s CPU baseline: the oracle's PRO on the same 2^27 x 2^27 relations
+    the GPU joined, on the host CPUs this process may use (at most 16, the box's
+    share of its host)."""
+    threads = min(host["usable_cpus"] or 1, 16)
+    ms = cpu_pro_timing(R, S, threads, 3)
+    n = len(R) + len(S)
+    return {"value": round(n / (ms * 1e-3) / 1e6, 2), "unit": "M tuples/s", "cores": threads,
+            "kind": "port",
+            "sample": f"oracle PRO (the reference's multithreaded PRO restated) on the same "
+                      f"|R|=|S|=2^{len(R).bit_length() - 1} relations, {threads} threads, 8 radix "
+                      f"bits / 2 passes (reference defaults), median of 3 after a warm-up, "
+                      f"temporaries included; host {host['model']}, {host['sockets']} socket(s), "
+                      f"nproc {host['nproc']}, {host['usable_cpus']} CPUs usable"}
+
+
+def cpu_baseline_small(aj, host):
+    """2^20 x 2^20, 8 threads, median of 5 after a warm-up."""
+    n = 1 << 20
     R = aj.gen_unique(n, 12345)
     S = aj.gen_uniform(n, 1, n, 54321)
-    cores = min(os.cpu_count() or 1, 16)
-    times = []
-    for _ in range(3):
-        r = oracle.pro(R, S, nthreads=cores, radix_bits=8, passes=2)
-        assert r.count == n
-        times.append(r.total_ms)
-    ms = sorted(times)[len(times) // 2]
-    return {"value": round(2 * n / (ms * 1e-3) / 1e6, 2), "unit": "M tuples/s",
-            "cores": cores, "kind": "port",
-            "sample": f"PRO_CPU |R|=|S|=2^{log2n}, same key distribution, 8 radix bits / "
-                      f"2 passes (reference defaults), median of 3 (temporaries included)"}
+    ms = cpu_pro_timing(R, S, 8, 5)
+    return {"value": round(2 * n / (ms * 1e-3) / 1e6, 2), "unit": "M tuples/s", "cores": 8,
+            "kind": "port", "ms": round(ms, 3),
+            "sample": "oracle PRO |R|=|S|=2^20, R a permutation, S uniform, 8 threads, 8 radix "
+                      "bits / 2 passes, median of 5 after a warm-up"}
 
 
 def cpu_reference_pro(aj, log2n: int):
@@ -184,38 +233,59 @@ def autotune(aj, ctx, dR, nR, dS, nS, algo, kw, args):
 
 
 def timed_joins(aj, ctx, dR, nR, dS, nS, algo, kw, steps, warmup, want):
+    """W warm-up joins, then exactly K timed joins (host clock around them, a device
+    synchronisation on both sides), then K more joins with HIP events around every
+    kernel launch on the join stream for the per-kernel breakdown.  The events cost
+    about 4% of a join, so the timed region runs without them."""
     for _ in range(warmup):
         out = ctx.join(dR.ptr, nR, dS.ptr, nS, algo, **kw)
         assert (out.count, out.checksum) == want, (out.count, out.checksum, want)
     aj.device_sync()
     print("[bench] warmup done", file=sys.stderr, flush=True)
-    aj.set_option("kernel_timing", 1)
-    dev_ms, phases = [], []
+    dev_ms = []
     t0 = time.perf_counter()
     for _ in range(steps):
         out = ctx.join(dR.ptr, nR, dS.ptr, nS, algo, **kw)
         dev_ms.append(out.device_ms)
-        phases.append(out.phases())
         assert (out.count, out.checksum) == want
     aj.device_sync()
     wall = time.perf_counter() - t0
-    aj.set_option("kernel_timing", 0)
-    return out, float(np.mean(dev_ms)), wall * 1e3 / steps, phases
+    aj.set_option("kernel_timing", 1)
+    phases, ev_ms = [], []
+    try:
+        for _ in range(steps):
+            o2 = ctx.join(dR.ptr, nR, dS.ptr, nS, algo, **kw)
+            phases.append(o2.phases())
+            ev_ms.append(o2.device_ms)
+            assert (o2.count, o2.checksum) == want
+    finally:
+        aj.set_option("kernel_timing", 0)
+    return out, float(np.mean(dev_ms)), wall * 1e3 / steps, phases, float(np.mean(ev_ms))
 
 
-def join_line(aj, args, workload_name, nR, nS, algo, out, dev, ms_step, phases, metric):
+def join_line(aj, args, workload_name, nR, nS, algo, out, dev, ms_step, phases, metric,
+              ev_dev=None):
     tuples = nR + nS
     mean_ph = {k: float(np.mean([p[k] for p in phases])) for k in phases[0]}
     if algo == aj.AJ_ALGO_SHJ:  # one kernel type: the steps are the whole device time
         mean_ph = {"shj_step": dev}
     dom = max(KERNEL_BYTES, key=lambda k: mean_ph.get(k, 0.0))
-    achieved = KERNEL_BYTES[dom] * tuples / (mean_ph[dom] * 1e-3) / 1e9
+    side = KERNEL_SIDE.get(dom)
+    kt = nR if side == "R" else nS if side == "S" else tuples
+    achieved = KERNEL_BYTES[dom] * kt / (mean_ph[dom] * 1e-3) / 1e9
+    staged = bool(getattr(out, "flags", 0) & aj.AJ_OUT_STAGED)
+    kname = (STAGED_NAMES.get(dom) if staged else None) or KERNEL_NAMES[dom]
+    launches = 1
+    if staged and dom in STAGED_NAMES:
+        chunk = aj.get_option("stage_chunk") or STAGE_CHUNK
+        launches = -(-nR // chunk) + -(-nS // chunk)
     bpt = {aj.AJ_ALGO_PRO: PRO_BYTES_PER_TUPLE, aj.AJ_ALGO_NPO: NPO_BYTES_PER_TUPLE,
            aj.AJ_ALGO_SHJ: SHJ_BYTES_PER_TUPLE}[algo]
-    pipeline_gbs = bpt * tuples / (dev * 1e-3) / 1e9
+    whole_gbs = bpt * tuples / (ms_step * 1e-3) / 1e9
+    dev_gbs = bpt * tuples / (dev * 1e-3) / 1e9
     return {
         "metric": metric,
-        "value": round(tuples / (dev * 1e-3) / 1e6, 1),
+        "value": round(tuples / (ms_step * 1e-3) / 1e6, 1),
         "unit": "M tuples/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -229,12 +299,21 @@ def join_line(aj, args, workload_name, nR, nS, algo, out, dev, ms_step, phases, 
         "config": {"workload": workload_name, "n_r": nR, "n_s": nS},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "kernel": KERNEL_NAMES[dom],
-                     "bytes_per_tuple": KERNEL_BYTES[dom]},
+                     "traffic": None, "kernel": kname, "launches_per_join": launches,
+                     "bytes_per_tuple": KERNEL_BYTES[dom],
+                     "bytes_per_launch": KERNEL_BYTES[dom] * kt // launches,
+                     "mean_launch_ms": round(mean_ph[dom] / launches, 4),
+                     "whole_join": {"achieved": round(whole_gbs, 1),
+                                    "frac": round(whole_gbs / HBM_PEAK_GBS, 4),
+                                    "bytes_per_tuple": bpt, "ms": round(ms_step, 3)}},
         "pipeline": {"device_ms": round(dev, 3), "alg_bytes_per_tuple": bpt,
-                     "alg_GBps": round(pipeline_gbs, 1),
-                     "frac_of_peak": round(pipeline_gbs / HBM_PEAK_GBS, 4),
-                     "kernel_ms": {k: round(v, 4) for k, v in mean_ph.items() if v > 0}},
+                     "alg_GBps": round(dev_gbs, 1),
+                     "frac_of_peak": round(dev_gbs / HBM_PEAK_GBS, 4),
+                     "partitioning": "staged" if staged else "histogram passes",
+                     "kernel_ms": {k: round(v, 4) for k, v in mean_ph.items() if v > 0.0005},
+                     "kernel_ms_from": f"{args.steps} further joins with HIP events around "
+                                       f"every launch (device {ev_dev:.3f} ms per join with "
+                                       f"the events)" if ev_dev else None},
         "checksum": str(out.checksum),
     }
 
@@ -249,7 +328,6 @@ def run_pro(aj, args):
           flush=True)
     dR, dS = aj.DeviceArray.from_host(R), aj.DeviceArray.from_host(S)
     aj.device_sync()
-    del R, S
     if args.calib:
         dst = aj.DeviceArray(dR.nbytes)
         aj.check(aj.lib.aj_copy_d2d(dst.ptr, dR.ptr, dR.nbytes))
@@ -260,30 +338,35 @@ def run_pro(aj, args):
         kw = dict(batch=args.batch)
     with aj.Context(0) as ctx:
         tuned = autotune(aj, ctx, dR, n, dS, n, algo, kw, args)
-        out, dev, ms_step, phases = timed_joins(aj, ctx, dR, n, dS, n, algo, kw, args.steps,
-                                                args.warmup, want)
+        out, dev, ms_step, phases, ev_dev = timed_joins(aj, ctx, dR, n, dS, n, algo, kw,
+                                                        args.steps, args.warmup, want)
+    dR.free()
+    dS.free()
     name = (f"{args.algo} |R|=|S|=2^{args.log2n} 8-byte tuples, "
             f"unique R keys x uniform S keys")
     line = join_line(aj, args, name, n, n, algo, out, dev, ms_step, phases,
-                     "join throughput M tuples/s (|R|+|S|)")
+                     "join throughput M tuples/s (|R|+|S|)", ev_dev)
     if tuned:
-        line["config"]["geometry"] = {k: tuned[k] for k in ("radix_bits", "passes", "cap",
-                                                            "part_block")}
+        line["config"]["geometry"] = {k: tuned[k] for k in ("radix_bits", "passes", "cap")}
         line["config"]["geometry"]["chosen_by"] = "autotuner (aj_tune_pro)"
-    # the traffic summary covers the partition geometry (14 bits in 7 + 7, 512-lane
-    # pass 2); the LDS round size only changes the join kernel
+    # the traffic summary covers the default geometry (14 bits in 7 + 7)
     default_geom = algo == aj.AJ_ALGO_PRO and not (args.radix_bits or args.passes) and (
-        not tuned or (tuned["radix_bits"], tuned["passes"], tuned["part_block"]) == (14, 2, 512))
+        not tuned or (tuned["radix_bits"], tuned["passes"]) == (14, 2))
     if default_geom:
         line["roofline"]["traffic"] = measured_traffic(line["roofline"]["kernel"], name)
     if algo == aj.AJ_ALGO_SHJ:
         line["metric"] = "eager join throughput M tuples/s (|R|+|S|)"
         line["config"]["batch"] = args.batch or max(65536, -(-n // 256))
     if not args.no_cpu:
-        line["cpu_baseline"] = (cpu_baseline_shj(aj, args.cpu_log2n) if algo == aj.AJ_ALGO_SHJ
-                                else cpu_baseline(aj, args.cpu_log2n))
+        host = host_info()
+        if algo == aj.AJ_ALGO_SHJ:
+            line["cpu_baseline"] = cpu_baseline_shj(aj, min(args.cpu_log2n, 23))
+        else:
+            line["cpu_baseline"] = cpu_baseline(aj, R, S, host)
+            line["cpu_baseline_2e20_8threads"] = cpu_baseline_small(aj, host)
+        line["host"] = host
         if algo == aj.AJ_ALGO_PRO:
-            ref = cpu_reference_pro(aj, min(args.cpu_log2n, 22))
+            ref = cpu_reference_pro(aj, min(args.cpu_log2n, 25))
             if ref:
                 line["cpu_reference"] = ref
     return line
@@ -303,12 +386,12 @@ def run_skew(aj, args):
     kw = dict(radix_bits=args.radix_bits, passes=args.passes, cap=args.cap)
     with aj.Context(0) as ctx:
         tuned = autotune(aj, ctx, dR, nR, dS, nS, algo, kw, args)
-        out, dev, ms_step, phases = timed_joins(aj, ctx, dR, nR, dS, nS, algo, kw, args.steps,
-                                                args.warmup, want)
+        out, dev, ms_step, phases, ev_dev = timed_joins(aj, ctx, dR, nR, dS, nS, algo, kw,
+                                                        args.steps, args.warmup, want)
     name = (f"{args.algo} skewed foreign-key join |R|=2^24 unique x |S|=2^28 "
             f"Zipf(theta={args.theta})")
     line = join_line(aj, args, name, nR, nS, algo, out, dev, ms_step, phases,
-                     "join throughput M tuples/s (|R|+|S|)")
+                     "join throughput M tuples/s (|R|+|S|)", ev_dev)
     if tuned:
         line["config"]["geometry"] = {k: tuned[k] for k in ("radix_bits", "passes", "cap",
                                                             "part_block")}
@@ -461,6 +544,8 @@ def main():
               f"(this path does not shard across GPUs)", file=sys.stderr)
         sys.exit(2)
     import alliancedb_amd as aj
+    if args.no_staged:
+        aj.set_option("staged", 0)
     if args.workload == "pro":
         line = run_pro(aj, args)
     elif args.workload == "skew":
