@@ -192,6 +192,9 @@ def _load() -> ctypes.CDLL:
         "aj_ctx_destroy": (c.c_int, [P]),
         "aj_join_dev": (c.c_int, [P, P, c.c_uint64, P, c.c_uint64, c.POINTER(aj_join_opts_t),
                                   c.POINTER(aj_join_out_t), P]),
+        "aj_join_dev_noalloc": (c.c_int, [P, c.c_uint64, P, c.c_uint64,
+                                          c.POINTER(aj_join_opts_t), c.POINTER(aj_join_out_t),
+                                          P]),
         "aj_device_alloc": (c.c_int, [c.POINTER(P), c.c_uint64]),
         "aj_device_free": (c.c_int, [P]),
         "aj_copy_h2d": (c.c_int, [P, P, c.c_uint64]),

@@ -35,11 +35,21 @@ int last_status();
     } while (0)
 
 // ---------------------------------------------------------------- memory
+// Device memory handed out by bumping a cursor through one fixed region (the
+// no-allocation entry point's scratch array); reset before each call.
+struct ScratchArena {
+    uint8_t *base = nullptr;  // device address of the region
+    size_t size = 0, used = 0;
+};
+
 // A device buffer that only grows; reused across calls so that repeated joins
-// of the same size do not allocate.
+// of the same size do not allocate.  With an arena, ensure() carves the buffer
+// out of the arena instead (AJ_ERR_CAPACITY when it does not fit) and release()
+// only forgets it.
 struct DevBuf {
     void *p = nullptr;
     size_t cap = 0;
+    ScratchArena *arena = nullptr;
     int ensure(size_t bytes);
     void release();
     template <class T> T *as() const { return static_cast<T *>(p); }
@@ -81,11 +91,30 @@ struct Ctx {
     DevBuf hist2d;              // per-chunk (d1, d2) counts of a two-pass PRO
     DevBuf table2, steps;       // SHJ: S-side table, per-step match counts
     DevBuf stage;               // staged PRO: staging regions
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;  // none: no timing (device_ms 0)
+    bool no_events = false;     // never create events (no per-kernel timing either)
     hipEvent_t evk[AJ_NPHASES + 1] = {};  // per-kernel timing (option kernel_timing)
     std::vector<hipEvent_t> evs;          // per-launch timing of the staged passes
 };
 bool kernel_timing_enabled();
+// Events of a context that may have none: recording is skipped, times are 0.
+inline hipError_t record_event(hipEvent_t e, hipStream_t st) {
+    return e ? hipEventRecord(e, st) : hipSuccess;
+}
+inline float elapsed_ms(hipEvent_t a, hipEvent_t b) {
+    float ms = 0;
+    if (!a || !b) return 0;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return ms;
+}
+inline bool phase_timing(const Ctx *c);
+// The join behind aj_join_dev on a context the caller has locked and whose
+// device is current.
+int join_dev(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, uint64_t nS,
+             const aj_join_opts_t *opts, aj_join_out_t *out, hipStream_t st);
 // Eager symmetric hash join (shj.hip): steps of `batch` tuples per side.
 uint64_t shj_steps(uint64_t nR, uint64_t nS, uint64_t batch);
 int shj_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, uint64_t nS,
@@ -102,6 +131,7 @@ struct Marks {
         if (ev) (void)hipEventRecord(ev[i], st);
     }
 };
+inline bool phase_timing(const Ctx *c) { return kernel_timing_enabled() && !c->no_events; }
 Ctx *default_ctx(int *status);
 Ctx *ctx_of(aj_ctx_t *c);
 

@@ -65,6 +65,20 @@ void dev_free(void *p) {
 
 int DevBuf::ensure(size_t bytes) {
     if (bytes <= cap && p) return AJ_OK;
+    if (arena) {
+        const size_t o = (arena->used + 255) / 256 * 256;
+        if (o > arena->size || bytes > arena->size - o) {
+            set_error(AJ_ERR_CAPACITY,
+                      "the no-allocation scratch of %zu bytes does not hold this join's "
+                      "workspace (%zu more bytes needed at offset %zu)",
+                      arena->size, bytes, o);
+            return AJ_ERR_CAPACITY;
+        }
+        p = arena->base + o;
+        cap = bytes;
+        arena->used = o + bytes;
+        return AJ_OK;
+    }
     if (p) {
         dev_free(p);
         g_dev_used -= cap;
@@ -84,6 +98,11 @@ int DevBuf::ensure(size_t bytes) {
 }
 
 void DevBuf::release() {
+    if (arena) {
+        p = nullptr;
+        cap = 0;
+        return;
+    }
     if (p) {
         dev_free(p);
         g_dev_used -= cap;
@@ -493,6 +512,10 @@ struct aj_ctx {
 
 namespace aj {
 Ctx *ctx_of(aj_ctx_t *c) { return &c->c; }
+int join_dev(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, uint64_t nS,
+             const aj_join_opts_t *opts, aj_join_out_t *out, hipStream_t st) {
+    return join_dev_locked(ctx, dR, nR, dS, nS, opts, out, st);
+}
 }  // namespace aj
 
 extern "C" {

@@ -277,7 +277,7 @@ static int pro_staged(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t 
     sd.scur = at<uint32_t>(B, o_scur);
     sd.fcur = at<uint32_t>(B, o_fcur);
     sd.flag = reinterpret_cast<uint32_t *>(scal + 4);
-    const bool timing = kernel_timing_enabled();
+    const bool timing = phase_timing(ctx);
     std::vector<hipEvent_t> *evs = nullptr;
     Marks mk;
     const Marks *mkp = nullptr;
@@ -294,7 +294,7 @@ static int pro_staged(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t 
         mk.st = st;
         mkp = &mk;
     }
-    AJ_HIP(hipEventRecord(ctx->ev0, st));
+    AJ_HIP(record_event(ctx->ev0, st));
     mk.mark(0);
     hipLaunchKernelGGL(k_staged_setup, dim3(1), dim3(64), 0, st, psb, F, scal, 8);
     AJ_TRY(run_staged(sd, st, evs));
@@ -329,11 +329,11 @@ static int pro_staged(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t 
     j.num_cu = ctx->num_cu;
     j.tasks_ready = true;
     AJ_TRY(run_join(j, st, mkp, 9));
-    AJ_HIP(hipEventRecord(ctx->ev1, st));
+    AJ_HIP(record_event(ctx->ev1, st));
     uint64_t host[5];
     AJ_HIP(hipMemcpyAsync(host, scal, sizeof(host), hipMemcpyDeviceToHost, st));
     AJ_HIP(hipStreamSynchronize(st));
-    const float ms = event_ms(ctx->ev0, ctx->ev1);
+    const float ms = elapsed_ms(ctx->ev0, ctx->ev1);
     if ((uint32_t)host[4]) {
         *overflow = true;
         out->device_ms = ms;  // the attempt's time, added to the fallback's
@@ -393,10 +393,10 @@ int pro_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
     uint64_t *scal = at<uint64_t>(B, p.o_scal);
     uint32_t *psb = at<uint32_t>(B, p.o_psb);
 
-    AJ_HIP(hipEventRecord(ctx->ev0, st));
+    AJ_HIP(record_event(ctx->ev0, st));
     Marks mk;
     const Marks *mkp = nullptr;
-    if (kernel_timing_enabled()) {
+    if (phase_timing(ctx)) {
         AJ_TRY(ensure_phase_events(ctx));
         mk.ev = ctx->evk;
         mk.st = st;
@@ -546,12 +546,11 @@ int pro_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
     j.scan_tmp_bytes = p.scan_bytes;
     j.num_cu = ctx->num_cu;
     AJ_TRY(run_join(j, st, mkp, 9));
-    AJ_HIP(hipEventRecord(ctx->ev1, st));
+    AJ_HIP(record_event(ctx->ev1, st));
     uint64_t host[3];
     AJ_HIP(hipMemcpyAsync(host, scal, sizeof(host), hipMemcpyDeviceToHost, st));
     AJ_HIP(hipStreamSynchronize(st));
-    float ms = 0;
-    AJ_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    const float ms = elapsed_ms(ctx->ev0, ctx->ev1);
     out->count = host[0];
     out->checksum = host[1];
     out->npairs = pairs ? host[2] : 0;
@@ -568,7 +567,7 @@ int npo_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
     AJ_TRY(ctx->table.ensure(cap * 8));
     AJ_TRY(ctx->acc.ensure(8 * npo_acc_words(ctx->num_cu)));
     uint64_t *acc = ctx->acc.as<uint64_t>();
-    AJ_HIP(hipEventRecord(ctx->ev0, st));
+    AJ_HIP(record_event(ctx->ev0, st));
     AJ_TRY(fill_u64(acc, 8, 0, st));
     NpoDesc d;
     d.R = dR;
@@ -584,19 +583,18 @@ int npo_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
     d.num_cu = ctx->num_cu;
     Marks mk;
     const Marks *mkp = nullptr;
-    if (kernel_timing_enabled()) {
+    if (phase_timing(ctx)) {
         AJ_TRY(ensure_phase_events(ctx));
         mk.ev = ctx->evk;
         mk.st = st;
         mkp = &mk;
     }
     AJ_TRY(run_npo(d, st, mkp, AJ_PH_NPO_BUILD));
-    AJ_HIP(hipEventRecord(ctx->ev1, st));
+    AJ_HIP(record_event(ctx->ev1, st));
     uint64_t host[4];
     AJ_HIP(hipMemcpyAsync(host, acc, sizeof(host), hipMemcpyDeviceToHost, st));
     AJ_HIP(hipStreamSynchronize(st));
-    float ms = 0;
-    AJ_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    const float ms = elapsed_ms(ctx->ev0, ctx->ev1);
     out->count = host[0];
     out->checksum = host[1];
     out->npairs = pairs ? host[3] : 0;

@@ -258,7 +258,7 @@ int shj_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
     uint64_t *scnt = ctx->steps.as<uint64_t>();  // [steps][G] per-block step matches
     uint64_t *sum_part = scnt + steps * G;       // [G] per-block checksums
     uint64_t *prog = sum_part + G;               // [steps] cumulative matches
-    AJ_HIP(hipEventRecord(ctx->ev0, st));
+    AJ_HIP(record_event(ctx->ev0, st));
     AJ_TRY(fill_u64(acc, 8, 0, st));
     AJ_TRY(fill_u64(tr, capR, EMPTY_SLOT, st));
     AJ_TRY(fill_u64(ts, capS, EMPTY_SLOT, st));
@@ -295,14 +295,13 @@ int shj_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
     hipLaunchKernelGGL(k_shj_finish, dim3(1), dim3(SBLOCK), 0, st, steps, G,
                        (const uint64_t *)sum_part, prog, acc);
     AJ_HIP(hipGetLastError());
-    AJ_HIP(hipEventRecord(ctx->ev1, st));
+    AJ_HIP(record_event(ctx->ev1, st));
     uint64_t host[6];
     AJ_HIP(hipMemcpyAsync(host, acc, sizeof(host), hipMemcpyDeviceToHost, st));
     if (progress)
         AJ_HIP(hipMemcpyAsync(progress, prog, 8 * steps, hipMemcpyDefault, st));
     AJ_HIP(hipStreamSynchronize(st));
-    float ms = 0;
-    AJ_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    const float ms = elapsed_ms(ctx->ev0, ctx->ev1);
     out->count = host[0];
     out->checksum = host[1];
     out->npairs = mode ? host[4] + host[5] : 0;

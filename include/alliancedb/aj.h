@@ -349,6 +349,31 @@ int aj_join_dev(aj_ctx_t *ctx, const tuple_t *dR, uint64_t nR, const tuple_t *dS
                 uint64_t nS, const aj_join_opts_t *opts, aj_join_out_t *out,
                 void *stream);
 
+/*
+ * aj_join_dev without any device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (a serving loop's hot
+ * path, code that must not call hipMalloc or hipFree, harnesses that forbid
+ * allocation).  Joins PRO or NPO (opts->algo; NULL opts = PRO) of device-resident
+ * relations of at most AJ_NOALLOC_MAX_TUPLES tuples each, on `stream` (a
+ * hipStream_t, NULL = the null stream) of the current device, and blocks until
+ * the result is on the host, like aj_join_dev.  Every workspace buffer is carved
+ * from one fixed array of AJ_NOALLOC_SCRATCH_BYTES in the library's device code:
+ * each GPU the library runs on holds its own copy until the process ends, and the
+ * first call on a device looks up that copy's address once.  No hipMalloc,
+ * hipFree or other allocating call is made.  All calls on one GPU share its
+ * array, which keeps the last call's contents; every buffer the kernels need
+ * zeroed is cleared at the start of each call, and a call holds the device's lock
+ * until its result is on the host, so calls on one GPU never overlap (calls that
+ * overlapped on the array would race).  Larger relations, and geometries whose
+ * workspace does not fit the array, are AJ_ERR_CAPACITY; algo SHJ is AJ_ERR_ARG.
+ * No events are created: device_ms and phase_ms are 0.  The caller's opts->pairs
+ * buffer (device memory it owns) may hold pairs or records as in aj_join_dev.
+ */
+#define AJ_NOALLOC_MAX_TUPLES (1u << 20)
+#define AJ_NOALLOC_SCRATCH_BYTES (64u << 20)
+int aj_join_dev_noalloc(const tuple_t *dR, uint64_t nR, const tuple_t *dS, uint64_t nS,
+                        const aj_join_opts_t *opts, aj_join_out_t *out, void *stream);
+
 /* ------------------------------------------------------- batched windows */
 /*
  * Many tumbling windows of two timestamp-ordered streams of 12-byte
