@@ -91,6 +91,9 @@ struct Ctx {
     DevBuf hist2d;              // per-chunk (d1, d2) counts of a two-pass PRO
     DevBuf table2, steps;       // SHJ: S-side table, per-step match counts
     DevBuf stage;               // staged PRO: staging regions
+    DevBuf chain, chain2;       // NPO / SHJ with pairs: duplicate chain links (R, S)
+    DevBuf heads, heads2;       // NPO / SHJ with pairs: chain heads per slot (R, S)
+    DevBuf carry, carry2;       // SHJ: carries of the per-key payload sums (R, S)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;  // none: no timing (device_ms 0)
     bool no_events = false;     // never create events (no per-kernel timing either)
     hipEvent_t evk[AJ_NPHASES + 1] = {};  // per-kernel timing (option kernel_timing)
@@ -284,8 +287,10 @@ int run_join(const JoinDesc &d, hipStream_t st, const Marks *marks = nullptr, in
 struct NpoDesc {
     const uint64_t *R, *S;
     uint64_t nR, nS;
-    uint64_t *table;     // capacity slots
+    uint64_t *table;     // capacity 16-byte slots (2 words each)
     uint64_t capacity;   // power of two
+    uint32_t *head;      // with pairs: capacity chain heads (R index + 1, 0 = none)
+    uint32_t *next;      // with pairs: nR chain links (R index + 1 of the next tuple)
     uint64_t *acc;       // [0]=count [1]=sum [2]=sentinel R tuples [3]=pair cursor
     uint64_t *pairs;
     uint64_t pairs_cap;

@@ -5,21 +5,29 @@
 // with every S tuple, counting each key-equal entry of the bucket chain
 // (probe_hashtable_single, common_functions.cpp:237-280).
 //
-// CDNA4 form: open addressing over 64-bit slots that hold the whole tuple
-// (key << 32 | payload), linear probing, capacity = next power of two >= 2|R|
-// (load factor <= 1/2).  Insert is one 64-bit atomicCAS per probed slot (the
-// latch and the slot write of the reference in one instruction).  Duplicate
-// keys each take their own slot; a probe walks from the key's home slot to the
-// first empty slot and counts every slot with an equal key.  The empty marker
-// is the all-ones word; an R tuple with exactly that bit pattern (key -1,
-// payload -1) is counted on the side and matched arithmetically, so every
-// 64-bit tuple value stays joinable.
+// CDNA4 form: open addressing over 16-byte slots, one slot per distinct key,
+// linear probing, capacity = next power of two >= 2|R| (at most half the slots
+// hold a key).  Word 0 of a slot is the key's first tuple (key << 32 |
+// payload), claimed with one 64-bit atomicCAS (the latch and the slot write of
+// the reference in one instruction).  Word 1 aggregates the key's further R
+// tuples, (sum of their payloads mod 2^32) << 32 | their count, with one 64-bit
+// atomicAdd each.  A counting probe that finds its key adds count = 1 + the
+// further count and checksum = (sum of the key's R payloads) << 32 + count * S
+// payload -- exactly the sum of (R.payload << 32 | S.payload) over its matches
+// mod 2^64 -- and stops: neither an insert nor a probe walks past the copies of
+// a key, so duplicates cost O(1) per tuple.  Joins that write pairs or records
+// also chain the further tuples (R index + 1, head per slot in `head`, links in
+// `next`, pushed with one atomicExch) and read their payloads from R.  The
+// empty marker is the all-ones word; an R tuple with exactly that bit pattern
+// (key -1, payload -1) is counted on the side and matched arithmetically, so
+// every 64-bit tuple value stays joinable.
 //
 // The home slot is a murmur3-finalised key: the reference's `key & mask`
 // (common_functions.h:57-59) would turn dense keys into one long run under
 // linear probing.
 //
-// Bytes per tuple: R read 8 + slot write 8; S read 8 + slot read 8.
+// Bytes per tuple: R read 8 + slot write 16; S read 8 + slot read 16 (one
+// slot per probe for unique keys at half load).
 #include "aj_internal.h"
 #include "device_common.h"
 
@@ -80,8 +88,54 @@ int fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st) {
     return AJ_OK;
 }
 
-__global__ __launch_bounds__(NBLOCK) void k_npo_build(const uint64_t *R, uint64_t n, uint64_t *tab,
-                                                    uint64_t mask, uint64_t *acc) {
+// Empty slots: word 0 all ones, aggregate 0.
+__global__ __launch_bounds__(NBLOCK) void k_fill_slots(uint4 *t, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        t[i] = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u);
+}
+
+// Insert tuple i (value v, key k): claim k's slot with its first tuple, or add
+// v to the slot's aggregate of further tuples (and, when head is set, chain i
+// behind the first).
+__device__ __forceinline__ void npo_insert(uint64_t i, uint64_t v, uint32_t k, uint64_t *tab,
+                                           uint64_t mask, uint32_t *head, uint32_t *next) {
+    uint64_t h = fmix32(k) & mask;
+    // terminates: fewer distinct keys than slots
+    for (;;) {
+        const uint64_t old = atomicCAS((unsigned long long *)&tab[2 * h],
+                                       (unsigned long long)EMPTY_SLOT, (unsigned long long)v);
+        if (old == EMPTY_SLOT) return;
+        if (key_of(old) == k) {
+            atomicAdd((unsigned long long *)&tab[2 * h + 1],
+                      (unsigned long long)(((uint64_t)payload_of(v) << 32) | 1u));
+            if (head) next[i] = atomicExch(&head[h], (uint32_t)(i + 1));
+            return;
+        }
+        h = (h + 1) & mask;
+    }
+}
+
+// Matches of key k in the table's slot for k: calls agg(first tuple, count of
+// further tuples, sum of their payloads) once if k has a slot.
+template <class F>
+__device__ __forceinline__ uint64_t npo_find(uint32_t k, const uint64_t *tab, uint64_t mask,
+                                             F &&agg) {
+    typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
+    for (uint64_t h = fmix32(k) & mask;; h = (h + 1) & mask) {
+        const u64x2 w = *reinterpret_cast<const u64x2 *>(tab + 2 * h);  // one 16-byte slot
+        if (w.x == EMPTY_SLOT) return ~0ull;
+        if (key_of(w.x) == k) {
+            agg(w.x, (uint32_t)w.y, (uint32_t)(w.y >> 32));
+            return h;
+        }
+    }
+}
+
+__global__ __launch_bounds__(NBLOCK) void k_npo_build(const uint64_t *R, uint64_t n,
+                                                    uint64_t *tab, uint64_t mask,
+                                                    uint32_t *head, uint32_t *next,
+                                                    uint64_t *acc) {
     uint64_t sentinels = 0;
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
          i += (uint64_t)gridDim.x * blockDim.x) {
@@ -90,59 +144,54 @@ __global__ __launch_bounds__(NBLOCK) void k_npo_build(const uint64_t *R, uint64_
             ++sentinels;
             continue;
         }
-        uint64_t h = fmix32(key_of(v)) & mask;
-        // terminates: fewer tuples than slots
-        while (atomicCAS((unsigned long long *)&tab[h], (unsigned long long)EMPTY_SLOT,
-                         (unsigned long long)v) != EMPTY_SLOT)
-            h = (h + 1) & mask;
+        npo_insert(i, v, key_of(v), tab, mask, head, next);
     }
     if (sentinels) atomicAdd((unsigned long long *)&acc[2], (unsigned long long)sentinels);
 }
 
-template <bool WRITE>
-__device__ __forceinline__ void npo_probe_one(uint64_t s, const uint64_t *tab, uint64_t mask,
-                                              uint64_t nsent, uint64_t &cnt, uint64_t &sum,
-                                              uint64_t *pairs, uint64_t &wpos, uint64_t cap,
-                                              int records = 0, uint32_t tnow = 0) {
+// Count and checksum of S tuple s: O(1) from the key's aggregate.
+__device__ __forceinline__ void npo_count_one(uint64_t s, const uint64_t *tab, uint64_t mask,
+                                              uint64_t nsent, uint64_t &cnt, uint64_t &sum) {
     const uint32_t k = key_of(s);
-    uint64_t h = fmix32(k) & mask;
-    for (;;) {
-        const uint64_t v = tab[h];
-        if (v == EMPTY_SLOT) break;
-        if (key_of(v) == k) {
-            if (WRITE) {
-                if (wpos < cap)
-                    pairs[wpos] = records == 2 ? ((uint64_t)tnow << 32) | (uint32_t)s
-                                  : records ? s : pair_value(v, s);
-                ++wpos;
-            } else {
-                ++cnt;
-                sum += pair_value(v, s);
-            }
-        }
-        h = (h + 1) & mask;
-    }
+    npo_find(k, tab, mask, [&](uint64_t first, uint32_t more, uint32_t more_sum) {
+        const uint64_t c = 1ull + more;
+        const uint32_t rsum = payload_of(first) + more_sum;  // R payloads, mod 2^32
+        cnt += c;
+        sum += ((uint64_t)rsum << 32) + c * payload_of(s);
+    });
     if (k == 0xFFFFFFFFu && nsent) {
-        const uint64_t pv = pair_value(EMPTY_SLOT, s);
-        if (WRITE) {
-            for (uint64_t q = 0; q < nsent; ++q) {
-                if (wpos < cap)
-                    pairs[wpos] = records == 2 ? ((uint64_t)tnow << 32) | (uint32_t)s
-                                  : records ? s : pv;
-                ++wpos;
-            }
-        } else {
-            cnt += nsent;
-            sum += nsent * pv;
-        }
+        cnt += nsent;
+        sum += nsent * pair_value(EMPTY_SLOT, s);
     }
+}
+
+// Every match of S tuple s written from wpos on (the first tuple, then the
+// chain of further ones), as pairs, S records or timed records.
+__device__ __forceinline__ void npo_write_one(uint64_t s, const uint64_t *tab, uint64_t mask,
+                                              const uint64_t *R, const uint32_t *head,
+                                              const uint32_t *next, uint64_t nsent,
+                                              uint64_t *pairs, uint64_t &wpos, uint64_t cap,
+                                              int records, uint32_t tnow) {
+    auto emit = [&](uint64_t r) {
+        if (wpos < cap)
+            pairs[wpos] = records == 2 ? ((uint64_t)tnow << 32) | (uint32_t)s
+                          : records ? s : pair_value(r, s);
+        ++wpos;
+    };
+    const uint32_t k = key_of(s);
+    const uint64_t h = npo_find(k, tab, mask, [&](uint64_t first, uint32_t, uint32_t) { emit(first); });
+    if (h != ~0ull)
+        for (uint32_t j = head[h]; j; j = next[j - 1]) emit(R[j - 1]);
+    if (k == 0xFFFFFFFFu)
+        for (uint64_t q = 0; q < nsent; ++q) emit(EMPTY_SLOT);
 }
 
 template <bool PAIRS>
 __global__ __launch_bounds__(NBLOCK) void k_npo_probe(const uint64_t *S, uint64_t n,
                                                     const uint64_t *tab, uint64_t mask,
-                                                    uint64_t *acc, uint64_t *pairs, uint64_t cap,
-                                                    int records) {
+                                                    const uint64_t *R, const uint32_t *head,
+                                                    const uint32_t *next, uint64_t *acc,
+                                                    uint64_t *pairs, uint64_t cap, int records) {
     __shared__ uint64_t red[NBLOCK / 64];
     __shared__ uint64_t s_base;
     const uint64_t nsent = acc[2];
@@ -153,8 +202,8 @@ __global__ __launch_bounds__(NBLOCK) void k_npo_probe(const uint64_t *S, uint64_
         const uint64_t i = base + threadIdx.x;
         const bool valid = i < n;
         const uint64_t s = valid ? S[i] : 0;
-        uint64_t c = 0, x = 0, wpos = 0;
-        if (valid) npo_probe_one<false>(s, tab, mask, nsent, c, x, nullptr, wpos, 0);
+        uint64_t c = 0, x = 0;
+        if (valid) npo_count_one(s, tab, mask, nsent, c, x);
         if (PAIRS) {
             uint64_t total;
             const uint64_t pre = block_exclusive_scan<NBLOCK>(c, red, &total);
@@ -162,11 +211,10 @@ __global__ __launch_bounds__(NBLOCK) void k_npo_probe(const uint64_t *S, uint64_
                 s_base = atomicAdd((unsigned long long *)&acc[3], (unsigned long long)total);
             __syncthreads();
             if (valid && c) {
-                wpos = s_base + pre;
-                uint64_t dc = 0, dx = 0;
+                uint64_t wpos = s_base + pre;
                 const uint32_t tnow =
                     records == 2 ? (uint32_t)__builtin_amdgcn_s_memrealtime() : 0u;
-                npo_probe_one<true>(s, tab, mask, nsent, dc, dx, pairs, wpos, cap, records, tnow);
+                npo_write_one(s, tab, mask, R, head, next, nsent, pairs, wpos, cap, records, tnow);
             }
             __syncthreads();
         }
@@ -205,22 +253,30 @@ int run_npo(const NpoDesc &d, hipStream_t st, const Marks *mk, int m0) {
         set_error(AJ_ERR_ARG, "NPO table capacity must be a power of two above |R|");
         return AJ_ERR_ARG;
     }
-    AJ_TRY(fill_u64(d.table, d.capacity, EMPTY_SLOT, st));
+    if (d.pairs && (!d.head || !d.next)) {
+        set_error(AJ_ERR_ARG, "NPO with pairs needs the chain arrays");
+        return AJ_ERR_ARG;
+    }
+    hipLaunchKernelGGL(k_fill_slots, dim3(grid_for(d.capacity, 256)), dim3(NBLOCK), 0, st,
+                       reinterpret_cast<uint4 *>(d.table), d.capacity);
+    uint32_t *head = d.pairs ? d.head : nullptr;  // chains only when matches are written
+    if (head) AJ_TRY(fill_u32(head, d.capacity, 0, st));
     const uint64_t mask = d.capacity - 1;
     if (mk) mk->mark(m0);
     if (d.nR)
         hipLaunchKernelGGL(k_npo_build, dim3(grid_for(d.nR, d.num_cu)), dim3(NBLOCK), 0, st, d.R,
-                           d.nR, d.table, mask, d.acc);
+                           d.nR, d.table, mask, head, d.next, d.acc);
     if (mk) mk->mark(m0 + 1);
     if (d.nS) {
         const unsigned g = grid_for(d.nS, d.num_cu);  // <= num_cu * 16 (npo_acc_words)
         if (d.pairs)
             hipLaunchKernelGGL(k_npo_probe<true>, dim3(g), dim3(NBLOCK), 0, st, d.S, d.nS,
-                               (const uint64_t *)d.table, mask, d.acc, d.pairs, d.pairs_cap,
-                               d.records);
+                               (const uint64_t *)d.table, mask, d.R, (const uint32_t *)head,
+                               (const uint32_t *)d.next, d.acc, d.pairs, d.pairs_cap, d.records);
         else
             hipLaunchKernelGGL(k_npo_probe<false>, dim3(g), dim3(NBLOCK), 0, st, d.S, d.nS,
-                               (const uint64_t *)d.table, mask, d.acc, nullptr, (uint64_t)0, 0);
+                               (const uint64_t *)d.table, mask, d.R, (const uint32_t *)nullptr,
+                               (const uint32_t *)nullptr, d.acc, nullptr, (uint64_t)0, 0);
         hipLaunchKernelGGL(k_npo_reduce, dim3(1), dim3(NBLOCK), 0, st, d.acc, g);
     }
     if (mk) mk->mark(m0 + 2);

@@ -562,9 +562,13 @@ int pro_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
 int npo_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, uint64_t nS,
                uint64_t *pairs, uint64_t pairs_cap, int records, aj_join_out_t *out,
                hipStream_t st) {
-    uint64_t cap = 2;
-    while (cap <= 2 * nR) cap <<= 1;
-    AJ_TRY(ctx->table.ensure(cap * 8));
+    uint64_t cap = 2;  // load at most 1/2: one slot per distinct key
+    while (cap < 2 * nR) cap <<= 1;
+    AJ_TRY(ctx->table.ensure(cap * 16));
+    if (pairs) {  // chains of the further tuples of a key, to write every match
+        AJ_TRY(ctx->chain.ensure(nR * 4 + 4));
+        AJ_TRY(ctx->heads.ensure(cap * 4));
+    }
     AJ_TRY(ctx->acc.ensure(8 * npo_acc_words(ctx->num_cu)));
     uint64_t *acc = ctx->acc.as<uint64_t>();
     AJ_HIP(record_event(ctx->ev0, st));
@@ -576,6 +580,8 @@ int npo_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
     d.nS = nS;
     d.table = ctx->table.as<uint64_t>();
     d.capacity = cap;
+    d.head = pairs ? ctx->heads.as<uint32_t>() : nullptr;
+    d.next = pairs ? ctx->chain.as<uint32_t>() : nullptr;
     d.acc = acc;
     d.pairs = pairs;
     d.pairs_cap = pairs_cap;

@@ -15,12 +15,15 @@
 // against TR.  The kernel boundary is the only ordering needed: a pair (r_i,
 // s_j) is found exactly once, in the step of its later tuple (r first on a
 // tie), which for batch = 1 is the reference's alternating order.  Tables are
-// the NPO layout (open addressing over whole 64-bit tuples, capacity the next
-// power of two >= 2n, murmur3-finalised home slot, all-ones empty marker with
-// all-ones tuples counted on the side).  Per-step cumulative match counts give
-// the progressive result curve.
+// the NPO layout (16-byte slots, one per distinct key, holding its first tuple
+// and the count and payload sum of its further tuples; capacity the next power of two
+// >= 2n, murmur3-finalised home slot, all-ones empty marker with all-ones
+// tuples counted on the side): a counting probe takes its key's count and
+// payload sum from the slot in O(1), the writing modes walk the key's chain, and
+// an insert never walks past the copies of a key.  Per-step cumulative match
+// counts give the progressive result curve.
 //
-// Bytes per tuple: read 8 + own slot write 8 + probe slot reads (>= 8).
+// Bytes per tuple: read 8 + own slot write 16 + probe slot reads (>= 16).
 #include "aj_internal.h"
 #include "device_common.h"
 
@@ -28,35 +31,113 @@ namespace aj {
 
 constexpr int SBLOCK = 256;
 
-// Insert v into tab (the all-ones tuple is counted in *own_sent instead) and
-// count / checksum its matches in other (other_sent all-ones tuples there).
-// Work-group b adds its matches to step_cnt[b] (this step's row) and its
-// checksum to sum_part[b]: every launch has the same grid and launches are
-// stream-ordered, so each slot has one writer at a time and no atomics contend.
-__global__ __launch_bounds__(SBLOCK) void k_shj_step(const uint64_t *in, uint64_t lo,
-                                                     uint64_t hi, uint64_t *tab, uint64_t mask,
-                                                     uint64_t *own_sent, const uint64_t *other,
-                                                     uint64_t omask, const uint64_t *other_sent,
-                                                     int is_r, uint64_t *sum_part,
+// One side's table: 16-byte slots as NPO's (word 0 the first tuple of a
+// distinct key, all ones = empty; word 1 the aggregate of the key's further
+// tuples, payload sum mod 2^32 << 32 | count), the carries out of that sum per
+// slot (an S key's full payload sum is needed when R tuples probe it: the S
+// payloads fill the low word of a pair, so their sum carries into the high
+// word), the side's tuples, its all-ones tuple count, and -- in the writing
+// modes -- the chains of further tuples (head per slot, side index + 1; links in
+// next[]).
+struct ShjSide {
+    uint64_t *tab;
+    uint64_t mask;
+    uint32_t *carry; // per slot: carries out of the aggregate's 32-bit payload sum
+    uint32_t *head;  // null: no chains (counting mode)
+    uint32_t *next;
+    const uint64_t *in;
+    uint64_t *sent;
+};
+
+// k's slot in a side's table, or ~0 if k has none; w receives the slot.
+__device__ __forceinline__ uint64_t shj_find(uint32_t k, const ShjSide &o, uint64_t &first,
+                                             uint64_t &agg) {
+    typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
+    for (uint64_t h = fmix32(k) & o.mask;; h = (h + 1) & o.mask) {
+        const u64x2 w = *reinterpret_cast<const u64x2 *>(o.tab + 2 * h);
+        if (w.x == EMPTY_SLOT) return ~0ull;
+        if (key_of(w.x) == k) {
+            first = w.x;
+            agg = w.y;
+            return h;
+        }
+    }
+}
+
+// Count and checksum of v's matches in the other side's table in O(1): the
+// key's count is 1 + the further count, its payload sum the first payload plus
+// the further sum (mod 2^32).
+__device__ __forceinline__ void shj_count(uint64_t v, const ShjSide &o, int is_r, uint64_t &cnt,
+                                          uint64_t &sum) {
+    uint64_t first, agg;
+    const uint64_t h = shj_find(key_of(v), o, first, agg);
+    if (h == ~0ull) return;
+    const uint32_t more = (uint32_t)agg;
+    const uint64_t c = 1ull + more;
+    cnt += c;
+    if (is_r) {  // S payloads fill the low word of a pair: their full sum
+        const uint64_t carries = more ? o.carry[h] : 0u;
+        const uint64_t osum = payload_of(first) + ((carries << 32) | (agg >> 32));
+        sum += c * ((uint64_t)payload_of(v) << 32) + osum;
+    } else {  // R payloads fill the high word: their sum mod 2^32 suffices
+        const uint32_t osum = payload_of(first) + (uint32_t)(agg >> 32);
+        sum += ((uint64_t)osum << 32) + c * payload_of(v);
+    }
+}
+
+// f(t) for every tuple t of the other side with v's key (the first, then the
+// chain of further ones); the writing modes only.
+template <class F>
+__device__ __forceinline__ void shj_matches(uint32_t k, const ShjSide &o, F &&f) {
+    uint64_t first, agg;
+    const uint64_t h = shj_find(k, o, first, agg);
+    if (h == ~0ull) return;
+    f(first);
+    for (uint32_t j = o.head[h]; j; j = o.next[j - 1]) f(o.in[j - 1]);
+}
+
+// Insert tuple i (value v, key k) into its side's table: claim k's slot with a
+// 64-bit CAS, or add v to the slot's aggregate (one 64-bit add) and, in the
+// writing modes, chain i behind it with one exchange.
+__device__ __forceinline__ void shj_insert(uint64_t i, uint64_t v, uint32_t k, const ShjSide &s) {
+    uint64_t h = fmix32(k) & s.mask;
+    // terminates: fewer distinct keys than slots
+    for (;;) {
+        const uint64_t old = atomicCAS((unsigned long long *)&s.tab[2 * h],
+                                       (unsigned long long)EMPTY_SLOT, (unsigned long long)v);
+        if (old == EMPTY_SLOT) return;
+        if (key_of(old) == k) {
+            const uint32_t p = payload_of(v);
+            const uint64_t prev = atomicAdd((unsigned long long *)&s.tab[2 * h + 1],
+                                            (unsigned long long)(((uint64_t)p << 32) | 1u));
+            // this add carried out of the 32-bit sum (the count half never does:
+            // fewer than 2^32 tuples)
+            if ((uint32_t)(prev >> 32) + p < p) atomicAdd(&s.carry[h], 1u);
+            if (s.head) s.next[i] = atomicExch(&s.head[h], (uint32_t)(i + 1));
+            return;
+        }
+        h = (h + 1) & s.mask;
+    }
+}
+
+// Insert tuple i of this side into its table (the all-ones tuple is counted in
+// *own.sent instead) and count / checksum its matches in the other side's
+// table (other.sent all-ones tuples there).  Work-group b adds its matches to
+// step_cnt[b] (this step's row) and its checksum to sum_part[b]: every launch
+// has the same grid and launches are stream-ordered, so each slot has one
+// writer at a time and no atomics contend.
+__global__ __launch_bounds__(SBLOCK) void k_shj_step(uint64_t lo, uint64_t hi, ShjSide own,
+                                                     ShjSide other, int is_r, uint64_t *sum_part,
                                                      uint64_t *step_cnt) {
     __shared__ uint64_t red[SBLOCK / 64];
-    const uint64_t nsent = *other_sent;  // not written during this kernel
+    const uint64_t nsent = *other.sent;  // not written during this kernel
     uint64_t cnt = 0, sum = 0, sent = 0;
     for (uint64_t i = lo + blockIdx.x * (uint64_t)SBLOCK + threadIdx.x; i < hi;
          i += (uint64_t)gridDim.x * SBLOCK) {
-        const uint64_t v = in[i];
+        const uint64_t v = own.in[i];
         const uint32_t k = key_of(v);
         // probe the other side first (its table does not change in this kernel)
-        uint64_t h = fmix32(k) & omask;
-        for (;;) {
-            const uint64_t t = other[h];
-            if (t == EMPTY_SLOT) break;
-            if (key_of(t) == k) {
-                ++cnt;
-                sum += is_r ? pair_value(v, t) : pair_value(t, v);
-            }
-            h = (h + 1) & omask;
-        }
+        shj_count(v, other, is_r, cnt, sum);
         if (k == 0xFFFFFFFFu && nsent) {  // all-ones tuples of the other side
             cnt += nsent;
             const uint64_t pv = is_r ? pair_value(v, EMPTY_SLOT) : pair_value(EMPTY_SLOT, v);
@@ -66,17 +147,13 @@ __global__ __launch_bounds__(SBLOCK) void k_shj_step(const uint64_t *in, uint64_
             ++sent;
             continue;
         }
-        h = fmix32(k) & mask;
-        // terminates: fewer tuples than slots
-        while (atomicCAS((unsigned long long *)&tab[h], (unsigned long long)EMPTY_SLOT,
-                         (unsigned long long)v) != EMPTY_SLOT)
-            h = (h + 1) & mask;
+        shj_insert(i, v, k, own);
     }
     sent = block_sum<SBLOCK>(sent, red);
     cnt = block_sum<SBLOCK>(cnt, red);
     sum = block_sum<SBLOCK>(sum, red);
     if (threadIdx.x == 0) {
-        if (sent) atomicAdd((unsigned long long *)own_sent, (unsigned long long)sent);
+        if (sent) atomicAdd((unsigned long long *)own.sent, (unsigned long long)sent);
         if (cnt) {
             step_cnt[blockIdx.x] += cnt;
             sum_part[blockIdx.x] += sum;
@@ -90,34 +167,30 @@ __global__ __launch_bounds__(SBLOCK) void k_shj_step(const uint64_t *in, uint64_
 // record SHJJoiner::join's probe appends, common_functions.cpp:251-257), mode 3
 // a timed record (device clock << 32 | the probing tuple's payloadID, the
 // sample END_PROGRESSIVE_MEASURE takes there).  A wave counts its lanes'
-// matches, reserves one range with a single atomic on *cursor, then walks its
-// probe chains again to write; positions at or past cap are counted but not
-// written (the caller reports the shortfall).  In mode 3 an R tuple's records
-// fill out from the end backwards (its cursor counts them), so the caller can
-// tell the two probe sides apart.  The
-// loop runs a work-group-uniform number of rounds so the wave operations see
-// every lane.
-__global__ __launch_bounds__(SBLOCK) void k_shj_step_out(
-    const uint64_t *in, uint64_t lo, uint64_t hi, uint64_t *tab, uint64_t mask,
-    uint64_t *own_sent, const uint64_t *other, uint64_t omask, const uint64_t *other_sent,
-    int is_r, uint64_t *sum_part, uint64_t *step_cnt, int mode, uint64_t *out, uint64_t cap,
-    uint64_t *cursor) {
+// matches, reserves one range with a single atomic on *cursor, then finds its
+// matches again to write; positions at or past cap are counted but not written
+// (the caller reports the shortfall).  In mode 3 an R tuple's records fill out
+// from the end backwards (its cursor counts them), so the caller can tell the
+// two probe sides apart.  The loop runs a work-group-uniform number of rounds
+// so the wave operations see every lane.
+__global__ __launch_bounds__(SBLOCK) void k_shj_step_out(uint64_t lo, uint64_t hi, ShjSide own,
+                                                         ShjSide other, int is_r,
+                                                         uint64_t *sum_part, uint64_t *step_cnt,
+                                                         int mode, uint64_t *out, uint64_t cap,
+                                                         uint64_t *cursor) {
     __shared__ uint64_t red[SBLOCK / 64];
-    const uint64_t nsent = *other_sent;  // not written during this kernel
+    const uint64_t nsent = *other.sent;  // not written during this kernel
     uint64_t cnt = 0, sum = 0, sent = 0;
     for (uint64_t base = lo + blockIdx.x * (uint64_t)SBLOCK; base < hi;
          base += (uint64_t)gridDim.x * SBLOCK) {
         const uint64_t i = base + threadIdx.x;
         const bool valid = i < hi;
-        const uint64_t v = valid ? in[i] : 0;
+        const uint64_t v = valid ? own.in[i] : 0;
         const uint32_t k = key_of(v);
         uint64_t c = 0;
         if (valid) {
-            for (uint64_t h = fmix32(k) & omask;; h = (h + 1) & omask) {
-                const uint64_t t = other[h];
-                if (t == EMPTY_SLOT) break;
-                c += key_of(t) == k;
-            }
+            uint64_t first, agg;
+            if (shj_find(k, other, first, agg) != ~0ull) c = 1ull + (uint32_t)agg;
             if (k == 0xFFFFFFFFu) c += nsent;  // all-ones tuples of the other side
         }
         const uint64_t incl = wave_inclusive_scan(c);
@@ -139,11 +212,7 @@ __global__ __launch_bounds__(SBLOCK) void k_shj_step_out(
                 }
                 ++pos;
             };
-            for (uint64_t h = fmix32(k) & omask;; h = (h + 1) & omask) {
-                const uint64_t t = other[h];
-                if (t == EMPTY_SLOT) break;
-                if (key_of(t) == k) emit(t);
-            }
+            shj_matches(k, other, emit);
             if (k == 0xFFFFFFFFu)
                 for (uint64_t n = 0; n < nsent; ++n) emit(EMPTY_SLOT);
             cnt += c;
@@ -153,22 +222,25 @@ __global__ __launch_bounds__(SBLOCK) void k_shj_step_out(
             ++sent;
             continue;
         }
-        uint64_t h = fmix32(k) & mask;
-        // terminates: fewer tuples than slots
-        while (atomicCAS((unsigned long long *)&tab[h], (unsigned long long)EMPTY_SLOT,
-                         (unsigned long long)v) != EMPTY_SLOT)
-            h = (h + 1) & mask;
+        shj_insert(i, v, k, own);
     }
     sent = block_sum<SBLOCK>(sent, red);
     cnt = block_sum<SBLOCK>(cnt, red);
     sum = block_sum<SBLOCK>(sum, red);
     if (threadIdx.x == 0) {
-        if (sent) atomicAdd((unsigned long long *)own_sent, (unsigned long long)sent);
+        if (sent) atomicAdd((unsigned long long *)own.sent, (unsigned long long)sent);
         if (cnt) {
             step_cnt[blockIdx.x] += cnt;
             sum_part[blockIdx.x] += sum;
         }
     }
+}
+
+// Empty slots of a side's table: word 0 all ones, aggregate 0.
+__global__ __launch_bounds__(SBLOCK) void k_shj_fill(uint4 *t, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        t[i] = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u);
 }
 
 // Work-group k: rows[k] = matches of step k, the sum of the per-block row
@@ -217,9 +289,9 @@ unsigned shj_grid(uint64_t n, int num_cu) {
     return (unsigned)(g ? g : 1);
 }
 
-uint64_t table_cap(uint64_t n) {
+uint64_t table_cap(uint64_t n) {  // load at most 1/2: one slot per distinct key
     uint64_t c = 2;
-    while (c <= 2 * n) c <<= 1;
+    while (c < 2 * n) c <<= 1;
     return c;
 }
 
@@ -247,47 +319,63 @@ int shj_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
         return AJ_ERR_ARG;
     }
     const unsigned G = shj_grid(batch, ctx->num_cu);  // one grid for every launch
-    AJ_TRY(ctx->table.ensure(capR * 8));
-    AJ_TRY(ctx->table2.ensure(capS * 8));
+    AJ_TRY(ctx->table.ensure(capR * 16));
+    AJ_TRY(ctx->table2.ensure(capS * 16));
+    AJ_TRY(ctx->carry.ensure(capR * 4));
+    AJ_TRY(ctx->carry2.ensure(capS * 4));
+    if (mode) {  // chains of further tuples, to write every match
+        AJ_TRY(ctx->chain.ensure(nR * 4 + 4));
+        AJ_TRY(ctx->chain2.ensure(nS * 4 + 4));
+        AJ_TRY(ctx->heads.ensure(capR * 4));
+        AJ_TRY(ctx->heads2.ensure(capS * 4));
+    }
     AJ_TRY(ctx->acc.ensure(8 * 8));
     AJ_TRY(ctx->steps.ensure(8 * (steps * G + G + steps)));
     // [0] count [1] sum [2] R sentinels [3] S sentinels [4] output cursor
     // [5] R-probe output cursor (timed records)
     uint64_t *acc = ctx->acc.as<uint64_t>();
-    uint64_t *tr = ctx->table.as<uint64_t>(), *ts = ctx->table2.as<uint64_t>();
     uint64_t *scnt = ctx->steps.as<uint64_t>();  // [steps][G] per-block step matches
     uint64_t *sum_part = scnt + steps * G;       // [G] per-block checksums
     uint64_t *prog = sum_part + G;               // [steps] cumulative matches
+    const ShjSide sideR{ctx->table.as<uint64_t>(), capR - 1, ctx->carry.as<uint32_t>(),
+                        mode ? ctx->heads.as<uint32_t>() : nullptr,
+                        mode ? ctx->chain.as<uint32_t>() : nullptr, dR, acc + 2};
+    const ShjSide sideS{ctx->table2.as<uint64_t>(), capS - 1, ctx->carry2.as<uint32_t>(),
+                        mode ? ctx->heads2.as<uint32_t>() : nullptr,
+                        mode ? ctx->chain2.as<uint32_t>() : nullptr, dS, acc + 3};
     AJ_HIP(record_event(ctx->ev0, st));
     AJ_TRY(fill_u64(acc, 8, 0, st));
-    AJ_TRY(fill_u64(tr, capR, EMPTY_SLOT, st));
-    AJ_TRY(fill_u64(ts, capS, EMPTY_SLOT, st));
+    hipLaunchKernelGGL(k_shj_fill, dim3(shj_grid(capR, 32)), dim3(SBLOCK), 0, st,
+                       reinterpret_cast<uint4 *>(sideR.tab), capR);
+    hipLaunchKernelGGL(k_shj_fill, dim3(shj_grid(capS, 32)), dim3(SBLOCK), 0, st,
+                       reinterpret_cast<uint4 *>(sideS.tab), capS);
+    AJ_TRY(fill_u32(sideR.carry, capR, 0, st));
+    AJ_TRY(fill_u32(sideS.carry, capS, 0, st));
+    if (mode) {
+        AJ_TRY(fill_u32(sideR.head, capR, 0, st));
+        AJ_TRY(fill_u32(sideS.head, capS, 0, st));
+    }
     AJ_TRY(fill_u64(scnt, steps * G + G, 0, st));
     for (uint64_t k = 0; k < steps; ++k) {
         const uint64_t lo = k * batch;
         if (lo < nR) {
             const uint64_t hi = lo + batch < nR ? lo + batch : nR;
             if (mode)
-                hipLaunchKernelGGL(k_shj_step_out, dim3(G), dim3(SBLOCK), 0, st, dR, lo, hi, tr,
-                                   capR - 1, acc + 2, (const uint64_t *)ts, capS - 1,
-                                   (const uint64_t *)(acc + 3), 1, sum_part, scnt + k * G, mode,
-                                   pairs, pcap, mode == 3 ? acc + 5 : acc + 4);
+                hipLaunchKernelGGL(k_shj_step_out, dim3(G), dim3(SBLOCK), 0, st, lo, hi, sideR,
+                                   sideS, 1, sum_part, scnt + k * G, mode, pairs, pcap,
+                                   mode == 3 ? acc + 5 : acc + 4);
             else
-                hipLaunchKernelGGL(k_shj_step, dim3(G), dim3(SBLOCK), 0, st, dR, lo, hi, tr,
-                                   capR - 1, acc + 2, (const uint64_t *)ts, capS - 1,
-                                   (const uint64_t *)(acc + 3), 1, sum_part, scnt + k * G);
+                hipLaunchKernelGGL(k_shj_step, dim3(G), dim3(SBLOCK), 0, st, lo, hi, sideR, sideS,
+                                   1, sum_part, scnt + k * G);
         }
         if (lo < nS) {
             const uint64_t hi = lo + batch < nS ? lo + batch : nS;
             if (mode)
-                hipLaunchKernelGGL(k_shj_step_out, dim3(G), dim3(SBLOCK), 0, st, dS, lo, hi, ts,
-                                   capS - 1, acc + 3, (const uint64_t *)tr, capR - 1,
-                                   (const uint64_t *)(acc + 2), 0, sum_part, scnt + k * G, mode,
-                                   pairs, pcap, acc + 4);
+                hipLaunchKernelGGL(k_shj_step_out, dim3(G), dim3(SBLOCK), 0, st, lo, hi, sideS,
+                                   sideR, 0, sum_part, scnt + k * G, mode, pairs, pcap, acc + 4);
             else
-                hipLaunchKernelGGL(k_shj_step, dim3(G), dim3(SBLOCK), 0, st, dS, lo, hi, ts,
-                                   capS - 1, acc + 3, (const uint64_t *)tr, capR - 1,
-                                   (const uint64_t *)(acc + 2), 0, sum_part, scnt + k * G);
+                hipLaunchKernelGGL(k_shj_step, dim3(G), dim3(SBLOCK), 0, st, lo, hi, sideS, sideR,
+                                   0, sum_part, scnt + k * G);
         }
     }
     hipLaunchKernelGGL(k_shj_rows, dim3((unsigned)steps), dim3(SBLOCK), 0, st,

@@ -427,3 +427,43 @@ print("OK", r.count, flush=True)
     lines = r.stdout.split("\n")
     assert lines[:3] == [f"STATUS {aj.AJ_ERR_NOMEM}"] * 3, r.stdout
     assert lines[3] == f"OK {1 << 22}"
+
+
+@pytest.mark.parametrize("algo", ["PRO", "NPO", "SHJ_st"])
+def test_zipf_keys_on_both_sides(aj, oracle, algo):
+    """Duplicate-heavy R and S: both relations' keys Zipf(1.0) over their own sizes, as
+    the reference's -d 2 draws them (create_relation_zipf, generator.cpp:851-869); the
+    top key has ~73K copies on each side."""
+    n = 1 << 20
+    R = aj.gen_zipf(n, n, 1.0, 31)
+    S = aj.gen_zipf(n, n, 1.0, 32)
+    print(f"case zipf both sides {algo}", flush=True)
+    got = getattr(aj, algo)(R, S, 8)
+    assert (got.count, got.checksum) == oracle.aggregate(R, S)
+
+
+@pytest.mark.parametrize("algo", ["NPO", "SHJ_st"])
+@pytest.mark.parametrize("kind", ["zipf15", "one_key", "wide_payloads"])
+def test_hash_tables_with_hot_keys(aj, oracle, algo, kind):
+    """The hash-table joins with millions of copies of one key: an insert costs one
+    atomic once the key's slot is found and a counting probe reads one slot, so the
+    time does not grow with the copies (count up to ~10^13 matches).  With payloads
+    over the full 32-bit range the per-key payload sums wrap many times: the S sums
+    feed the low word of the checksum with their carries."""
+    n = 1 << 24
+    if kind == "zipf15":  # top key ~38% of each side
+        R = aj.gen_zipf(n, n, 1.5, 41)
+        S = aj.gen_zipf(n, n, 1.5, 42)
+    elif kind == "wide_payloads":  # 64 keys, ~2^18 copies each per side
+        rng = np.random.default_rng(44)
+        pay = lambda m: rng.integers(-2**31, 2**31, m, dtype=np.int64).astype(np.int32)
+        R = aj.as_tuples(rng.integers(0, 64, n >> 2).astype(np.int32), pay(n >> 2))
+        S = aj.as_tuples(rng.integers(0, 64, n >> 2).astype(np.int32), pay(n >> 2))
+    else:  # every R tuple has key 7; 1/16 of S matches
+        R = aj.as_tuples(np.full(n >> 2, 7), np.arange(n >> 2))
+        S = aj.gen_uniform(n >> 2, 1, 16, 43)
+        S["key"][S["key"] != 7] += 100
+    want = oracle.aggregate(R, S)
+    print(f"case hot keys {algo} {kind} count={want[0]}", flush=True)
+    got = getattr(aj, algo)(R, S, 8)
+    assert (got.count, got.checksum) == want
