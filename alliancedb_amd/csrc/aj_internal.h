@@ -100,6 +100,8 @@ struct Ctx {
     std::vector<hipEvent_t> evs;          // per-launch timing of the staged passes
 };
 bool kernel_timing_enabled();
+// option npo_region_min: smallest |R| whose NPO table is built by regions (0 = never)
+uint64_t npo_region_min();
 // Events of a context that may have none: recording is skipped, times are 0.
 inline hipError_t record_event(hipEvent_t e, hipStream_t st) {
     return e ? hipEventRecord(e, st) : hipSuccess;
@@ -212,7 +214,7 @@ struct StagedGeom {
     int nset = 1;             // staging cursor sets (work-group mod nset)
     int block = 1024;         // lanes per work-group (512 or 1024)
     int cache_mode = 9;       // nontemporal accesses: 1 input loads, 2 staging stores,
-                              // 4 staging loads, 8 final stores
+                              // 4 staging loads, 8 final stores; 16: keys hashed in pass 1
     uint64_t chunk = 0;       // tuples per super-chunk (a multiple of the tile)
     uint64_t tiles_per_wg = 0;
     uint32_t grid1 = 0;       // pass-1 work-groups of a full super-chunk
@@ -284,14 +286,29 @@ uint64_t join_max_tasks(uint64_t max_parts, uint64_t max_S, uint32_t split);
 int run_join(const JoinDesc &d, hipStream_t st, const Marks *marks = nullptr, int m0 = 0);
 
 // NPO global table
+// Slot geometry of the NPO table: 2^(b1+b2) regions of 2^rs_log2 slots each.
+// A key with hash h = fmix32(key) lives in region ((h mod 2^b1) << b2) |
+// ((h >> b1) mod 2^b2), from slot (h >> (b1+b2)) mod 2^rs_log2 of it on,
+// probing linearly and wrapping inside the region.  b1 = b2 = 0: one region,
+// the flat table.
+struct NpoGeom {
+    uint32_t b1 = 0, b2 = 0, rs_log2 = 0;
+};
+
 struct NpoDesc {
     const uint64_t *R, *S;
     uint64_t nR, nS;
     uint64_t *table;     // capacity 16-byte slots (2 words each)
     uint64_t capacity;   // power of two
+    // region build (counting joins only): R partitioned by hashed key into the
+    // table's regions (the staged passes with keys hashed in pass 1), each
+    // region built in LDS; null = the flat build with device-wide atomics
+    const StagedDesc *stage = nullptr;
+    NpoGeom geom;        // the region geometry when stage is set
     uint32_t *head;      // with pairs: capacity chain heads (R index + 1, 0 = none)
     uint32_t *next;      // with pairs: nR chain links (R index + 1 of the next tuple)
     uint64_t *acc;       // [0]=count [1]=sum [2]=sentinel R tuples [3]=pair cursor
+                         // [4]=region build overflow flag (u32; the staged flag)
     uint64_t *pairs;
     uint64_t pairs_cap;
     int records = 0;  // 1: <S.key, S.payloadID> records; 2: (clock << 32 | S.payloadID)

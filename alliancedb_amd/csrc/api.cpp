@@ -115,10 +115,12 @@ void DevBuf::release() {
 static std::atomic<int64_t> opt_materialize{0}, opt_radix_bits{0}, opt_passes{0}, opt_device{0},
     opt_cap{0}, opt_split{0}, opt_kernel_timing{0}, opt_part_block{0}, opt_part_chunk{0},
     opt_part_hist2d{1}, opt_tuned{1}, opt_timing{0}, opt_shj_batch{0}, opt_staged{1},
-    opt_stage_sets{0}, opt_stage_chunk{0}, opt_stage_block{0}, opt_stage_cache{-1};
+    opt_stage_sets{0}, opt_stage_chunk{0}, opt_stage_block{0}, opt_stage_cache{-1},
+    opt_npo_region_min{1ll << 22};
 
 static thread_local bool g_force_timing = false;  // phase timing for one drop-in call
 bool kernel_timing_enabled() { return g_force_timing || opt_kernel_timing.load() != 0; }
+uint64_t npo_region_min() { return (uint64_t)opt_npo_region_min.load(); }
 
 static int ctx_init(Ctx *c, int device) {
     int n = 0;
@@ -637,7 +639,8 @@ int aj_set_option(const char *name, int64_t value) {
                {"stage_sets", &opt_stage_sets, 0, 64},
                {"stage_chunk", &opt_stage_chunk, 0, 1ll << 31},
                {"stage_block", &opt_stage_block, 0, 1024},
-               {"stage_cache", &opt_stage_cache, -1, 15}};
+               {"stage_cache", &opt_stage_cache, -1, 15},
+               {"npo_region_min", &opt_npo_region_min, 0, 1ll << 32}};
     for (auto &e : tab) {
         if (strcmp(e.n, name) == 0) {
             if (value < e.lo || value > e.hi) {
@@ -693,6 +696,7 @@ int64_t aj_get_option(const char *name) {
     if (!strcmp(name, "stage_chunk")) return opt_stage_chunk.load();
     if (!strcmp(name, "stage_block")) return opt_stage_block.load();
     if (!strcmp(name, "stage_cache")) return opt_stage_cache.load();
+    if (!strcmp(name, "npo_region_min")) return opt_npo_region_min.load();
     if (!strcmp(name, "device_budget")) return (int64_t)g_dev_budget.load();
     if (!strcmp(name, "device_used")) return (int64_t)g_dev_used.load();
     return -1;

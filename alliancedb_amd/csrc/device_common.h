@@ -198,4 +198,16 @@ __device__ __forceinline__ uint32_t fmix32(uint32_t h) {
     return h;
 }
 
+__device__ __forceinline__ uint32_t fmix32_inv(uint32_t h) {
+    // fmix32 undone step by step: x ^ x >> 16 is its own inverse, x ^ x >> 13
+    // is undone by y ^ y >> 13 ^ y >> 26, and each multiplier by its inverse
+    // modulo 2^32 (0x85ebca6b * 0xa5cb9243 = 0xc2b2ae35 * 0x7ed1b41d = 1 mod 2^32)
+    h ^= h >> 16;
+    h *= 0x7ed1b41du;
+    h ^= (h >> 13) ^ (h >> 26);
+    h *= 0xa5cb9243u;
+    h ^= h >> 16;
+    return h;
+}
+
 }  // namespace aj

@@ -26,8 +26,26 @@
 // (common_functions.h:57-59) would turn dense keys into one long run under
 // linear probing.
 //
-// Bytes per tuple: R read 8 + slot write 16; S read 8 + slot read 16 (one
-// slot per probe for unique keys at half load).
+// Building: random device-wide atomics run at about 13 G CAS/s over a table of
+// this size (each is a round trip to memory), while the same table written
+// out sequentially from LDS takes about a tenth of that time.  So for
+// counting joins of large R the table is cut into regions of 4096 or 8192
+// slots (NpoGeom: the hash picks the region and the slot in it, probing wraps
+// inside the region), R is partitioned by hashed key into the regions with
+// the staged two passes (keys replaced by fmix32(key) in pass 1: a bijection,
+// undone here), and one work-group per region builds it in LDS -- inserts and
+// aggregates are LDS atomics -- and writes all its slots out with coalesced
+// 16-byte stores, so no separate fill of the table is needed.  The
+// partitions are sized far below a region's slots, so a region never fills;
+// a partition that overflows its capacity (heavy duplicates) sets a flag, and
+// the flat build then runs on the device in its place (its kernels return at
+// once when the flag is clear) and the probe reads the flat geometry.  The
+// probe is the same either way: one random slot per S tuple.
+//
+// Bytes per tuple: R read 8 + slot write 16 (flat build), or R read 8 +
+// partitions written 8 + read 8 + 32 of table slots (region build, every slot
+// written once); S read 8 + slot read 16 (one slot per probe for unique keys
+// at half load).
 #include "aj_internal.h"
 #include "device_common.h"
 
@@ -88,8 +106,13 @@ int fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st) {
     return AJ_OK;
 }
 
-// Empty slots: word 0 all ones, aggregate 0.
-__global__ __launch_bounds__(NBLOCK) void k_fill_slots(uint4 *t, uint64_t n) {
+// Empty slots: word 0 all ones, aggregate 0.  With only_if, only when the
+// region build overflowed (*only_if set), and then the sentinel count that
+// build may have added is cleared too.
+__global__ __launch_bounds__(NBLOCK) void k_fill_slots(uint4 *t, uint64_t n,
+                                                     const uint32_t *only_if, uint64_t *acc) {
+    if (only_if && !*only_if) return;
+    if (only_if && blockIdx.x == 0 && threadIdx.x == 0) acc[2] = 0;
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
          i += (uint64_t)gridDim.x * blockDim.x)
         t[i] = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u);
@@ -116,26 +139,34 @@ __device__ __forceinline__ void npo_insert(uint64_t i, uint64_t v, uint32_t k, u
     }
 }
 
-// Matches of key k in the table's slot for k: calls agg(first tuple, count of
-// further tuples, sum of their payloads) once if k has a slot.
+// Matches of key k in the table's slot for k (geometry g): calls agg(first
+// tuple, count of further tuples, sum of their payloads) once if k has a
+// slot, and returns the slot's index (~0 if none).
 template <class F>
-__device__ __forceinline__ uint64_t npo_find(uint32_t k, const uint64_t *tab, uint64_t mask,
+__device__ __forceinline__ uint64_t npo_find(uint32_t k, const uint64_t *tab, NpoGeom g,
                                              F &&agg) {
     typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
-    for (uint64_t h = fmix32(k) & mask;; h = (h + 1) & mask) {
-        const u64x2 w = *reinterpret_cast<const u64x2 *>(tab + 2 * h);  // one 16-byte slot
+    const uint32_t h = fmix32(k);
+    const uint64_t region = ((uint64_t)(h & ((1u << g.b1) - 1u)) << g.b2) |
+                            ((h >> g.b1) & ((1u << g.b2) - 1u));
+    const uint64_t rmask = (1ull << g.rs_log2) - 1, base = region << g.rs_log2;
+    for (uint64_t s = (h >> (g.b1 + g.b2)) & rmask;; s = (s + 1) & rmask) {
+        const uint64_t i = base | s;
+        const u64x2 w = *reinterpret_cast<const u64x2 *>(tab + 2 * i);  // one 16-byte slot
         if (w.x == EMPTY_SLOT) return ~0ull;
         if (key_of(w.x) == k) {
             agg(w.x, (uint32_t)w.y, (uint32_t)(w.y >> 32));
-            return h;
+            return i;
         }
     }
 }
 
+// The flat build (with only_if: only when the region build overflowed).
 __global__ __launch_bounds__(NBLOCK) void k_npo_build(const uint64_t *R, uint64_t n,
                                                     uint64_t *tab, uint64_t mask,
                                                     uint32_t *head, uint32_t *next,
-                                                    uint64_t *acc) {
+                                                    uint64_t *acc, const uint32_t *only_if) {
+    if (only_if && !*only_if) return;
     uint64_t sentinels = 0;
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
          i += (uint64_t)gridDim.x * blockDim.x) {
@@ -149,11 +180,76 @@ __global__ __launch_bounds__(NBLOCK) void k_npo_build(const uint64_t *R, uint64_
     if (sentinels) atomicAdd((unsigned long long *)&acc[2], (unsigned long long)sentinels);
 }
 
+// Region build: work-group p builds region p from partition p of the hashed
+// R tuples (parts + p * fcap, fcur[p] of them) -- the region's slots in LDS,
+// each tuple's key recovered from its hash, inserts with LDS atomics as in
+// npo_insert -- then writes every slot of the region out.  Returns at once when
+// the staged partitioning overflowed (*flag); sets *flag itself if a region
+// would be more than 15/16 full (the partition capacities keep regions under
+// 5/8, so this guards the probe's termination, it does not happen).
+template <int RS_LOG2, int RBLOCK>
+__global__ __launch_bounds__(RBLOCK) void k_npo_region(const uint64_t *parts, const uint32_t *fcur,
+                                                     uint32_t fcap, uint32_t rbits,
+                                                     uint64_t *tab, uint64_t *acc,
+                                                     uint32_t *flag) {
+    constexpr uint32_t RS = 1u << RS_LOG2;
+    extern __shared__ uint64_t lds[];  // RS slots of 2 words
+    __shared__ uint32_t s_sent, s_new, s_bad;
+    if (*flag) return;
+    for (uint32_t i = threadIdx.x; i < RS; i += RBLOCK)
+        reinterpret_cast<uint4 *>(lds)[i] = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u);
+    if (threadIdx.x == 0) s_sent = s_new = s_bad = 0;
+    __syncthreads();
+    const uint32_t p = blockIdx.x;
+    const uint32_t n = min(fcur[p], fcap);
+    const uint64_t *in = parts + (uint64_t)p * fcap;
+    uint32_t sent = 0, fresh = 0, bad = 0;
+    for (uint32_t i = threadIdx.x; i < n; i += RBLOCK) {
+        const uint64_t t = in[i];
+        const uint32_t h = key_of(t), k = fmix32_inv(h);
+        const uint64_t v = ((uint64_t)k << 32) | (uint32_t)t;
+        if (v == EMPTY_SLOT) {
+            ++sent;
+            continue;
+        }
+        uint32_t s = (h >> rbits) & (RS - 1);
+        for (uint32_t j = 0;; ++j, s = (s + 1) & (RS - 1)) {
+            if (j == RS) {  // region full
+                bad = 1;
+                break;
+            }
+            const uint64_t old = atomicCAS((unsigned long long *)&lds[2 * s],
+                                           (unsigned long long)EMPTY_SLOT, (unsigned long long)v);
+            if (old == EMPTY_SLOT) {
+                ++fresh;
+                break;
+            }
+            if (key_of(old) == k) {
+                atomicAdd((unsigned long long *)&lds[2 * s + 1],
+                          (unsigned long long)(((uint64_t)(uint32_t)t << 32) | 1u));
+                break;
+            }
+        }
+    }
+    if (sent) atomicAdd(&s_sent, sent);
+    if (fresh) atomicAdd(&s_new, fresh);
+    if (bad) s_bad = 1;
+    __syncthreads();
+    if (s_bad || s_new > RS - RS / 16) {
+        if (threadIdx.x == 0) atomicOr(flag, 1u);
+        return;
+    }
+    uint4 *out = reinterpret_cast<uint4 *>(tab) + ((uint64_t)p << RS_LOG2);
+    for (uint32_t i = threadIdx.x; i < RS; i += RBLOCK) out[i] = reinterpret_cast<const uint4 *>(lds)[i];
+    if (threadIdx.x == 0 && s_sent)
+        atomicAdd((unsigned long long *)&acc[2], (unsigned long long)s_sent);
+}
+
 // Count and checksum of S tuple s: O(1) from the key's aggregate.
-__device__ __forceinline__ void npo_count_one(uint64_t s, const uint64_t *tab, uint64_t mask,
+__device__ __forceinline__ void npo_count_one(uint64_t s, const uint64_t *tab, NpoGeom g,
                                               uint64_t nsent, uint64_t &cnt, uint64_t &sum) {
     const uint32_t k = key_of(s);
-    npo_find(k, tab, mask, [&](uint64_t first, uint32_t more, uint32_t more_sum) {
+    npo_find(k, tab, g, [&](uint64_t first, uint32_t more, uint32_t more_sum) {
         const uint64_t c = 1ull + more;
         const uint32_t rsum = payload_of(first) + more_sum;  // R payloads, mod 2^32
         cnt += c;
@@ -167,7 +263,7 @@ __device__ __forceinline__ void npo_count_one(uint64_t s, const uint64_t *tab, u
 
 // Every match of S tuple s written from wpos on (the first tuple, then the
 // chain of further ones), as pairs, S records or timed records.
-__device__ __forceinline__ void npo_write_one(uint64_t s, const uint64_t *tab, uint64_t mask,
+__device__ __forceinline__ void npo_write_one(uint64_t s, const uint64_t *tab, NpoGeom g,
                                               const uint64_t *R, const uint32_t *head,
                                               const uint32_t *next, uint64_t nsent,
                                               uint64_t *pairs, uint64_t &wpos, uint64_t cap,
@@ -179,22 +275,29 @@ __device__ __forceinline__ void npo_write_one(uint64_t s, const uint64_t *tab, u
         ++wpos;
     };
     const uint32_t k = key_of(s);
-    const uint64_t h = npo_find(k, tab, mask, [&](uint64_t first, uint32_t, uint32_t) { emit(first); });
+    const uint64_t h = npo_find(k, tab, g, [&](uint64_t first, uint32_t, uint32_t) { emit(first); });
     if (h != ~0ull)
         for (uint32_t j = head[h]; j; j = next[j - 1]) emit(R[j - 1]);
     if (k == 0xFFFFFFFFu)
         for (uint64_t q = 0; q < nsent; ++q) emit(EMPTY_SLOT);
 }
 
+// The probe; with flag, the table has the flat geometry instead of g when the
+// region build overflowed (*flag set).
 template <bool PAIRS>
 __global__ __launch_bounds__(NBLOCK) void k_npo_probe(const uint64_t *S, uint64_t n,
-                                                    const uint64_t *tab, uint64_t mask,
-                                                    const uint64_t *R, const uint32_t *head,
-                                                    const uint32_t *next, uint64_t *acc,
-                                                    uint64_t *pairs, uint64_t cap, int records) {
+                                                    const uint64_t *tab, NpoGeom g,
+                                                    const uint32_t *flag, const uint64_t *R,
+                                                    const uint32_t *head, const uint32_t *next,
+                                                    uint64_t *acc, uint64_t *pairs, uint64_t cap,
+                                                    int records) {
     __shared__ uint64_t red[NBLOCK / 64];
     __shared__ uint64_t s_base;
     const uint64_t nsent = acc[2];
+    if (flag && *flag) {
+        g.rs_log2 += g.b1 + g.b2;
+        g.b1 = g.b2 = 0;
+    }
     uint64_t cnt = 0, sum = 0;
     // uniform trip count per block: every thread reaches the block scans
     for (uint64_t base = blockIdx.x * (uint64_t)NBLOCK; base < n;
@@ -203,7 +306,7 @@ __global__ __launch_bounds__(NBLOCK) void k_npo_probe(const uint64_t *S, uint64_
         const bool valid = i < n;
         const uint64_t s = valid ? S[i] : 0;
         uint64_t c = 0, x = 0;
-        if (valid) npo_count_one(s, tab, mask, nsent, c, x);
+        if (valid) npo_count_one(s, tab, g, nsent, c, x);
         if (PAIRS) {
             uint64_t total;
             const uint64_t pre = block_exclusive_scan<NBLOCK>(c, red, &total);
@@ -214,7 +317,7 @@ __global__ __launch_bounds__(NBLOCK) void k_npo_probe(const uint64_t *S, uint64_
                 uint64_t wpos = s_base + pre;
                 const uint32_t tnow =
                     records == 2 ? (uint32_t)__builtin_amdgcn_s_memrealtime() : 0u;
-                npo_write_one(s, tab, mask, R, head, next, nsent, pairs, wpos, cap, records, tnow);
+                npo_write_one(s, tab, g, R, head, next, nsent, pairs, wpos, cap, records, tnow);
             }
             __syncthreads();
         }
@@ -248,6 +351,17 @@ __global__ __launch_bounds__(NBLOCK) void k_npo_reduce(uint64_t *acc, unsigned n
 
 size_t npo_acc_words(int num_cu) { return 8 + 2 * (size_t)num_cu * 16; }
 
+template <int RS_LOG2, int RBLOCK>
+static int launch_region(const NpoDesc &d, uint32_t *flag, hipStream_t st) {
+    const StagedGeom &g = d.stage->g;
+    const size_t lds = (size_t)16 << RS_LOG2;
+    AJ_TRY(lds_limit_set((const void *)k_npo_region<RS_LOG2, RBLOCK>, lds));
+    hipLaunchKernelGGL((k_npo_region<RS_LOG2, RBLOCK>), dim3(1u << (g.b1 + g.b2)), dim3(RBLOCK),
+                       lds, st, (const uint64_t *)d.stage->out[0], (const uint32_t *)d.stage->fcur,
+                       g.fcap[0], (uint32_t)(g.b1 + g.b2), d.table, d.acc, flag);
+    return AJ_OK;
+}
+
 int run_npo(const NpoDesc &d, hipStream_t st, const Marks *mk, int m0) {
     if (d.capacity < 2 || (d.capacity & (d.capacity - 1)) || d.capacity <= d.nR) {
         set_error(AJ_ERR_ARG, "NPO table capacity must be a power of two above |R|");
@@ -257,26 +371,57 @@ int run_npo(const NpoDesc &d, hipStream_t st, const Marks *mk, int m0) {
         set_error(AJ_ERR_ARG, "NPO with pairs needs the chain arrays");
         return AJ_ERR_ARG;
     }
-    hipLaunchKernelGGL(k_fill_slots, dim3(grid_for(d.capacity, 256)), dim3(NBLOCK), 0, st,
-                       reinterpret_cast<uint4 *>(d.table), d.capacity);
+    int cap_log2 = 0;
+    while ((1ull << cap_log2) < d.capacity) ++cap_log2;
+    NpoGeom flat;
+    flat.rs_log2 = (uint32_t)cap_log2;
+    NpoGeom geom = flat;
+    uint32_t *flag = nullptr;
+    if (d.stage) {
+        const StagedGeom &g = d.stage->g;
+        if (d.pairs || d.stage->n[0] != d.nR || d.stage->n[1] != 0 || (g.cache_mode & 16) == 0 ||
+            d.geom.b1 != (uint32_t)g.b1 || d.geom.b2 != (uint32_t)g.b2 ||
+            (d.geom.rs_log2 != 12 && d.geom.rs_log2 != 13) ||
+            d.geom.b1 + d.geom.b2 + d.geom.rs_log2 != (uint32_t)cap_log2 ||
+            d.stage->flag != reinterpret_cast<uint32_t *>(d.acc + 4)) {
+            set_error(AJ_ERR_ARG, "NPO region build: geometry does not match the table");
+            return AJ_ERR_ARG;
+        }
+        geom = d.geom;
+        flag = d.stage->flag;
+    }
     uint32_t *head = d.pairs ? d.head : nullptr;  // chains only when matches are written
-    if (head) AJ_TRY(fill_u32(head, d.capacity, 0, st));
-    const uint64_t mask = d.capacity - 1;
     if (mk) mk->mark(m0);
+    if (d.stage) {
+        // hashed partitioning into the regions, the regions in LDS, and the flat
+        // build in their place if a partition overflowed (returns at once if not)
+        AJ_TRY(run_staged(*d.stage, st));
+        if (d.geom.rs_log2 == 12)
+            AJ_TRY((launch_region<12, 512>(d, flag, st)));
+        else
+            AJ_TRY((launch_region<13, 1024>(d, flag, st)));
+    }
+    hipLaunchKernelGGL(k_fill_slots, dim3(grid_for(d.capacity, 256)), dim3(NBLOCK), 0, st,
+                       reinterpret_cast<uint4 *>(d.table), d.capacity, (const uint32_t *)flag,
+                       d.acc);
+    if (head) AJ_TRY(fill_u32(head, d.capacity, 0, st));
     if (d.nR)
         hipLaunchKernelGGL(k_npo_build, dim3(grid_for(d.nR, d.num_cu)), dim3(NBLOCK), 0, st, d.R,
-                           d.nR, d.table, mask, head, d.next, d.acc);
+                           d.nR, d.table, d.capacity - 1, head, d.next, d.acc,
+                           (const uint32_t *)flag);
     if (mk) mk->mark(m0 + 1);
     if (d.nS) {
         const unsigned g = grid_for(d.nS, d.num_cu);  // <= num_cu * 16 (npo_acc_words)
         if (d.pairs)
             hipLaunchKernelGGL(k_npo_probe<true>, dim3(g), dim3(NBLOCK), 0, st, d.S, d.nS,
-                               (const uint64_t *)d.table, mask, d.R, (const uint32_t *)head,
-                               (const uint32_t *)d.next, d.acc, d.pairs, d.pairs_cap, d.records);
+                               (const uint64_t *)d.table, flat, (const uint32_t *)nullptr, d.R,
+                               (const uint32_t *)head, (const uint32_t *)d.next, d.acc, d.pairs,
+                               d.pairs_cap, d.records);
         else
             hipLaunchKernelGGL(k_npo_probe<false>, dim3(g), dim3(NBLOCK), 0, st, d.S, d.nS,
-                               (const uint64_t *)d.table, mask, d.R, (const uint32_t *)nullptr,
-                               (const uint32_t *)nullptr, d.acc, nullptr, (uint64_t)0, 0);
+                               (const uint64_t *)d.table, geom, (const uint32_t *)flag, d.R,
+                               (const uint32_t *)nullptr, (const uint32_t *)nullptr, d.acc,
+                               nullptr, (uint64_t)0, 0);
         hipLaunchKernelGGL(k_npo_reduce, dim3(1), dim3(NBLOCK), 0, st, d.acc, g);
     }
     if (mk) mk->mark(m0 + 2);

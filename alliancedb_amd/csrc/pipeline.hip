@@ -563,8 +563,37 @@ int npo_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
                uint64_t *pairs, uint64_t pairs_cap, int records, aj_join_out_t *out,
                hipStream_t st) {
     uint64_t cap = 2;  // load at most 1/2: one slot per distinct key
-    while (cap < 2 * nR) cap <<= 1;
+    int cap_log2 = 1;
+    while (cap < 2 * nR) cap <<= 1, ++cap_log2;
     AJ_TRY(ctx->table.ensure(cap * 16));
+    // Region build (npo.hip) for counting joins of at least npo_region_min R
+    // tuples whose table splits into 4 to 2^16 regions of 4096 slots (8192 for
+    // a table of 2^29 slots): R partitioned by hashed key with the staged passes.
+    const int rs_log2 = cap_log2 <= 28 ? 12 : 13, rbits = cap_log2 - rs_log2;
+    const uint64_t rmin = npo_region_min();
+    const bool region = !pairs && rmin && nR >= rmin && rbits >= 2 && rbits <= 16;
+    StagedDesc sd;
+    if (region) {
+        const int b1 = (rbits + 1) / 2, b2 = rbits / 2;
+        AJ_TRY(staged_geometry(&sd.g, nR, nR, b1, b2, 0, 4, 1024, ctx->num_cu));
+        sd.g.cache_mode = 9 | 16;  // keys hashed in pass 1
+        const uint32_t F1 = 1u << b1, F = F1 << b2;
+        Layout L;
+        const size_t o_scur = L.add<uint32_t>(2ull * sd.g.nset * F1);
+        const size_t o_fcur = L.add<uint32_t>(2ull * F);
+        AJ_TRY(ctx->plan.ensure(L.off));
+        AJ_TRY(ctx->stage.ensure(sd.g.stage_tuples * 8));
+        AJ_TRY(ctx->p2R.ensure(sd.g.out_tuples[0] * 8));
+        sd.in[0] = dR;
+        sd.in[1] = nullptr;
+        sd.n[0] = nR;
+        sd.n[1] = 0;
+        sd.stage = ctx->stage.as<uint64_t>();
+        sd.out[0] = ctx->p2R.as<uint64_t>();
+        sd.out[1] = nullptr;
+        sd.scur = at<uint32_t>(ctx->plan, o_scur);
+        sd.fcur = at<uint32_t>(ctx->plan, o_fcur);
+    }
     if (pairs) {  // chains of the further tuples of a key, to write every match
         AJ_TRY(ctx->chain.ensure(nR * 4 + 4));
         AJ_TRY(ctx->heads.ensure(cap * 4));
@@ -583,6 +612,13 @@ int npo_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
     d.head = pairs ? ctx->heads.as<uint32_t>() : nullptr;
     d.next = pairs ? ctx->chain.as<uint32_t>() : nullptr;
     d.acc = acc;
+    if (region) {
+        sd.flag = reinterpret_cast<uint32_t *>(acc + 4);
+        d.stage = &sd;
+        d.geom.b1 = (uint32_t)sd.g.b1;
+        d.geom.b2 = (uint32_t)sd.g.b2;
+        d.geom.rs_log2 = (uint32_t)rs_log2;
+    }
     d.pairs = pairs;
     d.pairs_cap = pairs_cap;
     d.records = records;
@@ -597,7 +633,7 @@ int npo_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
     }
     AJ_TRY(run_npo(d, st, mkp, AJ_PH_NPO_BUILD));
     AJ_HIP(record_event(ctx->ev1, st));
-    uint64_t host[4];
+    uint64_t host[5];
     AJ_HIP(hipMemcpyAsync(host, acc, sizeof(host), hipMemcpyDeviceToHost, st));
     AJ_HIP(hipStreamSynchronize(st));
     const float ms = elapsed_ms(ctx->ev0, ctx->ev1);
@@ -605,6 +641,7 @@ int npo_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
     out->checksum = host[1];
     out->npairs = pairs ? host[3] : 0;
     out->device_ms = ms;
+    out->flags = region ? ((uint32_t)host[4] ? AJ_OUT_FALLBACK : AJ_OUT_STAGED) : 0u;
     if (mkp) read_phases(ctx, out, AJ_PH_NPO_BUILD, AJ_PH_NPO_PROBE + 1);
     return AJ_OK;
 }

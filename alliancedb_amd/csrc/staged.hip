@@ -44,6 +44,13 @@ namespace aj {
 
 constexpr int STAGE_MAXF = 256;  // digits per pass (8 bits)
 
+// Keys replaced by their murmur3 finalisation (a bijection, undone by
+// fmix32_inv): the hashed partitioning of NPO's region build.
+template <int N> __device__ __forceinline__ void hash_keys(uint64_t (&t)[N]) {
+#pragma unroll
+    for (int j = 0; j < N; ++j) t[j] = ((uint64_t)fmix32(key_of(t[j])) << 32) | (uint32_t)t[j];
+}
+
 // Where the runs of a tile go: region d starts at out index base + d * stride,
 // holds cap tuples and has filled *cur[d] of them.
 struct StageRes {
@@ -59,7 +66,7 @@ struct StageRes {
 // Wave 0 scans the tile's F counters and issues one atomic add per non-empty
 // digit; the returned offsets are needed only for the write phase, so they are
 // turned into write bases after wave 0 has placed its own tuples.
-template <int PBLOCK, bool NT_LOAD, bool NT_STORE>
+template <int PBLOCK, bool NT_LOAD, bool NT_STORE, bool HASH = false>
 __device__ __forceinline__ void stage_tiles(const uint64_t *in, uint64_t begin, uint64_t end,
                                             uint64_t *out, int shift, uint32_t F, uint64_t *buf,
                                             uint32_t *cnt, uint32_t *dst, uint32_t *s_len,
@@ -74,6 +81,7 @@ __device__ __forceinline__ void stage_tiles(const uint64_t *in, uint64_t begin, 
     uint64_t t[PITEMS];
     bool v[PITEMS];
     load_tile<PBLOCK, PITEMS, NT_LOAD>(in, 8, base, begin, end, t, v);
+    if constexpr (HASH) hash_keys<PITEMS>(t);
     for (;;) {
         uint32_t rank2[PITEMS / 2];
 #pragma unroll
@@ -134,7 +142,10 @@ __device__ __forceinline__ void stage_tiles(const uint64_t *in, uint64_t begin, 
         const uint32_t len = *s_len;
         const uint64_t nbase = base + PTILE;
         const bool more = nbase < end;
-        if (more) load_tile<PBLOCK, PITEMS, NT_LOAD>(in, 8, nbase, begin, end, t, v);
+        if (more) {
+            load_tile<PBLOCK, PITEMS, NT_LOAD>(in, 8, nbase, begin, end, t, v);
+            if constexpr (HASH) hash_keys<PITEMS>(t);
+        }
 #pragma unroll
         for (int k0 = 0; k0 < PITEMS; k0 += WB) {
             uint64_t u[WB];
@@ -181,7 +192,8 @@ struct Stage1Args {
     uint32_t *flag;
 };
 
-// MODE bits: 1 = input loads nontemporal, 2 = staging stores nontemporal
+// MODE bits: 1 = input loads nontemporal, 2 = staging stores nontemporal,
+// 16 = keys replaced by fmix32(key) as they are loaded
 template <int PBLOCK, int MODE>
 __global__ __launch_bounds__(PBLOCK, 4) void k_stage1(Stage1Args a) {
     constexpr int PTILE = PBLOCK * PITEMS;
@@ -204,8 +216,8 @@ __global__ __launch_bounds__(PBLOCK, 4) void k_stage1(Stage1Args a) {
     r.cap = a.scap;
     r.trash = (uint32_t)a.nset * F * a.scap;
     r.flag = a.flag;
-    stage_tiles<PBLOCK, (MODE & 1) != 0, (MODE & 2) != 0>(a.in, b, e, a.stage, 0, F, buf, cnt, dst,
-                                                          &s_len, r);
+    stage_tiles<PBLOCK, (MODE & 1) != 0, (MODE & 2) != 0, (MODE & 16) != 0>(
+        a.in, b, e, a.stage, 0, F, buf, cnt, dst, &s_len, r);
 }
 
 struct Stage2Args {
@@ -344,8 +356,9 @@ static int launch_staged(const StagedDesc &d, hipStream_t st, std::vector<hipEve
     const StagedGeom &g = d.g;
     const uint32_t F1 = 1u << g.b1, F2 = 1u << g.b2;
     const size_t lds1 = stage_lds_bytes(PBLOCK, F1), lds2 = stage_lds_bytes(PBLOCK, F2);
+    constexpr int MODE2 = MODE & 15;  // pass 2 reads keys as pass 1 wrote them
     AJ_TRY(lds_limit_set((const void *)k_stage1<PBLOCK, MODE>, lds1));
-    AJ_TRY(lds_limit_set((const void *)k_stage2<PBLOCK, MODE>, lds2));
+    AJ_TRY(lds_limit_set((const void *)k_stage2<PBLOCK, MODE2>, lds2));
     const uint32_t nreg = (uint32_t)g.nset * F1;
     int k = 0, ev = 0;
     for (int rel = 0; rel < 2; ++rel) {
@@ -381,8 +394,8 @@ static int launch_staged(const StagedDesc &d, hipStream_t st, std::vector<hipEve
             a2.fcap = g.fcap[rel];
             a2.fcur = d.fcur + (uint64_t)rel * (F1 * F2);
             a2.flag = d.flag;
-            hipLaunchKernelGGL((k_stage2<PBLOCK, MODE>), dim3(nreg * g.units), dim3(PBLOCK), lds2, st,
-                               a2);
+            hipLaunchKernelGGL((k_stage2<PBLOCK, MODE2>), dim3(nreg * g.units), dim3(PBLOCK), lds2,
+                               st, a2);
         }
     }
     if (evs) (void)hipEventRecord((*evs)[ev++], st);
@@ -408,12 +421,13 @@ int run_staged(const StagedDesc &d, hipStream_t st, std::vector<hipEvent_t> *evs
     hipLaunchKernelGGL(k_stage_zero, dim3(64), dim3(256), 0, st, d.scur, nz);
     hipLaunchKernelGGL(k_stage_zero, dim3(64), dim3(256), 0, st, d.fcur, 2ull * F);
     hipLaunchKernelGGL(k_stage_zero, dim3(1), dim3(64), 0, st, d.flag, (uint64_t)1);
-    switch (g.block * 16 + g.cache_mode) {
+    switch (g.block * 32 + g.cache_mode) {
 #define AJ_STAGED_CASE(B, M) \
-    case B * 16 + M: return launch_staged<B, M>(d, st, evs);
+    case B * 32 + M: return launch_staged<B, M>(d, st, evs);
         AJ_STAGED_CASE(1024, 0) AJ_STAGED_CASE(1024, 1) AJ_STAGED_CASE(1024, 5)
         AJ_STAGED_CASE(1024, 9) AJ_STAGED_CASE(1024, 13) AJ_STAGED_CASE(1024, 15)
         AJ_STAGED_CASE(512, 1) AJ_STAGED_CASE(512, 5) AJ_STAGED_CASE(512, 9)
+        AJ_STAGED_CASE(1024, 25)
 #undef AJ_STAGED_CASE
         default:
             set_error(AJ_ERR_ARG, "staged partitioning: no kernels for block %d, cache mode %d",

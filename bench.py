@@ -37,20 +37,25 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 PRO_BYTES_PER_TUPLE = 32  # histogram read 8 + scatter read 8 + write 8 + build/probe read 8
-NPO_BYTES_PER_TUPLE = 16  # R read 8 + slot write 8 / S read 8 + slot read 8
-SHJ_BYTES_PER_TUPLE = 24  # read 8 + own-table slot write 8 + other-table slot read 8
+# NPO per R tuple: read 8 + 32 of table slots written (16-byte slots at half
+# load); per S tuple: read 8 + one 16-byte slot read.  Average over |R|+|S|.
+NPO_BYTES_PER_TUPLE = 32
+# SHJ per tuple: read 8 + its own table's slot 16 + a slot of the other table
+# 16 + 32 of its own table's slots initialised
+SHJ_BYTES_PER_TUPLE = 72
 WINDOW_BYTES_PER_TUPLE = 40  # 12 hist read + 12 scatter read + 8 write + 8 build/probe read
 # algorithmic bytes per input tuple of each kernel (8-byte tuples), credited for
 # the tuples that kernel reads: NPO's build reads R only, its probe S only.  With
 # the histogram passes pass 1's histogram kernel (k_hist2d) also counts the
 # pass-2 digits, and the "hist2" phase is only the pass-2 unit planning.
 KERNEL_BYTES = {"hist1": 8, "scatter1": 16, "hist2": 0, "scatter2": 16, "join": 8,
-                "npo_build": 16, "npo_probe": 16, "shj_step": 24}
+                "npo_build": 40, "npo_probe": 24, "shj_step": 72}
 KERNEL_SIDE = {"npo_build": "R", "npo_probe": "S"}  # others: |R| + |S|
 KERNEL_NAMES = {"hist1": "k_hist2d (pass 1)", "scatter1": "k_scatter_pipe (pass 1)",
                 "hist2": "pass-2 planning (k_p2_units, k_h2_usum, k_h2_base)",
                 "scatter2": "k_scatter_p2 (pass 2)",
-                "join": "k_join", "npo_build": "k_npo_build", "npo_probe": "k_npo_probe",
+                "join": "k_join", "npo_build": "NPO build (k_stage1 + k_stage2 hashing R into "
+                "the table's regions, k_npo_region)", "npo_probe": "k_npo_probe",
                 "shj_step": "k_shj_step (all eager steps)"}
 # the staged partitioning (aj_join_out_t.flags & AJ_OUT_STAGED): one pass-1 and one
 # pass-2 launch per super-chunk of STAGE_CHUNK tuples (the library's default)
@@ -309,7 +314,8 @@ def join_line(aj, args, workload_name, nR, nS, algo, out, dev, ms_step, phases, 
         "pipeline": {"device_ms": round(dev, 3), "alg_bytes_per_tuple": bpt,
                      "alg_GBps": round(dev_gbs, 1),
                      "frac_of_peak": round(dev_gbs / HBM_PEAK_GBS, 4),
-                     "partitioning": "staged" if staged else "histogram passes",
+                     "partitioning": ("region build" if staged else "flat build")
+                     if algo == aj.AJ_ALGO_NPO else "staged" if staged else "histogram passes",
                      "kernel_ms": {k: round(v, 4) for k, v in mean_ph.items() if v > 0.0005},
                      "kernel_ms_from": f"{args.steps} further joins with HIP events around "
                                        f"every launch (device {ev_dev:.3f} ms per join with "

@@ -334,7 +334,10 @@ typedef struct aj_join_out_t {
                             from the end of the pair buffer backwards */
     uint32_t flags;      /* PRO: AJ_OUT_STAGED when the staged partitioning produced
                             the result, AJ_OUT_FALLBACK when a staged attempt overflowed
-                            and the histogram passes joined again (device_ms covers both) */
+                            and the histogram passes joined again (device_ms covers both).
+                            NPO: AJ_OUT_STAGED when the table was built region by region,
+                            AJ_OUT_FALLBACK when that overflowed and the flat build ran
+                            in its place (on the device); 0 for the flat build alone. */
 } aj_join_out_t;
 
 enum aj_out_flags { AJ_OUT_STAGED = 1, AJ_OUT_FALLBACK = 2 };
