@@ -193,41 +193,54 @@ __global__ __launch_bounds__(RBLOCK) void k_npo_region(const uint64_t *parts, co
                                                      uint64_t *tab, uint64_t *acc,
                                                      uint32_t *flag) {
     constexpr uint32_t RS = 1u << RS_LOG2;
+    constexpr int B = 4;  // tuples loaded per lane before their inserts: loads in flight
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
     extern __shared__ uint64_t lds[];  // RS slots of 2 words
     __shared__ uint32_t s_sent, s_new, s_bad;
     if (*flag) return;
+    u32x4 *slots = reinterpret_cast<u32x4 *>(lds);
     for (uint32_t i = threadIdx.x; i < RS; i += RBLOCK)
-        reinterpret_cast<uint4 *>(lds)[i] = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u);
+        slots[i] = u32x4{0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u};
     if (threadIdx.x == 0) s_sent = s_new = s_bad = 0;
     __syncthreads();
     const uint32_t p = blockIdx.x;
     const uint32_t n = min(fcur[p], fcap);
     const uint64_t *in = parts + (uint64_t)p * fcap;
     uint32_t sent = 0, fresh = 0, bad = 0;
-    for (uint32_t i = threadIdx.x; i < n; i += RBLOCK) {
-        const uint64_t t = in[i];
-        const uint32_t h = key_of(t), k = fmix32_inv(h);
-        const uint64_t v = ((uint64_t)k << 32) | (uint32_t)t;
-        if (v == EMPTY_SLOT) {
-            ++sent;
-            continue;
+    for (uint32_t i0 = 0; i0 < n; i0 += RBLOCK * B) {
+        uint64_t t[B];
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            const uint32_t i = i0 + b * RBLOCK + threadIdx.x;
+            t[b] = i < n ? in[i] : 0;
         }
-        uint32_t s = (h >> rbits) & (RS - 1);
-        for (uint32_t j = 0;; ++j, s = (s + 1) & (RS - 1)) {
-            if (j == RS) {  // region full
-                bad = 1;
-                break;
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            if (i0 + b * RBLOCK + threadIdx.x >= n) continue;
+            const uint32_t h = key_of(t[b]), k = fmix32_inv(h);
+            const uint64_t v = ((uint64_t)k << 32) | (uint32_t)t[b];
+            if (v == EMPTY_SLOT) {
+                ++sent;
+                continue;
             }
-            const uint64_t old = atomicCAS((unsigned long long *)&lds[2 * s],
-                                           (unsigned long long)EMPTY_SLOT, (unsigned long long)v);
-            if (old == EMPTY_SLOT) {
-                ++fresh;
-                break;
-            }
-            if (key_of(old) == k) {
-                atomicAdd((unsigned long long *)&lds[2 * s + 1],
-                          (unsigned long long)(((uint64_t)(uint32_t)t << 32) | 1u));
-                break;
+            uint32_t s = (h >> rbits) & (RS - 1);
+            for (uint32_t j = 0;; ++j, s = (s + 1) & (RS - 1)) {
+                if (j == RS) {  // region full
+                    bad = 1;
+                    break;
+                }
+                const uint64_t old = atomicCAS((unsigned long long *)&lds[2 * s],
+                                               (unsigned long long)EMPTY_SLOT,
+                                               (unsigned long long)v);
+                if (old == EMPTY_SLOT) {
+                    ++fresh;
+                    break;
+                }
+                if (key_of(old) == k) {
+                    atomicAdd((unsigned long long *)&lds[2 * s + 1],
+                              (unsigned long long)(((uint64_t)(uint32_t)t[b] << 32) | 1u));
+                    break;
+                }
             }
         }
     }
@@ -239,8 +252,9 @@ __global__ __launch_bounds__(RBLOCK) void k_npo_region(const uint64_t *parts, co
         if (threadIdx.x == 0) atomicOr(flag, 1u);
         return;
     }
-    uint4 *out = reinterpret_cast<uint4 *>(tab) + ((uint64_t)p << RS_LOG2);
-    for (uint32_t i = threadIdx.x; i < RS; i += RBLOCK) out[i] = reinterpret_cast<const uint4 *>(lds)[i];
+    // the table is read next by random probes, not from cache: stream it out
+    u32x4 *out = reinterpret_cast<u32x4 *>(tab) + ((uint64_t)p << RS_LOG2);
+    for (uint32_t i = threadIdx.x; i < RS; i += RBLOCK) __builtin_nontemporal_store(slots[i], out + i);
     if (threadIdx.x == 0 && s_sent)
         atomicAdd((unsigned long long *)&acc[2], (unsigned long long)s_sent);
 }
@@ -397,7 +411,7 @@ int run_npo(const NpoDesc &d, hipStream_t st, const Marks *mk, int m0) {
         // build in their place if a partition overflowed (returns at once if not)
         AJ_TRY(run_staged(*d.stage, st));
         if (d.geom.rs_log2 == 12)
-            AJ_TRY((launch_region<12, 512>(d, flag, st)));
+            AJ_TRY((launch_region<12, 1024>(d, flag, st)));
         else
             AJ_TRY((launch_region<13, 1024>(d, flag, st)));
     }
