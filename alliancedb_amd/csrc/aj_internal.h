@@ -89,7 +89,7 @@ struct Ctx {
     DevBuf p1R, p1S, p2R, p2S;  // partitioned copies
     DevBuf hist, scan_tmp, plan, seg, tasks, acc, pairs, table, misc;
     DevBuf hist2d;              // per-chunk (d1, d2) counts of a two-pass PRO
-    DevBuf table2, steps;       // SHJ: S-side table, per-step match counts
+    DevBuf steps;               // SHJ: per-step match counts
     DevBuf stage;               // staged PRO: staging regions
     DevBuf chain, chain2;       // NPO / SHJ with pairs: duplicate chain links (R, S)
     DevBuf heads, heads2;       // NPO / SHJ with pairs: chain heads per slot (R, S)

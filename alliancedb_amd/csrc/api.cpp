@@ -149,7 +149,7 @@ static void ctx_fini(Ctx *c) {
     (void)hipSetDevice(c->device);
     DevBuf *bufs[] = {&c->inR, &c->inS, &c->p1R, &c->p1S, &c->p2R, &c->p2S, &c->hist,
                       &c->scan_tmp, &c->plan, &c->seg, &c->tasks, &c->acc, &c->pairs,
-                      &c->table, &c->misc, &c->hist2d, &c->table2, &c->steps, &c->stage,
+                      &c->table, &c->misc, &c->hist2d, &c->steps, &c->stage,
                       &c->chain, &c->chain2, &c->heads, &c->heads2, &c->carry, &c->carry2};
     for (DevBuf *b : bufs) b->release();
     for (hipEvent_t e : c->evs) (void)hipEventDestroy(e);

@@ -52,8 +52,12 @@ def test_shj_golden_fixtures(aj):
         assert (got.count, got.checksum) == (e["count"], int(e["checksum"])), name
 
 
+# steps of more than 2^19 tuples (more than one per lane of the step grid) use
+# two separate tables instead of interleaved slot pairs
 @pytest.mark.parametrize("nR,nS,batch", [(300, 300, 1), (5000, 7000, 7), (100000, 60000, 1000),
-                                         (1 << 20, 1 << 20, 65536)])
+                                         (1 << 20, 1 << 20, 65536),
+                                         ((1 << 22) + 999, 1 << 22, 1 << 20),
+                                         (3 << 20, (1 << 20) + 5, 1 << 21)])
 def test_shj_progress(aj, oracle, nR, nS, batch):
     R = aj.gen_uniform(nR, 1, max(nR // 4, 1), 21)
     S = aj.gen_uniform(nS, 1, max(nR // 4, 1), 22)
@@ -218,3 +222,24 @@ def test_shj_timed_records_both_sides(aj, oracle, tmp_path):
     for sub in ("timestamps", "latency", "gaps"):
         v = np.array((res / sub / "SHJ_JM_NP_5.txt").read_text().split(), float)
         assert len(v) == got.count and np.all(np.diff(v) >= 0), sub
+
+
+@pytest.mark.parametrize("batch", [1 << 19, (1 << 19) + 1])  # interleaved, then separate tables
+def test_shj_pairs_both_table_layouts(aj, oracle, batch):
+    n = 3 << 19
+    R, S = aj.gen_uniform(n, 1, n // 8, 41), aj.gen_uniform(n, 1, n // 8, 42)
+    cnt, chk = oracle.aggregate(R, S)
+    dR, dS = aj.DeviceArray.from_host(R), aj.DeviceArray.from_host(S)
+    buf = aj.DeviceArray(8 * cnt)
+    print(f"case SHJ pairs batch={batch} matches={cnt}", flush=True)
+    try:
+        with aj.Context(0) as ctx:
+            out = ctx.join(dR.ptr, n, dS.ptr, n, algo=aj.AJ_ALGO_SHJ, batch=batch,
+                           pairs_ptr=buf.ptr, pairs_capacity=cnt)
+            got = np.sort(buf.to_host(np.uint64, cnt))
+    finally:
+        dR.free()
+        dS.free()
+        buf.free()
+    assert (out.count, out.checksum, out.npairs) == (cnt, chk, cnt)
+    assert np.array_equal(got, oracle.pairs_sortmerge(R, S))
