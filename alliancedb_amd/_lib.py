@@ -34,6 +34,7 @@ AJ_ALGO_PRO = 0
 # aj_join_out_t.flags (enum aj_out_flags)
 AJ_OUT_STAGED = 1
 AJ_OUT_FALLBACK = 2
+AJ_ACCESS_LOAD, AJ_ACCESS_CAS, AJ_ACCESS_LOAD_CAS = 0, 1, 2
 AJ_ALGO_NPO = 1
 AJ_ALGO_SHJ = 2
 
@@ -201,6 +202,8 @@ def _load() -> ctypes.CDLL:
         "aj_copy_d2h": (c.c_int, [P, P, c.c_uint64]),
         "aj_device_sync": (c.c_int, []),
         "aj_copy_d2d": (c.c_int, [P, P, c.c_uint64]),
+        "aj_access_kernel": (c.c_int, [c.c_int, P, c.c_uint64, c.c_uint64,
+                                       c.POINTER(c.c_float)]),
         "aj_batch_plan_create": (c.c_int, [P, c.c_uint64, c.c_uint64, c.c_int, c.c_uint64,
                                            c.POINTER(aj_batch_opts_t), c.POINTER(P)]),
         "aj_batch_plan_destroy": (c.c_int, [P]),
@@ -637,3 +640,10 @@ def gen_zipf(n: int, nkeys: int, theta: float, seed: int) -> np.ndarray:
     t = np.empty(n, dtype=TUPLE_DTYPE)
     check(lib.aj_gen_zipf(_ptr(t), n, nkeys, float(theta), seed))
     return t
+
+
+def access_rate(kind: int, table_ptr: int, slots: int, n: int) -> float:
+    """G operations/s of one aj_access_kernel launch (see aj.h)."""
+    ms = ctypes.c_float(0)
+    check(lib.aj_access_kernel(kind, table_ptr, slots, n, ctypes.byref(ms)))
+    return n / (ms.value * 1e-3) / 1e9

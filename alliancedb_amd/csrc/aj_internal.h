@@ -370,5 +370,8 @@ int enqueue_ranges(BatchPlan *p, const aj_stream_tuple_t *dR, uint64_t nR,
 int fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t st);
 int fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st);
 int copy_d2d(void *dst, const void *src, uint64_t bytes, int num_cu, hipStream_t st);
+// one random-access calibration launch (aj_access_kernel) over `slots` 16-byte slots
+int access_kernel(int kind, uint64_t *tab, uint64_t slots, uint64_t n, int num_cu,
+                  hipStream_t st);
 
 }  // namespace aj

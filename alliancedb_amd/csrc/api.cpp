@@ -775,6 +775,38 @@ int aj_copy_d2d(void *dst, const void *src, uint64_t bytes) {
     return AJ_OK;
 }
 
+int aj_access_kernel(int kind, void *table, uint64_t slots, uint64_t n, float *ms) {
+    clear_error();
+    if (!table) {
+        set_error(AJ_ERR_ARG, "NULL table");
+        return AJ_ERR_ARG;
+    }
+    int st = AJ_OK;
+    Ctx *c = default_ctx(&st);
+    if (!c) return st;
+    AJ_HIP(hipSetDevice(c->device));
+    uint64_t *tab = static_cast<uint64_t *>(table);
+    AJ_TRY(fill_u64(tab, 2 * slots, ~0ull, nullptr));
+    hipEvent_t a = nullptr, b = nullptr;
+    AJ_HIP(hipEventCreate(&a));
+    if (hipEventCreate(&b) != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return set_error(AJ_ERR_HIP, "hipEventCreate failed");
+    }
+    int rc = hipEventRecord(a, nullptr) == hipSuccess ? AJ_OK : AJ_ERR_HIP;
+    if (rc == AJ_OK) rc = access_kernel(kind, tab, slots, n, c->num_cu, nullptr);
+    if (rc == AJ_OK && hipEventRecord(b, nullptr) != hipSuccess) rc = AJ_ERR_HIP;
+    if (rc == AJ_OK && hipEventSynchronize(b) != hipSuccess) rc = AJ_ERR_HIP;
+    float t = 0;
+    if (rc == AJ_OK && hipEventElapsedTime(&t, a, b) != hipSuccess) rc = AJ_ERR_HIP;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (rc == AJ_ERR_HIP) return set_error(AJ_ERR_HIP, "aj_access_kernel: HIP event failure");
+    if (rc != AJ_OK) return rc;
+    if (ms) *ms = t;
+    return AJ_OK;
+}
+
 int aj_device_sync(void) {
     clear_error();
     AJ_TRY(ensure_default_device());

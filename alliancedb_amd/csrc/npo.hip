@@ -92,6 +92,44 @@ int copy_d2d(void *dst, const void *src, uint64_t bytes, int num_cu, hipStream_t
     return AJ_OK;
 }
 
+// Random-access calibration: lane i loads the 16-byte slot fmix32(i + 1) mod
+// the table size (KIND 0), claims slot fmix32((i + 1) ^ 0x5bd1e995) with a
+// 64-bit CAS of an empty word (KIND 1), or both at once (KIND 2: the pattern
+// of an SHJ step).  The loaded words are kept live through a store that the
+// sum never triggers.
+template <int KIND>
+__global__ __launch_bounds__(NBLOCK) void k_access(uint64_t *tab, uint64_t mask, uint64_t n) {
+    typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
+    uint64_t acc = 0;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t k = (uint32_t)(i + 1);
+        if (KIND != 1) {
+            const u64x2 w = *reinterpret_cast<const u64x2 *>(tab + 2 * (fmix32(k) & mask));
+            acc += w.x + w.y;
+        }
+        if (KIND != 0)
+            acc += atomicCAS((unsigned long long *)&tab[2 * (fmix32(k ^ 0x5bd1e995u) & mask)],
+                             (unsigned long long)EMPTY_SLOT,
+                             (unsigned long long)(((uint64_t)k << 32) | (uint32_t)i));
+    }
+    if (acc == 0x5A5A5A5A5A5A5A5Aull) tab[1] = acc;
+}
+
+int access_kernel(int kind, uint64_t *tab, uint64_t slots, uint64_t n, int num_cu,
+                  hipStream_t st) {
+    if (kind < 0 || kind > 2 || slots < 2 || (slots & (slots - 1))) {
+        set_error(AJ_ERR_ARG, "access kernel: kind 0..2 and a power-of-two slot count");
+        return AJ_ERR_ARG;
+    }
+    const dim3 g(grid_for(n, num_cu)), b(NBLOCK);
+    if (kind == 0) hipLaunchKernelGGL(k_access<0>, g, b, 0, st, tab, slots - 1, n);
+    if (kind == 1) hipLaunchKernelGGL(k_access<1>, g, b, 0, st, tab, slots - 1, n);
+    if (kind == 2) hipLaunchKernelGGL(k_access<2>, g, b, 0, st, tab, slots - 1, n);
+    AJ_HIP(hipGetLastError());
+    return AJ_OK;
+}
+
 int fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t st) {
     if (!n) return AJ_OK;
     hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n, 256)), dim3(NBLOCK), 0, st, p, n, v);
@@ -197,22 +235,33 @@ __global__ __launch_bounds__(RBLOCK) void k_npo_region(const uint64_t *parts, co
     typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
     extern __shared__ uint64_t lds[];  // RS slots of 2 words
     __shared__ uint32_t s_sent, s_new, s_bad;
-    if (*flag) return;
+    // the flag, the partition's size and its first tuples are loaded together
+    // (the tuple loads stay inside the partition whatever the flag says); the
+    // work-group returns once the flag is in
+    const uint32_t p = blockIdx.x;
+    const uint32_t flagged = *flag;
+    const uint32_t n = min(fcur[p], fcap);
+    const uint64_t *in = parts + (uint64_t)p * fcap;
+    uint64_t t[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const uint32_t i = b * RBLOCK + threadIdx.x;
+        t[b] = i < n ? in[i] : 0;
+    }
     u32x4 *slots = reinterpret_cast<u32x4 *>(lds);
     for (uint32_t i = threadIdx.x; i < RS; i += RBLOCK)
         slots[i] = u32x4{0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u};
     if (threadIdx.x == 0) s_sent = s_new = s_bad = 0;
+    if (flagged) return;  // uniform: every lane read the same word
     __syncthreads();
-    const uint32_t p = blockIdx.x;
-    const uint32_t n = min(fcur[p], fcap);
-    const uint64_t *in = parts + (uint64_t)p * fcap;
     uint32_t sent = 0, fresh = 0, bad = 0;
     for (uint32_t i0 = 0; i0 < n; i0 += RBLOCK * B) {
-        uint64_t t[B];
+        if (i0) {
 #pragma unroll
-        for (int b = 0; b < B; ++b) {
-            const uint32_t i = i0 + b * RBLOCK + threadIdx.x;
-            t[b] = i < n ? in[i] : 0;
+            for (int b = 0; b < B; ++b) {
+                const uint32_t i = i0 + b * RBLOCK + threadIdx.x;
+                t[b] = i < n ? in[i] : 0;
+            }
         }
 #pragma unroll
         for (int b = 0; b < B; ++b) {
@@ -244,8 +293,12 @@ __global__ __launch_bounds__(RBLOCK) void k_npo_region(const uint64_t *parts, co
             }
         }
     }
-    if (sent) atomicAdd(&s_sent, sent);
-    if (fresh) atomicAdd(&s_new, fresh);
+    sent = wave_sum(sent);
+    fresh = wave_sum(fresh);
+    if (lane_id() == 0) {
+        if (sent) atomicAdd(&s_sent, sent);
+        if (fresh) atomicAdd(&s_new, fresh);
+    }
     if (bad) s_bad = 1;
     __syncthreads();
     if (s_bad || s_new > RS - RS / 16) {

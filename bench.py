@@ -324,6 +324,35 @@ def join_line(aj, args, workload_name, nR, nS, algo, out, dev, ms_step, phases, 
     }
 
 
+def random_access_ceiling(aj, line, algo, nR, nS):
+    """The hash-table joins are bound by random slot accesses, not streaming
+    bytes: the same accesses timed alone (aj_access_kernel, a table of 2^28
+    16-byte slots as the joins' at 2^27, 2^27 accesses), and the join's kernel
+    time against that ceiling."""
+    slots, ops = 1 << 28, 1 << 27
+    tab = aj.DeviceArray(16 * slots)
+    try:
+        rate = {k: aj.access_rate(kind, tab.ptr, slots, ops)
+                for k, kind in (("load", aj.AJ_ACCESS_LOAD), ("cas", aj.AJ_ACCESS_CAS),
+                                ("load_cas", aj.AJ_ACCESS_LOAD_CAS))}
+    finally:
+        tab.free()
+    out = {"table_bytes": 16 * slots, "accesses": ops,
+           "G_per_s": {k: round(v, 2) for k, v in rate.items()}}
+    km = line["pipeline"]["kernel_ms"]
+    if algo == aj.AJ_ALGO_NPO and km.get("npo_probe"):
+        # one random slot load per S tuple
+        t = nS / (rate["load"] * 1e9) * 1e3
+        out["probe"] = {"ceiling_ms": round(t, 4), "kernel_ms": km["npo_probe"],
+                        "frac": round(t / km["npo_probe"], 3)}
+    if algo == aj.AJ_ALGO_SHJ:
+        # a probe load and an independent claim per tuple of either side
+        t = (nR + nS) / (rate["load_cas"] * 1e9) * 1e3
+        out["steps"] = {"ceiling_ms": round(t, 3), "join_ms": line["pipeline"]["device_ms"],
+                        "frac": round(t / line["pipeline"]["device_ms"], 3)}
+    return out
+
+
 def run_pro(aj, args):
     n = 1 << args.log2n
     t0 = time.time()
@@ -363,6 +392,8 @@ def run_pro(aj, args):
     if algo == aj.AJ_ALGO_SHJ:
         line["metric"] = "eager join throughput M tuples/s (|R|+|S|)"
         line["config"]["batch"] = args.batch or max(65536, -(-n // 256))
+    if algo != aj.AJ_ALGO_PRO:
+        line["roofline"]["random_access"] = random_access_ceiling(aj, line, algo, n, n)
     if not args.no_cpu:
         host = host_info()
         if algo == aj.AJ_ALGO_SHJ:

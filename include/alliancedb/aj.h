@@ -277,6 +277,15 @@ int aj_copy_d2h(void *dst_host, const void *src_device, uint64_t bytes);
 /* Device-to-device copy by a plain streaming kernel (aj_copy_kernel), also
  * the reference point for calibrating memory-traffic counters. */
 int aj_copy_d2d(void *dst_device, const void *src_device, uint64_t bytes);
+/* Random-access ceilings of the hash-table joins (no counterpart in the
+ * reference; measurement only).  One timed launch over a caller's table of
+ * `slots` 16-byte slots (a power of two): n random 16-byte slot loads
+ * (AJ_ACCESS_LOAD, an NPO probe's pattern), n random 64-bit CAS claims of empty
+ * words (AJ_ACCESS_CAS, an insert's), or both at independent slots
+ * (AJ_ACCESS_LOAD_CAS, an SHJ step's).  The table is first filled with the
+ * empty marker (not timed); *ms receives the launch's device time. */
+enum aj_access_kind { AJ_ACCESS_LOAD = 0, AJ_ACCESS_CAS = 1, AJ_ACCESS_LOAD_CAS = 2 };
+int aj_access_kernel(int kind, void *table_device, uint64_t slots, uint64_t n, float *ms);
 int aj_device_sync(void);
 
 typedef struct aj_ctx aj_ctx_t;
