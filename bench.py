@@ -203,16 +203,17 @@ def cpu_reference_pro(aj, log2n: int):
 
 def measured_traffic(kernel: str, workload: str):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 traffic summary of
-    this workload, or None when none was measured for it."""
-    path = os.path.join(ROOT, "profiles", "pro_2e27_traffic.json")
-    try:
-        with open(path) as f:
-            d = json.load(f)
-    except OSError:
-        return None
-    if d.get("workload") != workload:
-        return None
-    return d["hbm_bytes_per_launch"].get(kernel)
+    this workload (profiles/*_traffic.json), or None when none was measured for it."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_traffic.json"))):
+        try:
+            with open(path) as f:
+                d = json.load(f)
+        except (OSError, ValueError):
+            continue
+        if d.get("workload") == workload:
+            return d["hbm_bytes_per_launch"].get(kernel)
+    return None
 
 
 def closed_form_fk(R: np.ndarray, S: np.ndarray):
@@ -387,7 +388,7 @@ def run_pro(aj, args):
     # the traffic summary covers the default geometry (14 bits in 7 + 7)
     default_geom = algo == aj.AJ_ALGO_PRO and not (args.radix_bits or args.passes) and (
         not tuned or (tuned["radix_bits"], tuned["passes"]) == (14, 2))
-    if default_geom:
+    if default_geom or (algo != aj.AJ_ALGO_PRO and not args.batch):
         line["roofline"]["traffic"] = measured_traffic(line["roofline"]["kernel"], name)
     if algo == aj.AJ_ALGO_SHJ:
         line["metric"] = "eager join throughput M tuples/s (|R|+|S|)"

@@ -33,3 +33,19 @@ def test_bench_reads_traffic_summary():
     k = "k_stage2 (pass 2)"
     assert bench.measured_traffic(k, d["workload"]) == d["hbm_bytes_per_launch"][k]
     assert bench.measured_traffic(k, "another workload") is None
+
+
+def test_hash_join_traffic_summaries():
+    """NPO's probe moves about one 128-byte line per S tuple (random slot loads)
+    plus the tuple itself; its region build reads R and writes every table slot
+    (8 + 32 B per R tuple); an SHJ step launch moves at least a line per tuple."""
+    n = 1 << 27
+    with open(os.path.join(ROOT, "profiles", "npo_2e27_traffic.json")) as f:
+        npo = json.load(f)["hbm_bytes_per_launch"]
+    assert 128 * n <= npo["k_npo_probe"] <= 160 * n
+    assert 40 * n <= npo["k_npo_region"] <= 1.1 * 40 * n
+    with open(os.path.join(ROOT, "profiles", "shj_2e27_traffic.json")) as f:
+        d = json.load(f)
+    per_join = d["hbm_bytes_per_launch"]["k_shj_step (all eager steps)"]
+    assert d["launches_per_join"]["k_shj_step (all eager steps)"] == 512
+    assert 128 * 2 * n <= per_join <= 400 * 2 * n
