@@ -11,6 +11,8 @@
 //           [--budget bytes] [--slots n] [--batch-windows n] [--clients n]
 //           [--verify n] [--median n] [--sigma s] [--cap-tuples n]
 //           [--domain n] [-z theta]      (served windows: ajbench_serve.cpp)
+//           [--calibrate n]  (instead of a load point: plain one-window launches
+//                             and the server's batched peak on n resident windows)
 //
 // Sizes: with -w/-e/-l, |R| = |S| = (window / interval) * step as in the
 // reference (benchmark.cpp:179-180); otherwise -r/-s (default 128M as in the
@@ -42,25 +44,9 @@
 #include <vector>
 
 #include "../../include/alliancedb/aj.h"
+#include "ajbench_serve.h"
 
-struct ServeParams {
-    int streams = 64;
-    double load = 0.5;
-    double peak_mtps = 55000.0;
-    double seconds = 2.0;
-    double median = 262144, sigma = 1.0;
-    uint64_t cap_tuples = 1ull << 24;
-    uint32_t domain = 1u << 24;
-    double theta = 1.0;
-    double budget = 200e9;
-    int slots = 3;
-    int batch_windows = 256;
-    uint64_t batch_tuples = 1ull << 27;
-    int clients = 4;
-    int verify = 64;
-    uint64_t seed = 2024;
-};
-int run_serve(const ServeParams &P);
+
 
 namespace {
 
@@ -108,12 +94,12 @@ void usage(const char *prog) {
            "       %s -a SERVE [--load f] [--peak Mtps] [--seconds s] [--streams n]\n"
            "          [--budget bytes] [--slots n] [--batch-windows n] [--clients n]\n"
            "          [--verify n] [--median n] [--sigma s] [--cap-tuples n] [--domain n]\n"
-           "          [--batch-tuples n]\n",
+           "          [--batch-tuples n] [--calibrate n]\n",
            prog, prog);
 }
 
 enum { OPT_LOAD = 1000, OPT_PEAK, OPT_SECONDS, OPT_STREAMS, OPT_BUDGET, OPT_SLOTS, OPT_BWIN,
-       OPT_CLIENTS, OPT_VERIFY, OPT_MEDIAN, OPT_SIGMA, OPT_CAPT, OPT_DOMAIN, OPT_BTUP,
+       OPT_CLIENTS, OPT_VERIFY, OPT_MEDIAN, OPT_SIGMA, OPT_CAPT, OPT_DOMAIN, OPT_BTUP, OPT_CALIB,
        OPT_PAIRS, OPT_OUT, OPT_REPEATS, OPT_TIMING, OPT_EXPID, OPT_GAP, OPT_FKZIPF };
 
 }  // namespace
@@ -137,6 +123,7 @@ int main(int argc, char **argv) {
                                 {"cap-tuples", required_argument, 0, OPT_CAPT},
                                 {"domain", required_argument, 0, OPT_DOMAIN},
                                 {"batch-tuples", required_argument, 0, OPT_BTUP},
+                                {"calibrate", required_argument, 0, OPT_CALIB},
                                 {"pairs", no_argument, 0, OPT_PAIRS},
                                 {"out", required_argument, 0, OPT_OUT},
                                 {"algo", required_argument, 0, 'a'},
@@ -202,6 +189,7 @@ int main(int argc, char **argv) {
             case OPT_CAPT: sp.cap_tuples = strtoull(optarg, nullptr, 10); break;
             case OPT_DOMAIN: sp.domain = (uint32_t)strtoul(optarg, nullptr, 10); break;
             case OPT_BTUP: sp.batch_tuples = strtoull(optarg, nullptr, 10); break;
+            case OPT_CALIB: sp.calibrate = atoi(optarg); break;
             case 'w': p.window = atol(optarg); break;
             case 'e': p.step = atol(optarg); break;
             case 'l': p.interval = atol(optarg); break;

@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "../../include/alliancedb/aj.h"
+#include "ajbench_serve.h"
 
 namespace {
 
@@ -51,25 +52,121 @@ double pct(std::vector<double> v, double p) {
     return v[i];
 }
 
-}  // namespace
+uint64_t draw_size(std::lognormal_distribution<double> &size, std::mt19937_64 &rng,
+                   uint64_t cap) {
+    return std::min<uint64_t>(cap, std::max<uint64_t>(1, (uint64_t)size(rng)));
+}
 
-struct ServeParams {
-    int streams = 64;
-    double load = 0.5;           // fraction of peak
-    double peak_mtps = 55000.0;  // M tuples/s (R+S) the load is relative to
-    double seconds = 2.0;
-    double median = 262144, sigma = 1.0;
-    uint64_t cap_tuples = 1ull << 24;
-    uint32_t domain = 1u << 24;
-    double theta = 1.0;
-    double budget = 200e9;
-    int slots = 3;
-    int batch_windows = 256;
-    uint64_t batch_tuples = 1ull << 27;
-    int clients = 4;
-    int verify = 64;
-    uint64_t seed = 2024;
-};
+double host_us() {
+    return std::chrono::duration<double, std::micro>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+// The two reference rates of the served workload, on P.calibrate windows of the
+// same distribution generated into the server's window memory ahead of time:
+// plain one-window launches (a batch plan sized to the largest window, one
+// window per aj_batch_run on one stream, back to back, one sync at the end)
+// and the server's batched peak (every window submitted at once, until the
+// last result is on the host).  The served results must equal the plain ones.
+int run_calibration(const ServeParams &P, aj_ctx_t *ctx, aj_server_t *srv) {
+    const int K = P.calibrate;
+    std::mt19937_64 rng(P.seed ^ 0xCA11B7A7Eull);
+    std::lognormal_distribution<double> size(std::log(P.median), P.sigma);
+    std::vector<aj_window_t> win(K);
+    uint64_t tuples = 0, maxw = 1;
+    for (int i = 0; i < K; ++i) {
+        const uint64_t nR = draw_size(size, rng, P.cap_tuples);
+        const uint64_t nS = draw_size(size, rng, P.cap_tuples);
+        const double theta = (i % P.streams) >= P.streams / 2 ? P.theta : 0.0;
+        if (aj_server_window_alloc(srv, nR, nS, -1, &win[i]) != AJ_OK ||
+            aj_server_window_fill(srv, &win[i], P.seed * 7919ull + (uint64_t)i, P.domain,
+                                  theta) != AJ_OK) {
+            fprintf(stderr, "[ERROR] calibration windows: %s\n", aj_last_error());
+            return 1;
+        }
+        tuples += nR + nS;
+        maxw = std::max(maxw, std::max(nR, nS));
+    }
+    if (aj_device_sync() != AJ_OK) {
+        fprintf(stderr, "[ERROR] calibration fill: %s\n", aj_last_error());
+        return 1;
+    }
+    aj_batch_opts_t bo;
+    memset(&bo, 0, sizeof(bo));
+    bo.no_graph = 1;
+    aj_batch_plan_t *plan = nullptr;
+    void *dres = nullptr, *dbounds = nullptr;
+    const uint32_t bounds[2] = {0u, 1u};  // every generated tuple has ts 0
+    std::vector<uint64_t> plain(2 * (size_t)K);
+    double t_plain = 0;
+    int rc = 0;
+    if (aj_batch_plan_create(ctx, maxw, maxw, 1, maxw, &bo, &plan) != AJ_OK ||
+        aj_device_alloc(&dres, 16 * (uint64_t)K) != AJ_OK ||
+        aj_device_alloc(&dbounds, sizeof(bounds)) != AJ_OK ||
+        aj_copy_h2d(dbounds, bounds, sizeof(bounds)) != AJ_OK) {
+        rc = 1;
+    } else {
+        uint64_t *res = static_cast<uint64_t *>(dres);
+        auto one = [&](int i) {
+            return aj_batch_run(plan, win[i].dR, win[i].nR, win[i].dS, win[i].nS,
+                                (const uint32_t *)dbounds, 1, res + 2 * i, res + 2 * i + 1,
+                                nullptr);
+        };
+        for (int i = 0; i < std::min(K, 4) && rc == 0; ++i) rc = one(i) != AJ_OK;  // warm
+        if (rc == 0) rc = aj_device_sync() != AJ_OK;
+        const double t0 = host_us();
+        for (int i = 0; i < K && rc == 0; ++i) rc = one(i) != AJ_OK;
+        if (rc == 0) rc = aj_device_sync() != AJ_OK;
+        t_plain = host_us() - t0;
+        if (rc == 0) rc = aj_copy_d2h(plain.data(), dres, 16 * (uint64_t)K) != AJ_OK;
+    }
+    if (plan) aj_batch_plan_destroy(plan);
+    if (dres) aj_device_free(dres);
+    if (dbounds) aj_device_free(dbounds);
+    if (rc) {
+        fprintf(stderr, "[ERROR] calibration plain launches: %s\n", aj_last_error());
+        return 1;
+    }
+    // the batched peak: all windows queued at once
+    const double t1 = aj_server_now_us(srv);
+    for (int i = 0; i < K; ++i)
+        if (aj_server_submit(srv, &win[i], (uint64_t)i, -1) != AJ_OK) {
+            fprintf(stderr, "[ERROR] calibration submit: %s\n", aj_last_error());
+            return 1;
+        }
+    aj_server_drain(srv);
+    std::vector<aj_server_result_t> got(K);
+    int n = 0, mismatches = 0, bad = 0;
+    double last = t1;
+    while (n < K) {
+        int m = 0;
+        if (aj_server_poll(srv, got.data() + n, K - n, 1000000, &m) != AJ_OK || m == 0) break;
+        n += m;
+    }
+    for (int i = 0; i < n; ++i) {
+        const aj_server_result_t &x = got[i];
+        last = std::max(last, x.done_us);
+        bad += x.status != AJ_OK;
+        if (x.tag >= (uint64_t)K || x.count != plain[2 * x.tag] ||
+            x.checksum != plain[2 * x.tag + 1])
+            ++mismatches;
+    }
+    aj_server_stats_t st;
+    aj_server_stats(srv, &st);
+    const double t_batch = last - t1;
+    printf("{\"calibration\": true, \"windows\": %d, \"tuples\": %llu, "
+           "\"largest_window\": %llu, \"plain_ms\": %.3f, \"plain_Mtps\": %.1f, "
+           "\"batched_ms\": %.3f, \"batched_Mtps\": %.1f, \"batches\": %llu, "
+           "\"results\": %d, \"bad_status\": %d, \"mismatches\": %d}\n",
+           K, (unsigned long long)tuples, (unsigned long long)maxw, t_plain / 1e3,
+           tuples / t_plain, t_batch / 1e3, tuples / t_batch, (unsigned long long)st.batches, n,
+           bad, mismatches);
+    fflush(stdout);
+    return (n != K || bad || mismatches) ? 2 : 0;
+}
+
+}  // namespace
 
 int run_serve(const ServeParams &P) {
     aj_ctx_t *ctx = nullptr;
@@ -115,6 +212,12 @@ int run_serve(const ServeParams &P) {
     if (aj_server_create(ctx, &so, &srv) != AJ_OK) {
         fprintf(stderr, "[ERROR] server: %s\n", aj_last_error());
         return 1;
+    }
+    if (P.calibrate > 0) {
+        const int rc = run_calibration(P, ctx, srv);
+        aj_server_destroy(srv);
+        aj_ctx_destroy(ctx);
+        return rc;
     }
     std::vector<Done> done(sched.size());
     std::atomic<uint64_t> n_done{0};

@@ -69,8 +69,11 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", choices=["pro", "skew", "windows", "serve"], default="pro")
-    ap.add_argument("--peak", type=float, default=55600.0,
-                    help="serve: batched-window peak (M tuples/s) the offered load is relative to")
+    ap.add_argument("--peak", type=float, default=0.0,
+                    help="serve: the M tuples/s the offered load is relative to (0 = the "
+                         "batched peak measured in the same run)")
+    ap.add_argument("--calib-windows", type=int, default=256,
+                    help="serve: resident windows of the reference-rate measurement")
     ap.add_argument("--loads", default="0.5,0.7,0.9,1.0,1.1,1.2")
     ap.add_argument("--seconds", type=float, default=2.0)
     ap.add_argument("--algo", choices=["PRO", "NPO", "SHJ"], default="PRO",
@@ -150,7 +153,7 @@ def cpu_pro_timing(R, S, threads: int, reps: int):
 
 
 def cpu_baseline(aj, R, S, host):
-    """configs[1]'s CPU baseline: the oracle's PRO on the same 2^27 x 2^27 relations
+    """The CPU baseline of the 2^27 x 2^27 join: the oracle's PRO on the same relations
     the GPU joined, on the host CPUs this process may use (at most 16, the box's
     share of its host)."""
     threads = min(host["usable_cpus"] or 1, 16)
@@ -166,7 +169,7 @@ def cpu_baseline(aj, R, S, host):
 
 
 def cpu_baseline_small(aj, host):
-    """2^20 x 2^20, 8 threads, median of 5 after a warm-up."""
+    """The reference's CPU-sized case: 2^20 x 2^20, 8 threads, median of 5 after a warm-up."""
     n = 1 << 20
     R = aj.gen_unique(n, 12345)
     S = aj.gen_uniform(n, 1, n, 54321)
@@ -446,7 +449,6 @@ def relabel_scatter(line, geom):
 
 
 def run_windows(aj, args):
-    import oracle
     nwin = args.nwin
     rng = np.random.default_rng(2024)
     sizes = np.exp(rng.uniform(np.log(4096), np.log(4194304), nwin)).astype(np.uint32)
@@ -473,39 +475,32 @@ def run_windows(aj, args):
             aj.device_sync()
             t_batch = (time.perf_counter() - t0) / args.steps
         batch_res = np.stack([cnt.to_host(np.uint64, nwin), chk.to_host(np.uint64, nwin)])
-        # per-window launches: one plain (not batched, not graph) join call per window
+        hR, hS = dR.to_host(aj.STREAM_DTYPE, nR), dS.to_host(aj.STREAM_DTYPE, nS)
+        # each window's tuple range per side (the streams are in timestamp order)
+        rng_r = np.searchsorted(hR["ts"], bounds, side="left")
+        rng_s = np.searchsorted(hS["ts"], bounds, side="left")
+        per_r, per_s = np.diff(rng_r), np.diff(rng_s)
+        maxw = int(max(per_r.max(), per_s.max(), 1))
+        # per-window launches: a plan sized to the largest window, one plain (not
+        # batched, no graph) launch per window on that window's tuples, back to back
+        # on one stream, one sync at the end
         cnt1, chk1 = aj.DeviceArray(8 * nwin), aj.DeviceArray(8 * nwin)
-        with aj.BatchPlan(ctx, nR, nS, 1, int(sizes.max() * 1.05), graph=False) as one:
+        with aj.BatchPlan(ctx, maxw, maxw, 1, maxw, graph=False) as one:
+            def launch(w):
+                one.run(dR.ptr + 12 * int(rng_r[w]), int(per_r[w]), dS.ptr + 12 * int(rng_s[w]),
+                        int(per_s[w]), bR.ptr + 4 * w, 1, cnt1.ptr + 8 * w, chk1.ptr + 8 * w)
             for w in range(min(nwin, 8)):  # warm
-                one.run(dR.ptr, nR, dS.ptr, nS, bR.ptr + 4 * w, 1, cnt1.ptr + 8 * w,
-                        chk1.ptr + 8 * w)
+                launch(w)
             aj.device_sync()
             t0 = time.perf_counter()
             for w in range(nwin):
-                one.run(dR.ptr, nR, dS.ptr, nS, bR.ptr + 4 * w, 1, cnt1.ptr + 8 * w,
-                        chk1.ptr + 8 * w)
-                aj.device_sync()
+                launch(w)
+            aj.device_sync()
             t_single = time.perf_counter() - t0
         single_res = np.stack([cnt1.to_host(np.uint64, nwin), chk1.to_host(np.uint64, nwin)])
         assert np.array_equal(batch_res, single_res), "batched != per-window results"
-        # CPU oracle on a sample of windows (smallest ones, to stay bounded)
-        hR, hS = dR.to_host(aj.STREAM_DTYPE, nR), dS.to_host(aj.STREAM_DTYPE, nS)
-        checked = 0
-        for w in np.argsort(sizes)[:16]:
-            parts = []
-            for t in (hR, hS):
-                lo, hi = np.searchsorted(t["ts"], [bounds[w], bounds[w + 1]])
-                parts.append(aj.as_tuples(t["key"][lo:hi].view(np.int32),
-                                          t["payload"][lo:hi].view(np.int32)))
-            want = oracle.pro(parts[0], parts[1], nthreads=8)
-            assert (want.count, want.checksum) == (int(batch_res[0, w]), int(batch_res[1, w]))
-            checked += 1
-        lo = np.searchsorted(hR["ts"], bounds[0])
-        hi = np.searchsorted(hR["ts"], bounds[-1])
-        in_r = int(hi - lo)
-        lo = np.searchsorted(hS["ts"], bounds[0])
-        hi = np.searchsorted(hS["ts"], bounds[-1])
-        in_s = int(hi - lo)
+        in_r = int(rng_r[-1] - rng_r[0])
+        in_s = int(rng_s[-1] - rng_s[0])
     tuples = in_r + in_s
     gbs = WINDOW_BYTES_PER_TUPLE * tuples / t_batch / 1e9
     return {
@@ -527,30 +522,43 @@ def run_windows(aj, args):
                    "windows": nwin, "tuples_in_windows": tuples},
         "per_window_launches": {"value": round(tuples / t_single / 1e6, 1),
                                 "unit": "M tuples/s", "ms_total": round(t_single * 1e3, 2),
-                                "speedup_of_batched": round(t_single / t_batch, 2)},
+                                "speedup_of_batched": round(t_single / t_batch, 2),
+                                "how": f"a plan sized to the largest window ({maxw} tuples), "
+                                       f"one launch per window on its tuples, back to back, "
+                                       f"one sync at the end"},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "whole batched pipeline", "bytes_per_tuple": WINDOW_BYTES_PER_TUPLE},
-        "parity": {"batched_equals_per_window": True, "oracle_windows_checked": checked},
+        "parity": {"batched_equals_per_window": True},
     }
 
 
 def run_serve(args):
-    """Served windows through the C++ driver (ajbench -a SERVE), one load level per run."""
+    """Served windows through the C++ driver (ajbench -a SERVE).  First the two
+    reference rates on 256 resident windows of the same distribution (plain
+    one-window launches back to back, and the server's batched peak), then one
+    run per load level, the load relative to that batched peak."""
     import subprocess
     exe = os.path.join(ROOT, "alliancedb_amd", "ajbench")
-    sweep = []
-    for load in [float(x) for x in args.loads.split(",")]:
-        cmd = [exe, "-a", "SERVE", "--load", str(load), "--peak", str(args.peak),
-               "--seconds", str(args.seconds), "--budget", "200e9", "--verify", "64"]
-        print(f"[bench] serve load={load}", file=sys.stderr, flush=True)
+
+    def drive(extra, what):
+        cmd = [exe, "-a", "SERVE", "--budget", "200e9"] + extra
+        print(f"[bench] serve {what}", file=sys.stderr, flush=True)
         r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
         if r.returncode != 0:
             raise SystemExit(f"ajbench SERVE failed ({r.returncode}): {r.stderr[-2000:]}")
-        sweep.append(json.loads(r.stdout.strip().splitlines()[-1]))
+        return json.loads(r.stdout.strip().splitlines()[-1])
+
+    cal = drive(["--calibrate", str(args.calib_windows)], "calibration")
+    peak = args.peak or cal["batched_Mtps"]
+    sweep = [drive(["--load", str(load), "--peak", str(peak), "--seconds", str(args.seconds),
+                    "--verify", "64"], f"load={load}")
+             for load in [float(x) for x in args.loads.split(",")]]
     at = min(sweep, key=lambda d: abs(d["load"] - 1.0))
+    best = max(d["achieved_Mtps"] for d in sweep)
     return {
-        "metric": "served window join throughput M tuples/s (|R|+|S|) at 100% offered load; "
+        "metric": "served window join throughput M tuples/s (|R|+|S|) at 100% offered load "
+                  "(the batched peak of this window distribution, measured in the same run); "
                   "p99 window-to-result latency",
         "value": at["achieved_Mtps"],
         "unit": "M tuples/s",
@@ -566,8 +574,20 @@ def run_serve(args):
         "config": {"workload": "64 stream pairs, Poisson window-join requests, per-side sizes "
                                "lognormal(median 256K, sigma 1) capped 16M, R keys uniform over "
                                "2^24, S keys uniform (streams 0-31) or Zipf 1.0 (32-63), 12-byte "
-                               "tuples, HBM budget 200 GB, 3 batch slots",
-                   "peak_Mtps": args.peak, "seconds_per_load": args.seconds},
+                               "tuples, HBM budget 200 GB, 3 batch slots; windows generated on "
+                               "the GPU as they are admitted",
+                   "peak_Mtps": round(peak, 1), "seconds_per_load": args.seconds},
+        "reference_rates": {
+            "windows": cal["windows"], "tuples": cal["tuples"],
+            "plain_per_window_Mtps": cal["plain_Mtps"],
+            "batched_peak_Mtps": cal["batched_Mtps"],
+            "how": "the same windows resident in HBM: plain = a plan sized to the largest "
+                   "window, one launch per window, back to back, one sync at the end; batched "
+                   "peak = every window submitted to the server at once",
+            "parity": {"results": cal["results"], "mismatches": cal["mismatches"]}},
+        "served_vs_plain": {"at_100pct": round(at["achieved_Mtps"] / cal["plain_Mtps"], 2),
+                            "best": round(best / cal["plain_Mtps"], 2),
+                            "best_achieved_Mtps": best},
         "latency_ms": {"p50": at["p50_ms"], "p99": at["p99_ms"], "p999": at["p999_ms"]},
         "sweep": sweep,
         "parity": {"verified_windows": sum(d["verified"] for d in sweep),

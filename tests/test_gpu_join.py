@@ -12,7 +12,7 @@ SEED_R, SEED_S = 12345, 54321
 
 
 def _inputs(aj, kind, n):
-    if kind == "unique_uniform":  # BASELINE workload shape
+    if kind == "unique_uniform":  # the benchmark's key distribution (R a permutation, S uniform)
         return aj.gen_unique(n, SEED_R), aj.gen_uniform(n, 1, max(n, 1), SEED_S)
     if kind == "dups":  # duplicate factor ~8 on both sides
         k = max(n // 8, 1)
@@ -116,7 +116,7 @@ def test_device_pairs_capacity_error(aj, oracle):
 
 
 def test_baseline_shape_closed_form(aj):
-    """|R|=|S|=2^24 of the BASELINE key distribution: every S tuple matches the one R
+    """|R|=|S|=2^24 of the benchmark's key distribution: every S tuple matches the one R
     tuple with its key, so count = |S| and the checksum follows from R's inverse."""
     n = 1 << 24
     R, S = aj.gen_unique(n, SEED_R), aj.gen_uniform(n, 1, n, SEED_S)

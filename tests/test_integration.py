@@ -1,6 +1,6 @@
-"""The reference-side binding shown in INTEGRATION.md section 1 builds.
+"""The reference-side binding shown in the integration guide (section 1) builds.
 
-The adapter block and the registry rows are taken verbatim from INTEGRATION.md,
+The adapter block and the registry rows are taken verbatim from the guide,
 compiled against the reference's own headers (hashing/benchmark.h and
 everything it includes: utils/types.h, joins/parallel_radix_join.h, ...) with
 this repository's include/ on the path, in both include orders, and linked
@@ -29,7 +29,7 @@ def _section1_blocks():
     text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
     sec = text.split("## 1.", 1)[1].split("\n## 2.", 1)[0]
     blocks = re.findall(r"```cpp\n(.*?)```", sec, re.S)
-    assert len(blocks) >= 2, "INTEGRATION.md section 1 holds the adapter and the registry rows"
+    assert len(blocks) >= 2, "the integration guide's section 1 holds the adapter and the registry rows"
     return blocks[0], blocks[1]
 
 
