@@ -370,6 +370,13 @@ int enqueue_ranges(BatchPlan *p, const aj_stream_tuple_t *dR, uint64_t nR,
 int fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t st);
 int fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st);
 int copy_d2d(void *dst, const void *src, uint64_t bytes, int num_cu, hipStream_t st);
+// Prefix sums of n counters into out (not in place), modulo the type's width,
+// on st; tmp of scan_tmp_bytes(n, sizeof element) bytes (partition.hip).
+size_t scan_tmp_bytes(uint64_t n, size_t elem);
+int exclusive_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, void *tmp,
+                       size_t tmp_bytes, hipStream_t st);
+int inclusive_scan_u64(const uint64_t *in, uint64_t *out, uint64_t n, void *tmp,
+                       size_t tmp_bytes, hipStream_t st);
 // one random-access calibration launch (aj_access_kernel) over `slots` 16-byte slots
 int access_kernel(int kind, uint64_t *tab, uint64_t slots, uint64_t n, int num_cu,
                   hipStream_t st);

@@ -27,7 +27,6 @@
 //
 // Bytes per tuple: R and S are each read once (R partitions above CAP: the
 // extra rounds read their R tuples twice).
-#include <hipcub/hipcub.hpp>
 
 #include "aj_internal.h"
 #include "device_common.h"
@@ -507,12 +506,7 @@ __global__ __launch_bounds__(JB, 4) void k_join(JoinArgs a) {
     }
 }
 
-size_t join_scan_tmp_bytes(uint64_t max_parts) {
-    size_t bytes = 0;
-    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (uint32_t *)nullptr, (uint32_t *)nullptr,
-                                           (int)(max_parts + 1));
-    return bytes;
-}
+size_t join_scan_tmp_bytes(uint64_t max_parts) { return scan_tmp_bytes(max_parts + 1, 4); }
 
 uint32_t join_task_split(uint32_t split, int cap) {
     return split ? split : 8u * (uint32_t)cap;
@@ -586,13 +580,8 @@ int run_join(const JoinDesc &d, hipStream_t st, const Marks *mk, int m0) {
     if (!d.tasks_ready) {
         const unsigned g = (unsigned)std::min<uint64_t>((d.max_parts + 256) / 256, 4096);
         hipLaunchKernelGGL(k_task_count, dim3(g), dim3(256), 0, st, a);
-        size_t tmp = d.scan_tmp_bytes;
-        hipError_t e = hipcub::DeviceScan::ExclusiveSum(d.scan_tmp, tmp, d.task_cnt, d.task_base,
-                                                        (int)(d.max_parts + 1), st);
-        if (e != hipSuccess) {
-            set_error(AJ_ERR_HIP, "task scan failed: %s", hipGetErrorString(e));
-            return AJ_ERR_HIP;
-        }
+        AJ_TRY(exclusive_scan_u32(d.task_cnt, d.task_base, d.max_parts + 1, d.scan_tmp,
+                                  d.scan_tmp_bytes, st));
         const unsigned gt = (unsigned)std::min<uint64_t>((d.max_tasks + 255) / 256, 16384);
         hipLaunchKernelGGL(k_task_fill, dim3(gt), dim3(256), 0, st, a);
     }

@@ -25,7 +25,6 @@
 //   k_part_off  partition start offsets for the join
 //
 // Bytes per tuple: k_hist reads 8, k_scatter reads 8 and writes 8.
-#include <hipcub/hipcub.hpp>
 
 #include "aj_internal.h"
 #include "device_common.h"
@@ -666,13 +665,116 @@ __global__ void k_part_off(PassArgs a) {
     }
 }
 
-// ---------------------------------------------------------------- host side
-size_t pass_scan_tmp_bytes(uint64_t max_hist) {
-    size_t bytes = 0;
-    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (uint32_t *)nullptr, (uint32_t *)nullptr,
-                                     (int)max_hist);
-    return bytes;
+// ---------------------------------------------------------------- device scan
+// Prefix sums of n counters (u32: the histograms of a pass, the join's task
+// counts; u64: the window generator's arrival times), modulo the type's width,
+// in three launches: the sum of every tile of SCAN_TILE counters, one
+// work-group's exclusive scan of those sums, and each tile's own scan from its
+// start.  Loads and stores are coalesced (tiles staged through LDS); no
+// work-group waits for another, so no launch order is assumed.
+constexpr int SCAN_BLOCK = 256, SCAN_ITEMS = 16, SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
+constexpr int SUMS_BLOCK = 1024;
+
+template <class T>
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tiles(const T *in, uint64_t n, T *sums) {
+    __shared__ T red[SCAN_BLOCK / 64];
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+    T s = 0;
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; ++j) {
+        const uint64_t i = base + (uint64_t)j * SCAN_BLOCK + threadIdx.x;
+        s += i < n ? in[i] : T(0);
+    }
+    s = block_sum<SCAN_BLOCK>(s, red);
+    if (threadIdx.x == 0) sums[blockIdx.x] = s;
 }
+
+template <class T>
+__global__ __launch_bounds__(SUMS_BLOCK) void k_scan_sums(T *sums, uint32_t tiles) {
+    __shared__ T tmp[SUMS_BLOCK / 64];
+    T carry = 0;
+    for (uint32_t b = 0; b < tiles; b += SUMS_BLOCK) {
+        const uint32_t i = b + threadIdx.x;
+        const T v = i < tiles ? sums[i] : T(0);
+        T total;
+        const T pre = block_exclusive_scan<SUMS_BLOCK>(v, tmp, &total);
+        if (i < tiles) sums[i] = carry + pre;
+        carry += total;
+    }
+}
+
+__device__ __forceinline__ uint32_t scan_slot(uint32_t k) { return k + (k >> 5); }
+
+template <class T, bool INCLUSIVE>
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(const T *in, T *out, uint64_t n,
+                                                         const T *sums) {
+    __shared__ T sh[SCAN_TILE + SCAN_TILE / 32];  // padded: lane t's run spans banks
+    __shared__ T tmp[SCAN_BLOCK / 64];
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; ++j) {
+        const uint32_t k = j * SCAN_BLOCK + threadIdx.x;
+        const uint64_t i = base + k;
+        sh[scan_slot(k)] = i < n ? in[i] : T(0);
+    }
+    __syncthreads();
+    const uint32_t k0 = threadIdx.x * SCAN_ITEMS;
+    T s = 0;
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; ++j) s += sh[scan_slot(k0 + j)];
+    T pre = block_exclusive_scan<SCAN_BLOCK>(s, tmp, (T *)nullptr) + sums[blockIdx.x];
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; ++j) {
+        const T v = sh[scan_slot(k0 + j)];
+        sh[scan_slot(k0 + j)] = INCLUSIVE ? pre + v : pre;
+        pre += v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; ++j) {
+        const uint32_t k = j * SCAN_BLOCK + threadIdx.x;
+        const uint64_t i = base + k;
+        if (i < n) out[i] = sh[scan_slot(k)];
+    }
+}
+
+template <class T, bool INCLUSIVE>
+static int device_scan(const T *in, T *out, uint64_t n, void *tmp, size_t tmp_bytes,
+                       hipStream_t st) {
+    if (!n) return AJ_OK;
+    const uint64_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (tmp_bytes < sizeof(T) * (tiles + 1) || !tmp || tiles >= (1ull << 31) ||
+        (const void *)in == (const void *)out) {
+        set_error(AJ_ERR_ARG, "device scan: workspace of %zu bytes for %llu counters", tmp_bytes,
+                  (unsigned long long)n);
+        return AJ_ERR_ARG;
+    }
+    T *sums = static_cast<T *>(tmp);
+    hipLaunchKernelGGL(k_scan_tiles<T>, dim3((unsigned)tiles), dim3(SCAN_BLOCK), 0, st, in, n,
+                       sums);
+    hipLaunchKernelGGL(k_scan_sums<T>, dim3(1), dim3(SUMS_BLOCK), 0, st, sums, (uint32_t)tiles);
+    hipLaunchKernelGGL((k_scan_apply<T, INCLUSIVE>), dim3((unsigned)tiles), dim3(SCAN_BLOCK), 0,
+                       st, in, out, n, (const T *)sums);
+    AJ_HIP(hipGetLastError());
+    return AJ_OK;
+}
+
+size_t scan_tmp_bytes(uint64_t n, size_t elem) {
+    return elem * ((n + SCAN_TILE - 1) / SCAN_TILE + 1);
+}
+
+int exclusive_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, void *tmp,
+                       size_t tmp_bytes, hipStream_t st) {
+    return device_scan<uint32_t, false>(in, out, n, tmp, tmp_bytes, st);
+}
+
+int inclusive_scan_u64(const uint64_t *in, uint64_t *out, uint64_t n, void *tmp,
+                       size_t tmp_bytes, hipStream_t st) {
+    return device_scan<uint64_t, true>(in, out, n, tmp, tmp_bytes, st);
+}
+
+// ---------------------------------------------------------------- host side
+size_t pass_scan_tmp_bytes(uint64_t max_hist) { return scan_tmp_bytes(max_hist, 4); }
 
 // Raises a kernel's dynamic-LDS limit on the current device to the largest size
 // requested so far there; contexts on several devices or threads share the record.
@@ -726,15 +828,9 @@ static int launch_pass_kernels(const PassDesc &d, const PassWork &w, const PassA
                            lds_hist, st, a);
     }
     if (mk) mk->mark(m0 + 1);
-    size_t tmp_bytes = w.scan_tmp_bytes;
-    if (d.max_hist) {
-        hipError_t e = hipcub::DeviceScan::ExclusiveSum(w.scan_tmp, tmp_bytes, a.hist,
-                                                        a.hist_scan, (int)d.max_hist, st);
-        if (e != hipSuccess) {
-            set_error(AJ_ERR_HIP, "histogram scan failed: %s", hipGetErrorString(e));
-            return AJ_ERR_HIP;
-        }
-    }
+    if (d.max_hist)
+        AJ_TRY(exclusive_scan_u32(a.hist, a.hist_scan, d.max_hist, w.scan_tmp, w.scan_tmp_bytes,
+                                  st));
     if (mk) mk->mark(m0 + 2);
     if (d.max_chunks) {
         if (PBLOCK == 512 && (d.hist2d || STRIDE == 12) && d.max_bits <= PIPE_MAX_BITS) {

@@ -22,7 +22,6 @@
 //
 // Bytes per stream tuple: 12 (histogram read) + 12 (scatter read) + 8 (write)
 // + 8 (build/probe read) = 40.
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -456,10 +455,9 @@ int aj_gen_window_streams(aj_ctx_t *ctx, uint64_t seed, int nwin, const uint32_t
     uint64_t *arrival = gaps + ((n + 31) / 32 * 32);
     const unsigned grid = (unsigned)std::min<uint64_t>((n + 255) / 256, (uint64_t)c->num_cu * 32);
     hipLaunchKernelGGL(k_gen_gaps, dim3(grid), dim3(256), 0, st, gaps, n, seed, 1000.0 / rate_per_us);
-    size_t tmp = 0;
-    AJ_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, tmp, gaps, arrival, (int)n, st));
+    const size_t tmp = scan_tmp_bytes(n, sizeof(uint64_t));
     AJ_TRY(c->scan_tmp.ensure(tmp));
-    AJ_HIP(hipcub::DeviceScan::InclusiveSum(c->scan_tmp.p, tmp, gaps, arrival, (int)n, st));
+    AJ_TRY(inclusive_scan_u64(gaps, arrival, n, c->scan_tmp.p, tmp, st));
     hipLaunchKernelGGL(k_gen_fill, dim3(grid), dim3(256), 0, st, (const uint64_t *)arrival, n, seed,
                        (const uint32_t *)d_bounds, (const uint32_t *)c->seg.p, nwin, dup,
                        (uint32_t *)d_out);
