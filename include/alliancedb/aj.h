@@ -191,6 +191,10 @@ int aj_write_result_file(const result_t *r, const char *path);
  *                        stores); -1 = the default (9); modes 0, 1, 5, 9, 13, 15
  *                        have kernels for 1024 lanes, 1, 5, 9 for 512, others are
  *                        AJ_ERR_ARG
+ *   "npo_region_min" >=0 NPO: counting joins of at least this many R tuples build
+ *                        the table region by region (R partitioned by hashed key,
+ *                        each region built in LDS); 0 = always the flat build
+ *                        (default 2^22)
  *   "shj_batch"    >=0   tuples per side per eager step of the SHJ drop-ins
  *                        (0 = max(65536, max(|R|, |S|) / 256)).  The reference's
  *                        shj alternates R[i], S[i] one tuple at a time
