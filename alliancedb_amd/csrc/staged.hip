@@ -146,8 +146,12 @@ __device__ __forceinline__ void stage_tiles(const uint64_t *in, uint64_t begin, 
             load_tile<PBLOCK, PITEMS, NT_LOAD>(in, 8, nbase, begin, end, t, v);
             if constexpr (HASH) hash_keys<PITEMS>(t);
         }
+        // the first lane of this wave: batches past the tile's end are skipped
+        // (a wave-uniform branch; only a partly filled tile has them)
+        const uint32_t wave0 = __builtin_amdgcn_readfirstlane(threadIdx.x);
 #pragma unroll
         for (int k0 = 0; k0 < PITEMS; k0 += WB) {
+            if (wave0 + (uint32_t)k0 * PBLOCK >= len) break;
             uint64_t u[WB];
             uint32_t w[WB];
 #pragma unroll
