@@ -71,8 +71,18 @@ __device__ __forceinline__ void load_tile(const void *in, int stride, uint64_t t
             v[2 * j + 1] = r1 >= lo_r && r1 < hi_r;
         }
     } else {
-        // (conditional loads: the clamped form needs more registers than the
-        // kernels that read stream tuples have left)
+        const uint32_t *p = static_cast<const uint32_t *>(in);
+        if (tb >= lo && tb + PBLOCK * NI <= hi) {  // whole tile inside: unconditional loads
+#pragma unroll
+            for (int j = 0; j < NI; ++j) {
+                const uint32_t *q = p + 3 * (tb + (uint64_t)j * PBLOCK + threadIdx.x);
+                t[j] = ((uint64_t)load16<NT>(q) << 32) | load16<NT>(q + 1);
+                v[j] = true;
+            }
+            return;
+        }
+        // a partly filled tile: conditional loads (the clamped form needs more
+        // registers than the kernels that read stream tuples have left)
 #pragma unroll
         for (int j = 0; j < NI; ++j) {
             const uint64_t i = tb + (uint64_t)j * PBLOCK + threadIdx.x;
