@@ -13,5 +13,5 @@ from ._lib import (  # noqa: F401
     AjError, BatchPlan, Context, DeviceArray, JoinResult, Server, access_rate, as_tuples, check,
     device_sync, gen_stream_tuples, gen_uniform, gen_unique, gen_window_streams, gen_zipf,
     get_option, last_error, lib, load_relation, set_option, tune_clear, tune_get, tune_load,
-    tune_pro, tune_put, tune_save,
+    tune_batch, tune_pro, tune_put, tune_save,
 )

@@ -226,6 +226,8 @@ def _load() -> ctypes.CDLL:
         "aj_server_drain": (c.c_int, [P]),
         "aj_server_now_us": (c.c_double, [P]),
         "aj_server_stats": (c.c_int, [P, c.POINTER(aj_server_stats_t)]),
+        "aj_tune_batch": (c.c_int, [P, P, c.c_uint64, P, c.c_uint64, P, c.c_int, c.c_uint64,
+                                    c.c_int, c.POINTER(aj_batch_opts_t), c.POINTER(c.c_float)]),
         "aj_tune_pro": (c.c_int, [P, P, c.c_uint64, P, c.c_uint64, c.c_int,
                                   c.POINTER(aj_tune_entry_t)]),
         "aj_tune_put": (c.c_int, [c.POINTER(aj_tune_entry_t)]),
@@ -471,6 +473,17 @@ class BatchPlan:
 
     def __exit__(self, *exc):
         self.close()
+
+
+def tune_batch(ctx: "Context", dR: int, nR: int, dS: int, nS: int, d_bounds: int, nwin: int,
+               max_window_tuples: int = 0, reps: int = 2) -> tuple:
+    """aj_tune_batch: the fastest batch geometry for these resident windows, as
+    (dict of BatchPlan keyword arguments, mean launch ms)."""
+    o = aj_batch_opts_t()
+    ms = ctypes.c_float(0)
+    check(lib.aj_tune_batch(ctx.handle, dR, nR, dS, nS, d_bounds, nwin, max_window_tuples,
+                            reps, ctypes.byref(o), ctypes.byref(ms)))
+    return {"cap": o.cap, "target": o.target, "chunk": o.chunk}, float(ms.value)
 
 
 def gen_window_streams(ctx: "Context", seed: int, sizes: np.ndarray, rate_per_us: float,

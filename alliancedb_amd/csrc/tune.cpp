@@ -1,5 +1,6 @@
 // tune.cpp -- autotuner of the PRO geometry (radix bits, passes, LDS round,
-// partition work-group size) per size bucket, with a persisted table.
+// partition work-group size) per size bucket, with a persisted table, and of
+// a window batch's geometry (LDS round, partition target, work-group chunk).
 //
 // No counterpart in the reference, whose geometry is fixed at compile time
 // (NUM_RADIX_BITS, NUM_PASSES: prj_params.h:16-21).  aj_tune_pro times a small
@@ -9,6 +10,7 @@
 // whose options leave the geometry open use the bucket's entry.  The table is
 // saved as JSON, one entry per line.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -250,6 +252,67 @@ int aj_tune_pro(aj_ctx_t *ctx, const tuple_t *dR, uint64_t nR, const tuple_t *dS
     e.device_ms = best_ms;
     AJ_TRY(aj_tune_put(&e));
     *best = e;
+    return AJ_OK;
+}
+
+int aj_tune_batch(aj_ctx_t *ctx, const aj_stream_tuple_t *dR, uint64_t nR,
+                  const aj_stream_tuple_t *dS, uint64_t nS, const uint32_t *d_bounds, int nwin,
+                  uint64_t max_window_tuples, int reps, aj_batch_opts_t *best, float *best_ms) {
+    if (!ctx || !dR || !dS || !d_bounds || nwin < 1 || reps < 1 || !best) {
+        set_error(AJ_ERR_ARG, "aj_tune_batch: bad arguments");
+        return AJ_ERR_ARG;
+    }
+    uint64_t *dres = nullptr;
+    AJ_TRY(aj_device_alloc(reinterpret_cast<void **>(&dres), 16 * (uint64_t)nwin));
+    std::vector<uint64_t> want, got(2 * (size_t)nwin);
+    float win_ms = 1e30f;
+    aj_batch_opts_t win{};
+    int rc = AJ_OK;
+    // candidates: LDS round x partition target (the round's power of two, or
+    // half of it: more, smaller partitions) x tuples per partition work-group
+    // (derived from the sizes, or fixed multiples of the 16384-tuple tile)
+    for (int cap : {8192, 8704, 12288}) {
+        uint32_t t = 1;
+        while (2 * t <= (uint32_t)cap) t *= 2;
+        for (uint32_t target : {t, t / 2})
+            for (uint32_t chunk : {0u, 65536u, 262144u}) {
+                aj_batch_opts_t o{};
+                o.cap = cap;
+                o.target = target;
+                o.chunk = chunk;
+                aj_batch_plan_t *plan = nullptr;
+                rc = aj_batch_plan_create(ctx, nR, nS, nwin, max_window_tuples, &o, &plan);
+                double ms = 0;
+                for (int r = -1; rc == AJ_OK && r < reps; ++r) {  // r = -1: warm-up (and capture)
+                    const auto t0 = std::chrono::steady_clock::now();
+                    rc = aj_batch_run(plan, dR, nR, dS, nS, d_bounds, nwin, dres, dres + nwin,
+                                      nullptr);
+                    if (rc == AJ_OK) rc = aj_device_sync();
+                    const auto t1 = std::chrono::steady_clock::now();
+                    if (r >= 0) ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
+                }
+                if (rc == AJ_OK) rc = aj_copy_d2h(got.data(), dres, 16 * (uint64_t)nwin);
+                if (plan) aj_batch_plan_destroy(plan);
+                if (rc != AJ_OK) break;
+                if (want.empty()) {
+                    want = got;
+                } else if (got != want) {
+                    rc = set_error(AJ_ERR_STATE,
+                                   "batch tuning candidate (cap %d, target %u, chunk %u) disagrees",
+                                   cap, target, chunk);
+                    break;
+                }
+                if (ms / reps < win_ms) {
+                    win_ms = (float)(ms / reps);
+                    win = o;
+                }
+            }
+        if (rc != AJ_OK) break;
+    }
+    aj_device_free(dres);
+    if (rc != AJ_OK) return rc;
+    *best = win;
+    if (best_ms) *best_ms = win_ms;
     return AJ_OK;
 }
 

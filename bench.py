@@ -465,7 +465,14 @@ def run_windows(aj, args):
         print(f"[bench] streams: {nR} + {nS} tuples, {nwin} windows", file=sys.stderr,
               flush=True)
         cnt, chk = aj.DeviceArray(8 * nwin), aj.DeviceArray(8 * nwin)
-        with aj.BatchPlan(ctx, nR, nS, nwin, int(sizes.max() * 1.05)) as plan:
+        geom = {}
+        if not args.no_tune:  # the batch geometry chosen on these windows
+            t0 = time.perf_counter()
+            geom, tuned_ms = aj.tune_batch(ctx, dR.ptr, nR, dS.ptr, nS, bR.ptr, nwin,
+                                           int(sizes.max() * 1.05), reps=2)
+            print(f"[bench] batch tuned in {time.perf_counter() - t0:.1f}s: {geom} "
+                  f"({tuned_ms:.2f} ms)", file=sys.stderr, flush=True)
+        with aj.BatchPlan(ctx, nR, nS, nwin, int(sizes.max() * 1.05), **geom) as plan:
             for _ in range(args.warmup):
                 plan.run(dR.ptr, nR, dS.ptr, nS, bR.ptr, nwin, cnt.ptr, chk.ptr)
             aj.device_sync()
@@ -519,7 +526,9 @@ def run_windows(aj, args):
         "config": {"workload": f"{nwin} tumbling windows, two Poisson streams at 1.6 tuples/us, "
                                f"per-side sizes log-uniform [4K, 4M], duplicate factor 4, "
                                f"12-byte tuples; one hipGraph launch per step",
-                   "windows": nwin, "tuples_in_windows": tuples},
+                   "windows": nwin, "tuples_in_windows": tuples,
+                   "geometry": dict(geom, chosen_by="autotuner (aj_tune_batch)") if geom
+                   else "plan defaults"},
         "per_window_launches": {"value": round(tuples / t_single / 1e6, 1),
                                 "unit": "M tuples/s", "ms_total": round(t_single * 1e3, 2),
                                 "speedup_of_batched": round(t_single / t_batch, 2),

@@ -569,6 +569,20 @@ int aj_tune_clear(void);
 int aj_tune_save(const char *path);
 int aj_tune_load(const char *path);
 
+/*
+ * Autotuner of a window batch's geometry (no counterpart in the reference,
+ * which joins one window per run): plans of LDS rounds {8192, 8704, 12288} x
+ * partition targets {the round's power of two, half of it} x tuples per
+ * partition work-group {derived, 65536, 262144} each join the caller's
+ * resident batch (a warm-up, then reps timed launches with a sync each); every
+ * candidate must return the same per-window counts and checksums
+ * (AJ_ERR_STATE otherwise).  *best receives the fastest options (graph on),
+ * *best_ms (nullable) its mean launch time.  Allocates a plan per candidate.
+ */
+int aj_tune_batch(aj_ctx_t *ctx, const aj_stream_tuple_t *dR, uint64_t nR,
+                  const aj_stream_tuple_t *dS, uint64_t nS, const uint32_t *d_bounds, int nwin,
+                  uint64_t max_window_tuples, int reps, aj_batch_opts_t *best, float *best_ms);
+
 /* ------------------------------------------------------------ generators */
 /*
  * Seeded host generators (counter-based splitmix64 stream: value i is output

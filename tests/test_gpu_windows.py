@@ -126,3 +126,27 @@ def test_batch_geometry_invariance(aj, oracle, cap, target, chunk):
             got = list(zip(cnt.to_host(np.uint64, nwin).tolist(),
                            chk.to_host(np.uint64, nwin).tolist()))
         assert got == [(int(c), int(x)) for c, x in want]
+
+
+def test_batch_autotuner_picks_a_geometry_with_identical_results(aj, oracle):
+    """aj_tune_batch times 18 plan geometries on the resident batch (all must agree);
+    a plan with the chosen geometry returns the oracle's per-window results."""
+    nwin = 64
+    with aj.Context(0) as ctx:
+        sizes, out = _streams(aj, ctx, nwin, 1 << 12, 1 << 18, seed=11)
+        bounds, host, want = _expected(aj, oracle, out, nwin)
+        (dR, nR, bR), (dS, nS, _) = out["R"], out["S"]
+        maxw = int(sizes.max() * 1.1)
+        print(f"case batch autotuner nR={nR} nS={nS}", flush=True)
+        opts, ms = aj.tune_batch(ctx, dR.ptr, nR, dS.ptr, nS, bR.ptr, nwin, maxw, reps=1)
+        assert opts["cap"] in (8192, 8704, 12288) and ms > 0
+        assert opts["target"] in (4096, 8192, 2048) and opts["chunk"] in (0, 65536, 262144)
+        cnt, chk = aj.DeviceArray(8 * nwin), aj.DeviceArray(8 * nwin)
+        with aj.BatchPlan(ctx, nR, nS, nwin, maxw, **opts) as plan:
+            plan.run(dR.ptr, nR, dS.ptr, nS, bR.ptr, nwin, cnt.ptr, chk.ptr)
+            aj.device_sync()
+            got = list(zip(cnt.to_host(np.uint64, nwin).tolist(),
+                           chk.to_host(np.uint64, nwin).tolist()))
+        assert got == [(int(c), int(x)) for c, x in want]
+        for d in (cnt, chk, out["R"][0], out["R"][2], out["S"][0], out["S"][2]):
+            d.free()
