@@ -218,7 +218,9 @@ struct StagedGeom {
     uint64_t chunk = 0;       // tuples per super-chunk (a multiple of the tile)
     uint64_t tiles_per_wg = 0;
     uint32_t grid1 = 0;       // pass-1 work-groups of a full super-chunk
-    uint32_t units = 1;       // pass-2 work-groups per staging region
+    uint32_t units = 1;       // pass-2 work items per staging region
+    uint32_t per_wg = 1;      // pass-2 work items per work-group
+    uint32_t grid2 = 0;       // pass-2 work-groups
     uint32_t scap = 0;        // tuples per staging region (nset * 2^b1 regions)
     uint32_t fcap[2] = {0, 0};  // tuples per final partition of R and S
     uint64_t stage_tuples = 0;  // staging buffer size (regions + a tile of trash)

@@ -60,29 +60,46 @@ struct StageRes {
     uint32_t *flag;
 };
 
+// One contiguous input range of the tile loop and where its runs go.
+struct StageSeg {
+    const uint64_t *in;
+    uint64_t begin, end;
+    StageRes r;
+};
+
 // The pipelined tile loop of the partition scatter (four barriers per tile,
 // branch-free phases, the next tile's loads in flight during the write), with
 // the write bases taken from atomic reservations instead of running cursors.
 // Wave 0 scans the tile's F counters and issues one atomic add per non-empty
 // digit; the returned offsets are needed only for the write phase, so they are
 // turned into write bases after wave 0 has placed its own tuples.
-template <int PBLOCK, bool NT_LOAD, bool NT_STORE, bool HASH = false>
-__device__ __forceinline__ void stage_tiles(const uint64_t *in, uint64_t begin, uint64_t end,
-                                            uint64_t *out, int shift, uint32_t F, uint64_t *buf,
-                                            uint32_t *cnt, uint32_t *dst, uint32_t *s_len,
-                                            const StageRes &r) {
+// The input is a sequence of segments src.get(0), src.get(1), ... (each with
+// its own destination regions); the loop runs through all of them as one
+// pipeline, so the first tile of a segment is loaded while the last tile of
+// the one before is written.  Every value src.get returns is wave-uniform.
+template <int PBLOCK, bool NT_LOAD, bool NT_STORE, bool HASH, class Src>
+__device__ __forceinline__ void stage_tiles(const Src &src, uint64_t *out, int shift, uint32_t F,
+                                            uint64_t *buf, uint32_t *cnt, uint32_t *dst,
+                                            uint32_t *s_len) {
     constexpr int PTILE = PBLOCK * PITEMS;
     constexpr int MAXPER = STAGE_MAXF / 64;
     constexpr int WB = 8;
     static_assert(PITEMS % WB == 0, "a tile's items split into whole batches");
     const uint32_t mask = F - 1;
-    uint64_t base = tile_start(begin, 8);
-    if (base >= end) return;
+    StageSeg sg;
+    uint32_t si = 0;
+    uint64_t base;
+    for (;; ++si) {  // the first segment with a tuple
+        if (!src.get(si, sg)) return;
+        base = tile_start(sg.begin, 8);
+        if (base < sg.end) break;
+    }
     uint64_t t[PITEMS];
     bool v[PITEMS];
-    load_tile<PBLOCK, PITEMS, NT_LOAD>(in, 8, base, begin, end, t, v);
+    load_tile<PBLOCK, PITEMS, NT_LOAD>(sg.in, 8, base, sg.begin, sg.end, t, v);
     if constexpr (HASH) hash_keys<PITEMS>(t);
     for (;;) {
+        const StageRes &r = sg.r;
         uint32_t rank2[PITEMS / 2];
 #pragma unroll
         for (int j = 0; j < PITEMS; ++j) {
@@ -140,10 +157,16 @@ __device__ __forceinline__ void stage_tiles(const uint64_t *in, uint64_t begin, 
         }
         __syncthreads();
         const uint32_t len = *s_len;
-        const uint64_t nbase = base + PTILE;
-        const bool more = nbase < end;
+        // the next tile: in this segment, else the first of a later one
+        uint64_t nbase = base + PTILE;
+        bool more = nbase < sg.end;
+        StageSeg ns = sg;
+        while (!more && src.get(++si, ns)) {
+            nbase = tile_start(ns.begin, 8);
+            more = nbase < ns.end;
+        }
         if (more) {
-            load_tile<PBLOCK, PITEMS, NT_LOAD>(in, 8, nbase, begin, end, t, v);
+            load_tile<PBLOCK, PITEMS, NT_LOAD>(ns.in, 8, nbase, ns.begin, ns.end, t, v);
             if constexpr (HASH) hash_keys<PITEMS>(t);
         }
         // the first lane of this wave: batches past the tile's end are skipped
@@ -177,6 +200,7 @@ __device__ __forceinline__ void stage_tiles(const uint64_t *in, uint64_t begin, 
         __syncthreads();
         if (!more) break;
         base = nbase;
+        sg = ns;
     }
 }
 
@@ -213,15 +237,24 @@ __global__ __launch_bounds__(PBLOCK, 4) void k_stage1(Stage1Args a) {
     for (uint32_t d = threadIdx.x; d < F + NSPARE; d += PBLOCK) cnt[d] = 0;
     __syncthreads();
     const uint32_t set = blockIdx.x % (uint32_t)a.nset;
-    StageRes r;
-    r.cur = a.scur + set * F;
-    r.base = set * F * a.scap;
-    r.stride = a.scap;
-    r.cap = a.scap;
-    r.trash = (uint32_t)a.nset * F * a.scap;
-    r.flag = a.flag;
+    struct {  // one segment: this work-group's run of tiles
+        StageSeg s;
+        __device__ bool get(uint32_t i, StageSeg &o) const {
+            o = s;
+            return i == 0;
+        }
+    } src;
+    src.s.in = a.in;
+    src.s.begin = b;
+    src.s.end = e;
+    src.s.r.cur = a.scur + set * F;
+    src.s.r.base = set * F * a.scap;
+    src.s.r.stride = a.scap;
+    src.s.r.cap = a.scap;
+    src.s.r.trash = (uint32_t)a.nset * F * a.scap;
+    src.s.r.flag = a.flag;
     stage_tiles<PBLOCK, (MODE & 1) != 0, (MODE & 2) != 0, (MODE & 16) != 0>(
-        a.in, b, e, a.stage, 0, F, buf, cnt, dst, &s_len, r);
+        src, a.stage, 0, F, buf, cnt, dst, &s_len);
 }
 
 struct Stage2Args {
@@ -230,11 +263,42 @@ struct Stage2Args {
     const uint32_t *scur;  // cursors k_stage1 filled
     uint32_t *scur_next;   // the other super-chunk's cursors: zeroed here
     int b1, b2, nset;
-    uint32_t units;        // units per staging region
+    uint32_t units;        // work items per staging region
+    uint32_t per_wg;       // consecutive work items per work-group
     uint64_t *out;         // final partitions of this relation
     uint32_t fcap;
     uint32_t *fcur;        // F1 * F2 cursors of this relation
     uint32_t *flag;
+};
+
+// Work item k of pass 2 is part j = k mod units of staging region k / units.
+// A work-group takes per_wg consecutive items and runs through them as one
+// tile pipeline (regions are about 4 tiles each at the default geometry, so a
+// work-group per region would load its first tile with nothing to overlap).
+struct Stage2Src {
+    const Stage2Args *a;
+    uint32_t k0, k1;  // this work-group's items
+    __device__ bool get(uint32_t i, StageSeg &o) const {
+        const uint32_t k = k0 + i;
+        if (k >= k1) return false;
+        const uint32_t F1 = 1u << a->b1, F2 = 1u << a->b2;
+        const uint32_t region = k / a->units, j = k % a->units;
+        const uint32_t d1 = region & (F1 - 1);
+        const uint32_t n = min(a->scur[region], a->scap);
+        // parts of whole tile pairs of 8-byte alignment: an even split point
+        // keeps the 16-byte loads aligned
+        const uint32_t per = ((n + a->units - 1) / a->units + 1) & ~1u;
+        o.in = a->stage + (uint64_t)region * a->scap;
+        o.begin = min(j * per, n);
+        o.end = min(j * per + per, n);
+        o.r.cur = a->fcur + d1 * F2;
+        o.r.base = d1 * F2 * a->fcap;
+        o.r.stride = a->fcap;
+        o.r.cap = a->fcap;
+        o.r.trash = F1 * F2 * a->fcap;
+        o.r.flag = a->flag;
+        return true;
+    }
 };
 
 // MODE bits: 4 = staging loads nontemporal, 8 = final stores nontemporal
@@ -243,34 +307,24 @@ __global__ __launch_bounds__(PBLOCK, 4) void k_stage2(Stage2Args a) {
     constexpr int PTILE = PBLOCK * PITEMS;
     extern __shared__ uint64_t smem[];
     __shared__ uint32_t s_len;
-    const uint32_t F1 = 1u << a.b1, F2 = 1u << a.b2;
-    const uint32_t nreg = (uint32_t)a.nset * F1;
+    const uint32_t F2 = 1u << a.b2;
+    const uint32_t nreg = (uint32_t)a.nset << a.b1;
     if (blockIdx.x == 0)
         for (uint32_t i = threadIdx.x; i < nreg; i += PBLOCK) a.scur_next[i] = 0;
-    const uint32_t region = blockIdx.x / a.units, j = blockIdx.x % a.units;
-    if (region >= nreg) return;
-    const uint32_t d1 = region & (F1 - 1);
-    const uint32_t n = min(a.scur[region], a.scap);
-    // units of whole tile pairs of 8-byte alignment: an even split point keeps
-    // the 16-byte loads aligned
-    const uint32_t per = ((n + a.units - 1) / a.units + 1) & ~1u;
-    const uint32_t ub = j * per, ue = min(ub + per, n);
-    if (ub >= ue) return;
+    const uint32_t items = nreg * a.units;
+    const uint32_t k0 = blockIdx.x * a.per_wg;
+    if (k0 >= items) return;
     uint64_t *buf = smem;
     uint32_t *cnt = reinterpret_cast<uint32_t *>(buf + PTILE + NSPARE);
     uint32_t *dst = cnt + F2 + NSPARE;
     for (uint32_t d = threadIdx.x; d < F2 + NSPARE; d += PBLOCK) cnt[d] = 0;
     __syncthreads();
-    StageRes r;
-    r.cur = a.fcur + d1 * F2;
-    r.base = d1 * F2 * a.fcap;
-    r.stride = a.fcap;
-    r.cap = a.fcap;
-    r.trash = F1 * F2 * a.fcap;
-    r.flag = a.flag;
-    stage_tiles<PBLOCK, (MODE & 4) != 0, (MODE & 8) != 0>(a.stage + (uint64_t)region * a.scap, ub,
-                                                          ue, a.out, a.b1, F2, buf, cnt, dst,
-                                                          &s_len, r);
+    Stage2Src src;
+    src.a = &a;
+    src.k0 = k0;
+    src.k1 = min(k0 + a.per_wg, items);
+    stage_tiles<PBLOCK, (MODE & 4) != 0, (MODE & 8) != 0, false>(src, a.out, a.b1, F2, buf, cnt,
+                                                                 dst, &s_len);
 }
 
 // Join tasks of the staged layout: T tasks per final partition p (S slices of
@@ -344,7 +398,13 @@ int staged_geometry(StagedGeom *g, uint64_t nR, uint64_t nS, int b1, int b2, uin
     g->fcap[0] = region_cap((double)nR / F);
     g->fcap[1] = region_cap((double)nS / F);
     const uint32_t nreg = (uint32_t)nset * F1;
-    g->units = (uint32_t)std::max<uint64_t>(1, ((uint64_t)num_cu * per_cu * 2 + nreg - 1) / nreg);
+    // pass 2: at least one work item per work-group of one resident round, and
+    // each work-group takes its share of the items in one pipeline
+    const uint64_t g2 = (uint64_t)num_cu * per_cu;
+    g->units = (uint32_t)std::max<uint64_t>(1, (g2 + nreg - 1) / nreg);
+    const uint64_t items = (uint64_t)nreg * g->units;
+    g->per_wg = (uint32_t)((items + g2 - 1) / g2);
+    g->grid2 = (uint32_t)((items + g->per_wg - 1) / g->per_wg);
     g->stage_tuples = (uint64_t)nreg * g->scap + ptile;
     for (int r = 0; r < 2; ++r) g->out_tuples[r] = (uint64_t)F * g->fcap[r] + ptile;
     if (g->stage_tuples >= (1ull << 32) || g->out_tuples[0] >= (1ull << 32) ||
@@ -394,12 +454,13 @@ static int launch_staged(const StagedDesc &d, hipStream_t st, std::vector<hipEve
             a2.b2 = g.b2;
             a2.nset = g.nset;
             a2.units = g.units;
+            a2.per_wg = g.per_wg;
             a2.out = d.out[rel];
             a2.fcap = g.fcap[rel];
             a2.fcur = d.fcur + (uint64_t)rel * (F1 * F2);
             a2.flag = d.flag;
-            hipLaunchKernelGGL((k_stage2<PBLOCK, MODE2>), dim3(nreg * g.units), dim3(PBLOCK), lds2,
-                               st, a2);
+            hipLaunchKernelGGL((k_stage2<PBLOCK, MODE2>), dim3(g.grid2), dim3(PBLOCK), lds2, st,
+                               a2);
         }
     }
     if (evs) (void)hipEventRecord((*evs)[ev++], st);
