@@ -49,9 +49,10 @@ __device__ __forceinline__ int bucket_lg(uint32_t m) {
 }
 
 // LDS bytes of one round's table: the tuples in bucket order (8 bytes each)
-// plus one spare slot, and nb+1 16-bit bucket starts.
+// plus one spare slot, and the 16-bit bucket bounds (a leading 0, nb bucket
+// ends, a spare cursor).
 __host__ __device__ constexpr size_t join_lds_bytes(uint32_t cap) {
-    return sizeof(uint64_t) * (cap + 1) + sizeof(uint16_t) * (max_buckets(cap) + 2);
+    return sizeof(uint64_t) * (cap + 1) + sizeof(uint16_t) * (max_buckets(cap) + 4);
 }
 
 // S tuples per lane probed together: their candidate tuples are read from LDS
@@ -164,16 +165,18 @@ __device__ __forceinline__ uint32_t opaque_tid() {
 
 // Bucket of a key among 2^lg (hp = lg | sh << 8, lg <= 12, sh <= 31): every
 // key of a partition has the same sh low bits (the partition bits), so the
-// bucket is taken from x = key >> sh, as in bucket_chaining_join
-// (HASH_BIT_MODULO, parallel_radix_join.cpp:68,309): the low lg bits of x,
-// xor the top lg bits of a multiplicative hash of the bits above them.  Keys
-// that differ only in those low bits never share a bucket (a dense key range
-// fills every bucket evenly), and keys that agree in them are spread by the
-// hash of the rest.
+// bucket is taken from the bits above them, as in bucket_chaining_join
+// (HASH_BIT_MODULO, parallel_radix_join.cpp:68,309): the lg bits above the
+// partition bits, xor the next lg bits, xor the lg bits after those (bit-field
+// offsets capped at 31).  Keys that differ only in the first field never share
+// a bucket (a dense key range fills every bucket evenly), and keys that agree
+// in it are spread by the fields above.  Three bit-field extracts and one
+// three-way xor: no multiply (a quarter-rate instruction).
 __device__ __forceinline__ uint32_t bucket_of(uint32_t key, uint32_t hp) {
-    const uint32_t lg = hp & 31u, x = key >> (hp >> 8);
-    const uint32_t hi = __umulhi((x >> lg) * 0x9E3779B1u, 1u << lg);
-    return (x ^ hi) & ((1u << lg) - 1u);
+    const uint32_t lg = hp & 31u, sh = hp >> 8;
+    const uint32_t o2 = min(sh + lg, 31u), o3 = min(sh + 2 * lg, 31u);
+    return __builtin_amdgcn_ubfe(key, sh, lg) ^ __builtin_amdgcn_ubfe(key, o2, lg) ^
+           __builtin_amdgcn_ubfe(key, o3, lg);
 }
 
 // Exclusive scan of the nb (a power of two, at most 8*JB) bucket counters
@@ -206,15 +209,15 @@ __device__ __forceinline__ void bucket_scan(const uint32_t *cnt, uint16_t *bst, 
 
 // Counting-sort the m (<= CAP) tuples of R held in registers (element q of
 // `rv` is tuple tid + q*JB, valid below m) by bucket into `tab`.  Count
-// (counters in `tab`), scan into the 16-bit bucket starts `bst`, then place
-// each tuple at a position taken by an LDS atomic on its bucket's 16-bit
-// cursor (two cursors share one 32-bit word; no carry crosses the halves since
-// every cursor stays below 2^16).  After the build bst[h] is the END of bucket
-// h: bucket h spans [h ? bst[h-1] : 0, bst[h]).  No per-tuple rank is kept in
-// registers.  Branch-free: lanes without a tuple count into the spare counter
-// nb+1, advance the spare cursor nb+1 and store into the spare slot tab[CAP].
-// `sh` is the partition's radix bits.  Returns the hash parameters
-// hp = log2(nb) | sh << 8 for bucket_of.
+// (counters in `tab`), scan into the 16-bit bounds `bst` (bst[0] = 0, bucket
+// h's cursor at bst[h+1], starting at its first slot), then place each tuple
+// at a position taken by an LDS atomic on its bucket's 16-bit cursor (two
+// cursors share one 32-bit word; no carry crosses the halves since every
+// cursor stays below 2^16).  After the build bucket h spans
+// [bst[h], bst[h+1]).  Branch-free: lanes without a tuple count into the spare
+// counter nb+1, advance the spare cursor bst[nb+2] and store into the spare
+// slot tab[CAP].  `sh` is the partition's radix bits.  Returns the hash
+// parameters hp = log2(nb) | sh << 8 for bucket_of.
 template <int JB, int CAP>
 __device__ __forceinline__ uint32_t build_regs(const uint64_t (&rv)[CAP / JB], uint32_t m,
                                                uint32_t sh, uint64_t *tab, uint16_t *bst,
@@ -226,29 +229,31 @@ __device__ __forceinline__ uint32_t build_regs(const uint64_t (&rv)[CAP / JB], u
     const uint32_t nb = 1u << lg;
     uint32_t *cnt = reinterpret_cast<uint32_t *>(tab);
     for (uint32_t i = threadIdx.x; i < nb + 2; i += JB) cnt[i] = 0;
-    if (threadIdx.x == 0) bst[nb + 1] = 0;
+    if (threadIdx.x == 0) {
+        bst[0] = 0;
+        bst[nb + 2] = 0;
+    }
     __syncthreads();
     // items no lane of this wave holds are skipped (a wave-uniform branch);
     // in a partly filled item, lanes without a tuple use the spare counter,
     // cursor and slot
     const uint32_t wave0 = __builtin_amdgcn_readfirstlane(tid);
+    uint32_t hq[ITEMS];
     unroll<ITEMS>([&](auto q) {
         if (wave0 + q * JB < m) {
-            const uint32_t h = bucket_of(key_of(rv[q]), hp);
-            atomicAdd(&cnt[tid + q * JB < m ? h : nb + 1], 1u);
+            hq[q] = tid + q * JB < m ? bucket_of(key_of(rv[q]), hp) : nb + 1;
+            atomicAdd(&cnt[hq[q]], 1u);
         }
     });
     __syncthreads();
-    bucket_scan<JB>(cnt, bst, nb, scan_tmp, tid);
+    bucket_scan<JB>(cnt, bst + 1, nb, scan_tmp, tid);
     uint32_t *cur = reinterpret_cast<uint32_t *>(bst);
     unroll<ITEMS>([&](auto q) {
         if (wave0 + q * JB < m) {
-            const bool ok = tid + q * JB < m;
-            const uint32_t hb = bucket_of(key_of(rv[q]), hp);
-            const uint32_t h = ok ? hb : nb + 1;
-            const uint32_t half = (h & 1u) * 16;
-            const uint32_t old = atomicAdd(&cur[h >> 1], 1u << half);
-            tab[ok ? (old >> half) & 0xffffu : (uint32_t)CAP] = rv[q];
+            const uint32_t c = hq[q] + 1;  // the bucket's cursor
+            const uint32_t half = (c & 1u) * 16;
+            const uint32_t old = atomicAdd(&cur[c >> 1], 1u << half);
+            tab[tid + q * JB < m ? (old >> half) & 0xffffu : (uint32_t)CAP] = rv[q];
         }
     });
     __syncthreads();
@@ -274,8 +279,8 @@ __device__ __forceinline__ void probe_part(const uint64_t (&sv)[N], uint32_t ns,
         uint32_t be[G];  // bucket cursor | end << 16
         unroll<G>([&](auto j) {
             const uint32_t h = bucket_of(key_of(sv[G0 + j]), hp);
-            const uint32_t lo = bst[h ? h - 1 : 0], hi = bst[h];
-            be[j] = tid + (uint32_t)(Q0 + G0 + j) * JB < ns ? (h ? lo : 0u) | (hi << 16) : 0u;
+            const uint32_t lo = bst[h], hi = bst[h + 1];
+            be[j] = tid + (uint32_t)(Q0 + G0 + j) * JB < ns ? lo | (hi << 16) : 0u;
         });
         for (;;) {
             uint64_t r[G];
@@ -310,12 +315,16 @@ __device__ __forceinline__ void probe_part(const uint64_t (&sv)[N], uint32_t ns,
 // Counting probe (no pair output): PGROUP S tuples of a lane step through their
 // buckets together, for as many steps as the longest bucket of any tuple in the
 // wave (a wave-uniform trip count, so the loop control is scalar and the body is
-// branch-free).  Step i reads candidate lo+i of every tuple (clamped to the
-// spare slot) and counts it when i < len and the keys are equal.  Matches of
-// tuple j are counted in m[j]; the checksum is rebuilt from its two halves:
-// sum of R payloads (mod 2^32, the high word) and m[j] * S payload.
+// branch-free).  A tuple whose bucket [lo, hi) is shorter reads the `steps`
+// slots from min(lo, m - steps): they cover its bucket and never leave the
+// round's m tuples, and a slot outside the bucket never holds the probing key
+// (every R tuple with that key is in the bucket), so each read needs neither a
+// clamp nor a bound test, only the key compare.  Matches of tuple j are counted
+// in mt[j]; the checksum is rebuilt from its two halves: the sum of R payloads
+// (mod 2^32, the high word) and mt[j] * S payload.  Lanes without a tuple
+// (past ns) are masked out of both.
 template <int JB, int CAP, int Q0, int N, int G0 = 0>
-__device__ __forceinline__ void probe_count(const uint64_t (&sv)[N], uint32_t ns,
+__device__ __forceinline__ void probe_count(const uint64_t (&sv)[N], uint32_t ns, uint32_t m,
                                             const uint64_t *tab, const uint16_t *bst, uint32_t hp,
                                             uint64_t &cnt, uint64_t &sum, uint32_t tid) {
     if constexpr (G0 < N) {
@@ -325,43 +334,56 @@ __device__ __forceinline__ void probe_count(const uint64_t (&sv)[N], uint32_t ns
         // candidates are read as {payload, key} word pairs so that the key test
         // is one 32-bit compare
         const uint2 *tab2 = reinterpret_cast<const uint2 *>(tab);
-        const uint2 *spare = tab2 + CAP;  // reads past a bucket are clamped here
-        const uint2 *lo[G];
-        uint32_t len[G], sk[G];
+        uint32_t lo[G], sk[G];
+        bool ok[G];
         uint32_t mx = 0;
         unroll<G>([&](auto j) {
             uint32_t k = key_of(sv[G0 + j]);
             asm volatile("" : "+v"(k));  // a 32-bit value to the compiler: 32-bit key tests
             sk[j] = k;
             const uint32_t h = bucket_of(k, hp);
-            const uint32_t a = bst[h ? h - 1 : 0], e = bst[h];
-            const uint32_t b = h ? a : 0u;
-            lo[j] = tab2 + b;
-            len[j] = tid + (uint32_t)(Q0 + G0 + j) * JB < ns ? e - b : 0u;
-            mx = mx > len[j] ? mx : len[j];
+            const uint32_t b = bst[h], e = bst[h + 1];
+            ok[j] = tid + (uint32_t)(Q0 + G0 + j) * JB < ns;
+            lo[j] = b;
+            const uint32_t len = ok[j] ? e - b : 0u;
+            mx = mx > len ? mx : len;
         });
-        const uint32_t steps = __builtin_amdgcn_readfirstlane(wave_max_u32(mx));
-        uint32_t m[G];
+        const uint32_t steps = __builtin_amdgcn_readfirstlane(wave_max_u32_dpp(mx));
+        const uint32_t lim = m - steps;  // steps <= the longest bucket <= m
+        const uint2 *q[G];
+        unroll<G>([&](auto j) { q[j] = tab2 + (lo[j] < lim ? lo[j] : lim); });
+        uint32_t mt[G];
         uint32_t rsum = 0;
-        unroll<G>([&](auto j) { m[j] = 0; });
-        for (uint32_t i = 0; i < steps; ++i) {
-            uint2 r[G];
+        unroll<G>([&](auto j) { mt[j] = 0; });
+        uint32_t i = 0;
+        for (; i + 2 <= steps; i += 2) {  // two slots per read pair
+            uint2 r0[G], r1[G];
             unroll<G>([&](auto j) {
-                const uint2 *q = lo[j] + i;
-                r[j] = *(q < spare ? q : spare);
+                r0[j] = q[j][0];
+                r1[j] = q[j][1];
+                q[j] += 2;
             });
             unroll<G>([&](auto j) {
-                const bool hit = (int)(i < len[j]) & (int)(r[j].y == sk[j]);
-                m[j] += hit ? 1u : 0u;
-                rsum += hit ? r[j].x : 0u;
+                const bool h0 = ok[j] & (r0[j].y == sk[j]), h1 = ok[j] & (r1[j].y == sk[j]);
+                mt[j] += (h0 ? 1u : 0u) + (h1 ? 1u : 0u);
+                rsum += (h0 ? r0[j].x : 0u) + (h1 ? r1[j].x : 0u);
+            });
+        }
+        if (i < steps) {
+            uint2 r0[G];
+            unroll<G>([&](auto j) { r0[j] = q[j][0]; });
+            unroll<G>([&](auto j) {
+                const bool h0 = ok[j] & (r0[j].y == sk[j]);
+                mt[j] += h0 ? 1u : 0u;
+                rsum += h0 ? r0[j].x : 0u;
             });
         }
         unroll<G>([&](auto j) {
-            cnt += m[j];
-            sum += (uint64_t)m[j] * payload_of(sv[G0 + j]);
+            cnt += mt[j];
+            sum += (uint64_t)mt[j] * payload_of(sv[G0 + j]);
         });
         sum += (uint64_t)rsum << 32;
-        probe_count<JB, CAP, Q0, N, G0 + PGROUP>(sv, ns, tab, bst, hp, cnt, sum, tid);
+        probe_count<JB, CAP, Q0, N, G0 + PGROUP>(sv, ns, m, tab, bst, hp, cnt, sum, tid);
     }
 }
 
@@ -401,7 +423,7 @@ __global__ __launch_bounds__(JB, 4) void k_join(JoinArgs a) {
     static_assert(ITEMS * JB == CAP, "CAP must be a multiple of the work-group size");
     extern __shared__ uint64_t jsm[];
     uint64_t *tab = jsm;  // CAP tuples in bucket order + a spare slot
-    uint16_t *bst = reinterpret_cast<uint16_t *>(jsm + CAP + 1);  // max_buckets(CAP) + 1 starts
+    uint16_t *bst = reinterpret_cast<uint16_t *>(jsm + CAP + 1);  // max_buckets(CAP) + 3 bounds
     __shared__ uint64_t red[2 * (JB / 64)];
     __shared__ uint32_t scan_tmp[JB / 64];
     __shared__ uint64_t s_base;
@@ -450,8 +472,8 @@ __global__ __launch_bounds__(JB, 4) void k_join(JoinArgs a) {
                 uint64_t sb[SB];
                 load_part<JB, SA, SB>(sb, a.S + cur.s0 + so, sub, opaque_tid());
                 uint64_t c = 0, x = 0, wpos = 0;
-                probe_count<JB, CAP, 0, SA>(sa, sub, tab, bst, hp, c, x, opaque_tid());
-                probe_count<JB, CAP, SA, SB>(sb, sub, tab, bst, hp, c, x, opaque_tid());
+                probe_count<JB, CAP, 0, SA>(sa, sub, m, tab, bst, hp, c, x, opaque_tid());
+                probe_count<JB, CAP, SA, SB>(sb, sub, m, tab, bst, hp, c, x, opaque_tid());
                 if (PAIRS) {
                     uint32_t total_c;
                     const uint32_t pre = block_exclusive_scan<JB>((uint32_t)c, scan_tmp, &total_c);
