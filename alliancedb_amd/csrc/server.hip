@@ -144,6 +144,59 @@ __global__ void k_gen_tuples(uint32_t *out, uint64_t n, uint64_t seed, uint32_t 
     }
 }
 
+// Many window fills in one launch: fill f writes n tuples at out as
+// k_gen_tuples would.  The work is cut into chunks of FILL_CHUNK tuples;
+// chunk0[f] is fill f's first chunk.  Fills of the same memory never share a
+// launch (the server flushes first).
+constexpr int FILL_MAX = 16;
+constexpr uint64_t FILL_CHUNK = 4096;
+struct FillDesc {
+    uint32_t *out;
+    uint64_t n, seed;
+    uint32_t domain, payload0;
+    int zipf;
+    ZipfC z;
+    Perm p;
+};
+struct FillBatch {
+    FillDesc d[FILL_MAX];
+    uint64_t chunk0[FILL_MAX + 1];
+    int nd;
+};
+
+__global__ __launch_bounds__(256) void k_gen_fills(FillBatch b) {
+    const uint64_t total = b.chunk0[b.nd];
+    for (uint64_t c = blockIdx.x; c < total; c += gridDim.x) {
+        int d = 0;
+        while (d + 1 < b.nd && b.chunk0[d + 1] <= c) ++d;
+        const FillDesc &f = b.d[d];
+        const uint64_t i0 = (c - b.chunk0[d]) * FILL_CHUNK;
+        const uint64_t i1 = min(i0 + FILL_CHUNK, f.n);
+        for (uint64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+            uint32_t key;
+            if (f.zipf) key = 1u + (uint32_t)perm_apply(f.p, zipf_rank(f.z, f.seed, i) - 1);
+            else key = 1u + (uint32_t)__umul64hi(g_stream_at(f.seed, i), (uint64_t)f.domain);
+            f.out[3 * i + 0] = key;
+            f.out[3 * i + 1] = f.payload0 + (uint32_t)i;
+            f.out[3 * i + 2] = 0u;
+        }
+    }
+}
+
+FillDesc make_fill(aj_stream_tuple_t *out, uint64_t n, uint64_t seed, uint32_t domain,
+                   double theta, uint32_t payload0) {
+    FillDesc f;
+    f.out = (uint32_t *)out;
+    f.n = n;
+    f.seed = seed;
+    f.domain = domain;
+    f.payload0 = payload0;
+    f.zipf = theta > 0;
+    f.z = f.zipf ? make_zipf(domain, theta) : ZipfC{};
+    f.p = make_perm(domain, seed ^ 0x5bd1e995ull);
+    return f;
+}
+
 }  // namespace
 
 int gen_stream_tuples(Ctx *c, aj_stream_tuple_t *d_out, uint64_t n, uint64_t seed,
@@ -208,6 +261,9 @@ struct Server {
     std::mutex ingest_mu;
     hipStream_t ingest = nullptr;
     hipEvent_t ingest_ev = nullptr;
+    // window fills not yet launched (under ingest_mu): launched together, at
+    // the latest before the next ingest-stream operation or batch
+    std::vector<FillDesc> fills;
 
     std::thread dispatcher, completer;
     std::chrono::steady_clock::time_point t0;
@@ -217,6 +273,41 @@ struct Server {
             .count();
     }
 };
+
+// Launches the pending window fills on the ingest stream, FILL_MAX per
+// launch (caller holds ingest_mu).  Everything enqueued on the ingest stream
+// later runs after them, as if each fill had been launched when it was made.
+static int flush_fills(Server *s) {
+    size_t i = 0;
+    while (i < s->fills.size()) {
+        FillBatch b;
+        b.nd = 0;
+        uint64_t ch = 0;
+        for (; i < s->fills.size() && b.nd < FILL_MAX; ++i) {
+            b.d[b.nd] = s->fills[i];
+            b.chunk0[b.nd] = ch;
+            ch += (s->fills[i].n + FILL_CHUNK - 1) / FILL_CHUNK;
+            ++b.nd;
+        }
+        b.chunk0[b.nd] = ch;
+        if (ch) {
+            const unsigned grid = (unsigned)std::min<uint64_t>(ch, (uint64_t)s->ctx->num_cu * 8);
+            hipLaunchKernelGGL(k_gen_fills, dim3(grid), dim3(256), 0, s->ingest, b);
+        }
+    }
+    s->fills.clear();
+    AJ_HIP(hipGetLastError());
+    return AJ_OK;
+}
+
+// Before window memory goes back to the arena without a batch having read it:
+// a pending fill of it must reach the stream before any later window's.
+static int flush_fills_locked(Server *s) {
+    std::lock_guard<std::mutex> lk(s->ingest_mu);
+    if (s->fills.empty()) return AJ_OK;
+    AJ_HIP(hipSetDevice(s->ctx->device));
+    return flush_fills(s);
+}
 
 static void publish(Server *s, SReq *r, uint64_t count, uint64_t sum, int status, int nwin,
                     double start_us, double done_us) {
@@ -256,6 +347,7 @@ static int launch_batch(Server *s, SSlot *sl) {
     }
     {
         std::lock_guard<std::mutex> lk(s->ingest_mu);
+        AJ_TRY(flush_fills(s));
         AJ_HIP(hipEventRecord(s->ingest_ev, s->ingest));
         AJ_HIP(hipStreamWaitEvent(sl->st, s->ingest_ev, 0));
     }
@@ -293,6 +385,7 @@ static void dispatcher_main(Server *s) {
             SReq *r = s->pending.front();
             if (r->nR == 0 || r->nS == 0) {  // nothing to join
                 s->pending.pop_front();
+                (void)flush_fills_locked(s);
                 publish(s, r, 0, 0, AJ_OK, 1, now, now);
                 continue;
             }
@@ -571,6 +664,7 @@ int aj_server_window_release(aj_server_t *srv, const aj_window_t *win) {
         set_error(AJ_ERR_STATE, "window %llu is already submitted", (unsigned long long)r->id);
         return AJ_ERR_STATE;
     }
+    AJ_TRY(flush_fills_locked(&s));
     s.adm.release(r->r_off, r->nR);
     s.adm.release(r->s_off, r->nS);
     s.live.erase(r->id);
@@ -594,11 +688,24 @@ int aj_server_window_fill(aj_server_t *srv, const aj_window_t *win, uint64_t see
             return AJ_ERR_STATE;
         }
     }
+    if (key_domain == 0 || !(theta_s >= 0) || theta_s > 10) {
+        set_error(AJ_ERR_ARG, "aj_server_window_fill: bad key domain or theta");
+        return AJ_ERR_ARG;
+    }
     AJ_HIP(hipSetDevice(s.ctx->device));
     std::lock_guard<std::mutex> lk(s.ingest_mu);
-    AJ_TRY(gen_stream_tuples(s.ctx, win->dR, win->nR, seed, key_domain, 0.0, 0u, s.ingest));
-    AJ_TRY(gen_stream_tuples(s.ctx, win->dS, win->nS, seed ^ 0x5DEECE66Dull, key_domain, theta_s,
-                             0u, s.ingest));
+    // a second fill of the window: the first goes out first (one launch must
+    // not write the same memory twice)
+    for (const FillDesc &f : s.fills)
+        if (f.out == (uint32_t *)win->dR || f.out == (uint32_t *)win->dS) {
+            AJ_TRY(flush_fills(&s));
+            break;
+        }
+    if (win->nR) s.fills.push_back(make_fill(win->dR, win->nR, seed, key_domain, 0.0, 0u));
+    if (win->nS)
+        s.fills.push_back(
+            make_fill(win->dS, win->nS, seed ^ 0x5DEECE66Dull, key_domain, theta_s, 0u));
+    if (s.fills.size() >= (size_t)FILL_MAX) AJ_TRY(flush_fills(&s));
     return AJ_OK;
 }
 
@@ -620,6 +727,7 @@ int aj_server_window_upload(aj_server_t *srv, const aj_window_t *win,
     }
     AJ_HIP(hipSetDevice(s.ctx->device));
     std::lock_guard<std::mutex> lk(s.ingest_mu);
+    AJ_TRY(flush_fills(&s));
     if (win->nR)
         AJ_HIP(hipMemcpyAsync(win->dR, hostR, 12 * win->nR, hipMemcpyHostToDevice, s.ingest));
     if (win->nS)
