@@ -155,3 +155,70 @@ def test_server_misuse_returns_status(aj):
         with pytest.raises(aj.AjError) as e:
             aj.Server(ctx, hbm_budget=1 << 20)    # budget below the workspaces
         assert e.value.status == aj.AJ_ERR_CAPACITY
+
+
+def test_server_fill_order_across_release_refill_and_upload(aj, oracle):
+    """Window fills are queued and launched together; every later use of the
+    same memory still sees them in the order they were made: a filled window
+    released unsubmitted gives its memory to the next window (whose own fill
+    must win), a window filled twice keeps the second fill, a filled window
+    overwritten by an upload keeps the uploaded tuples, and a window with an
+    empty side is published without a batch while its other side's fill is
+    queued."""
+    with aj.Context(0) as ctx:
+        with aj.Server(ctx, hbm_budget=3 << 30, slots=2, max_batch_windows=8,
+                       max_batch_tuples=1 << 20, max_window_tuples=1 << 18) as srv:
+            n = 1 << 15
+            specs = {}
+            # released after a fill: the memory goes to the next window
+            w0 = srv.alloc(n, n)
+            srv.fill(w0, 11, 1 << 12, 1.0)
+            srv.release(w0)
+            w1 = srv.alloc(n, n)
+            srv.fill(w1, 12, 1 << 14, 0.0)
+            srv.submit(w1, tag=1)
+            specs[1] = (n, n, 12, 1 << 14, 0.0)
+            # filled twice: the second fill is the window's content
+            w2 = srv.alloc(n, 2 * n)
+            srv.fill(w2, 21, 1 << 10, 0.0)
+            srv.fill(w2, 22, 1 << 13, 1.0)
+            srv.submit(w2, tag=2)
+            specs[2] = (n, 2 * n, 22, 1 << 13, 1.0)
+            # an empty side: published at once, its R fill still queued
+            w3 = srv.alloc(n, 0)
+            srv.fill(w3, 31, 1 << 12, 0.0)
+            srv.submit(w3, tag=3)
+            w4 = srv.alloc(n, n)
+            srv.fill(w4, 41, 1 << 12, 0.0)
+            srv.submit(w4, tag=4)
+            specs[4] = (n, n, 41, 1 << 12, 0.0)
+            # filled, then uploaded: the upload is the window's content
+            rng = np.random.default_rng(5)
+            up = []
+            for _ in range(2):
+                t = np.zeros(n, dtype=aj.STREAM_DTYPE)
+                t["key"] = rng.integers(1, 1 << 11, n)
+                t["payload"] = np.arange(n)
+                up.append(t)
+            R5, S5 = (aj.as_tuples(t["key"].view(np.int32), t["payload"].view(np.int32))
+                      for t in up)
+            w5 = srv.alloc(n, n)
+            srv.fill(w5, 52, 1 << 15, 1.0)
+            srv.upload(w5, up[0], up[1])
+            srv.submit(w5, tag=5)
+            srv.drain()
+            res = {}
+            while True:
+                got = srv.poll(64, 0)
+                if not got:
+                    break
+                for r in got:
+                    res[r.tag] = r
+            assert sorted(res) == [1, 2, 3, 4, 5]
+            assert (res[3].count, res[3].checksum) == (0, 0)
+            for tag, spec in specs.items():
+                _, _, want = _expected(aj, oracle, ctx, spec)
+                assert (res[tag].count, res[tag].checksum) == want, tag
+            assert (res[5].count, res[5].checksum) == oracle.aggregate(R5, S5)
+            assert all(r.status == aj.AJ_OK for r in res.values())
+            assert srv.stats()["window_bytes_used"] == 0
