@@ -518,8 +518,11 @@ int aj_server_window_alloc(aj_server_t *srv, uint64_t nR, uint64_t nS, int64_t t
 int aj_server_window_release(aj_server_t *srv, const aj_window_t *win);
 /* R keys uniform on [1, key_domain]; S keys uniform (theta_s = 0) or
  * Zipf(theta_s) over key_domain (aj_gen_stream_tuples with seeds
- * seed and seed ^ 0x5DEECE66D).  Asynchronous on the ingest stream; batches
- * wait for it on the device. */
+ * seed and seed ^ 0x5DEECE66D).  Asynchronous: fills are queued and launched
+ * together on the ingest stream (up to 16 window sides per launch), at the
+ * latest before the next batch, upload or release of window memory; batches
+ * wait for them on the device.  key_domain 0 or theta_s outside [0, 10] is
+ * AJ_ERR_ARG. */
 int aj_server_window_fill(aj_server_t *srv, const aj_window_t *win, uint64_t seed,
                           uint32_t key_domain, double theta_s);
 /* Copies host tuples into the window (blocking). */
