@@ -364,7 +364,7 @@ __device__ __forceinline__ void probe_count(const uint64_t (&sv)[N], uint32_t ns
                 q[j] += 2;
             });
             unroll<G>([&](auto j) {
-                const bool h0 = ok[j] & (r0[j].y == sk[j]), h1 = ok[j] & (r1[j].y == sk[j]);
+                const bool h0 = (int)ok[j] & (int)(r0[j].y == sk[j]), h1 = (int)ok[j] & (int)(r1[j].y == sk[j]);
                 mt[j] += (h0 ? 1u : 0u) + (h1 ? 1u : 0u);
                 rsum += (h0 ? r0[j].x : 0u) + (h1 ? r1[j].x : 0u);
             });
@@ -373,7 +373,7 @@ __device__ __forceinline__ void probe_count(const uint64_t (&sv)[N], uint32_t ns
             uint2 r0[G];
             unroll<G>([&](auto j) { r0[j] = q[j][0]; });
             unroll<G>([&](auto j) {
-                const bool h0 = ok[j] & (r0[j].y == sk[j]);
+                const bool h0 = (int)ok[j] & (int)(r0[j].y == sk[j]);
                 mt[j] += h0 ? 1u : 0u;
                 rsum += h0 ? r0[j].x : 0u;
             });
