@@ -100,6 +100,7 @@ struct Ctx {
     std::vector<hipEvent_t> evs;          // per-launch timing of the staged passes
 };
 bool kernel_timing_enabled();
+bool staged_enabled();  // option "staged"
 // option npo_region_min: smallest |R| whose NPO table is built by regions (0 = never)
 uint64_t npo_region_min();
 // Events of a context that may have none: recording is skipped, times are 0.
@@ -166,6 +167,10 @@ struct PassDesc {
     // per chunk into hist2d (chunks x 2^(max_bits + bits2)), see run_pass2_2d
     uint32_t *hist2d = nullptr;
     int bits2 = 0;
+    // nullable: every kernel of the pass except the planning returns at once
+    // while *guard is 0 (a fallback that runs only after an overflow)
+    const uint32_t *guard = nullptr;
+    bool skip_plan = false;  // the planning kernel already ran (run_partition_plan)
 };
 
 // Workspace carved out of ctx buffers by the pass (device pointers).
@@ -186,6 +191,49 @@ size_t pass_scan_tmp_bytes(uint64_t max_hist);
 // scan, m0+3 after the scatter
 int run_partition_pass(const PassDesc &d, const PassWork &w, hipStream_t st,
                        const Marks *marks = nullptr, int m0 = 0);
+// Only the planning kernel of a multi-segment pass (chunk, output and partition
+// bases per segment), for callers that partition by other means first.
+int run_partition_plan(const PassDesc &d, const PassWork &w, hipStream_t st);
+
+// Staged window pass (12-byte window tuples, one pass of bits[w] radix bits per
+// window, no histogram): every partition of window w gets cap[w] slots at
+// base[w] + d * cap[w]; a tile's run for a digit is placed by one atomic add on
+// that partition's cursor.  An S run that does not fit its partition goes to a
+// spill area (after the S layout) and is listed; run_window_parts then gathers
+// each partition's spilled runs into one extension range (after the spill
+// area).  When the layout does not fit the output buffers, an R run does not
+// fit its partition, or the spill area or its list is full, *flag is set; the
+// caller then runs the histogram pass with guard = flag, which rewrites the
+// output exactly.
+constexpr int WSTAGE_MAX_BITS = 11;  // the LDS of a 1024-lane tile holds 2^11 digits
+struct WStageDesc {
+    uint32_t *cap;            // 2 * W per-window partition capacities (R, then S)
+    uint64_t *base;           // 2 * W per-window output bases
+    uint32_t *cur[2];         // max_parts cursors per relation, zeroed by the caller
+    uint32_t *flag, *lflag;   // zeroed by the caller
+    uint64_t out_cap[2];      // tuples of each layout (the partitions + a trash tile)
+    int W;                    // stride of cap / base
+    uint64_t *pstart[2];      // out: 2 * max_parts partition starts per relation
+    uint32_t *plen[2];        // out: 2 * max_parts partition lengths per relation
+                              // (entry max_parts + p: partition p's S extension)
+    uint32_t *strad;          // max_parts (no init needed)
+    uint32_t *spill_cur, *spill_n, *ext_cur;  // zeroed by the caller
+    uint4 *spill_list;        // spill_max entries
+    uint64_t spill_cap;       // tuples of the spill area, and of the extension area
+    uint32_t spill_max;
+    uint32_t *ext_off;        // max_parts (no init needed)
+};
+// The S output buffer of a staged window pass holds out_cap[1] + 2 * spill_cap tuples.
+// Tuples an output buffer needs for n tuples in at most `parts` partitions.
+uint64_t window_stage_bound(uint64_t n, uint64_t parts);
+// After run_partition_plan: capacities, bases and the scatter.
+int run_window_stage(const PassDesc &d, const PassWork &w, const WStageDesc &ws,
+                     hipStream_t st);
+// After the guarded histogram pass: the partitions' starts and lengths, from
+// the staged layout or (after an overflow) the exact one, and the spilled S runs
+// gathered into their extensions.
+int run_window_parts(const PassDesc &d, const PassWork &w, const WStageDesc &ws,
+                     hipStream_t st);
 
 // Pass 2 of a two-pass one-segment PRO driven by the 2-D histogram of pass 1
 // (no pass-2 histogram kernel).  Reads the pass-1 output and workspace.
@@ -252,8 +300,12 @@ int staged_tasks(const StagedDesc &d, uint32_t split, uint4 *task_info, uint32_t
 struct JoinDesc {
     const uint64_t *R, *S;          // partitioned tuples
     const uint64_t *roff, *soff;    // partition offsets (nparts+1), device
+    const uint32_t *rlen = nullptr, *slen = nullptr;  // nullable: partition p is
+                                    // [roff[p], + rlen[p]) instead of up to roff[p+1]
     const uint64_t *nparts_dev;     // number of partitions (device scalar)
     uint64_t max_parts;
+    uint64_t nreal = 0;             // nonzero: entry p >= nreal is another S range of
+                                    // partition p - nreal (its segment is that one's)
     const uint32_t *part_seg_base;  // nseg+1: first partition of each segment
     int nseg;
     int hash_shift;                 // radix bits of every partition (bucket hash skips them)
@@ -323,6 +375,7 @@ size_t npo_acc_words(int num_cu);  // u64 words of NpoDesc::acc
 // Workspace and launch geometry of batched window joins (windows.hip).
 struct BatchLayout {
     size_t rb, re, sb, se, bits, cpre[2], hb[2], ob[2], pb, hdr, poff[2], tcnt, tbase, tinfo, tres, scal;
+    size_t wcap, wbase, wcur[2], pstart[2], plen[2], strad, extoff, splist;  // the staged window pass
     size_t bytes;
 };
 
@@ -336,6 +389,10 @@ struct BatchPlan {  // sized for at most max_R/max_S tuples in windows, max_win 
     uint32_t split = 1 << 16;
     uint32_t target = 4096;
     uint64_t max_chunks = 0, max_hist = 0, max_parts = 0;
+    bool staged = false;           // histogram-free window pass first (option "staged")
+    uint64_t out_cap[2] = {0, 0};  // tuples of the R / S layouts
+    uint64_t spill_cap = 0;        // tuples of the S spill area (and of the extension area)
+    uint32_t spill_max = 0;        // spilled runs listed
     BatchLayout L{};
     DevBuf work, hist, scan_tmp, outR, outS;
     size_t scan_bytes = 0;
@@ -375,8 +432,9 @@ int copy_d2d(void *dst, const void *src, uint64_t bytes, int num_cu, hipStream_t
 // Prefix sums of n counters into out (not in place), modulo the type's width,
 // on st; tmp of scan_tmp_bytes(n, sizeof element) bytes (partition.hip).
 size_t scan_tmp_bytes(uint64_t n, size_t elem);
+// guard (nullable): the scan's kernels return at once while *guard is 0
 int exclusive_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, void *tmp,
-                       size_t tmp_bytes, hipStream_t st);
+                       size_t tmp_bytes, hipStream_t st, const uint32_t *guard = nullptr);
 int inclusive_scan_u64(const uint64_t *in, uint64_t *out, uint64_t n, void *tmp,
                        size_t tmp_bytes, hipStream_t st);
 // one random-access calibration launch (aj_access_kernel) over `slots` 16-byte slots

@@ -120,6 +120,7 @@ static std::atomic<int64_t> opt_materialize{0}, opt_radix_bits{0}, opt_passes{0}
 
 static thread_local bool g_force_timing = false;  // phase timing for one drop-in call
 bool kernel_timing_enabled() { return g_force_timing || opt_kernel_timing.load() != 0; }
+bool staged_enabled() { return opt_staged.load() != 0; }
 uint64_t npo_region_min() { return (uint64_t)opt_npo_region_min.load(); }
 
 static int ctx_init(Ctx *c, int device) {

@@ -61,9 +61,11 @@ constexpr int PGROUP = 6;
 
 struct JoinArgs {
     const uint64_t *R, *S, *roff, *soff;
+    const uint32_t *rlen, *slen;  // nullable: partition p is [roff[p], + rlen[p]) (else up to roff[p+1])
     const uint32_t *part_seg_base;
     int nseg;
     uint64_t max_parts;
+    uint64_t nreal;  // nonzero: entry p >= nreal belongs to partition p - nreal
     uint32_t split;
     uint64_t *seg_count, *seg_sum;
     uint64_t *pairs;
@@ -78,26 +80,31 @@ struct JoinArgs {
 };
 
 // Per segment k: count/checksum = sums over its tasks, which are contiguous:
-// [task_base[part_seg_base[k]], task_base[part_seg_base[k+1]]).  One block per
-// segment (grid-stride), plain stores: the accumulators need no zeroing.
+// [task_base[part_seg_base[k]], task_base[part_seg_base[k+1]]), and with nreal
+// set also the tasks of its partitions' further S ranges, the same range of
+// entries shifted by nreal.  One block per segment (grid-stride), plain stores:
+// the accumulators need no zeroing.
 constexpr int RBLOCK = 1024;
 __global__ __launch_bounds__(RBLOCK) void k_seg_reduce(JoinArgs a) {
     __shared__ uint64_t red[2 * (RBLOCK / 64)];
     for (int k = blockIdx.x; k < a.nseg; k += gridDim.x) {
-        const uint32_t tb = a.task_base[a.part_seg_base[k]];
-        const uint32_t te = a.task_base[a.part_seg_base[k + 1]];
         uint64_t c = 0, x = 0;
-        uint32_t t = tb + threadIdx.x;
-        for (; t + 3 * RBLOCK < te; t += 4 * RBLOCK) {  // four loads in flight
-            const ulonglong2 r0 = a.task_res[t], r1 = a.task_res[t + RBLOCK];
-            const ulonglong2 r2 = a.task_res[t + 2 * RBLOCK], r3 = a.task_res[t + 3 * RBLOCK];
-            c += r0.x + r1.x + r2.x + r3.x;
-            x += r0.y + r1.y + r2.y + r3.y;
-        }
-        for (; t < te; t += RBLOCK) {
-            const ulonglong2 r = a.task_res[t];
-            c += r.x;
-            x += r.y;
+        for (int h = 0; h < (a.nreal ? 2 : 1); ++h) {
+            const uint64_t sh = h ? a.nreal : 0;
+            const uint32_t tb = a.task_base[sh + a.part_seg_base[k]];
+            const uint32_t te = a.task_base[sh + a.part_seg_base[k + 1]];
+            uint32_t t = tb + threadIdx.x;
+            for (; t + 3 * RBLOCK < te; t += 4 * RBLOCK) {  // four loads in flight
+                const ulonglong2 r0 = a.task_res[t], r1 = a.task_res[t + RBLOCK];
+                const ulonglong2 r2 = a.task_res[t + 2 * RBLOCK], r3 = a.task_res[t + 3 * RBLOCK];
+                c += r0.x + r1.x + r2.x + r3.x;
+                x += r0.y + r1.y + r2.y + r3.y;
+            }
+            for (; t < te; t += RBLOCK) {
+                const ulonglong2 r = a.task_res[t];
+                c += r.x;
+                x += r.y;
+            }
         }
         c = wave_sum(c);
         x = wave_sum(x);
@@ -124,8 +131,8 @@ __global__ void k_task_count(JoinArgs a) {
          p += (uint64_t)gridDim.x * blockDim.x) {
         uint32_t n = 0;
         if (p < a.max_parts) {
-            const uint64_t nr = a.roff[p + 1] - a.roff[p];
-            const uint64_t ns = a.soff[p + 1] - a.soff[p];
+            const uint64_t nr = a.rlen ? a.rlen[p] : a.roff[p + 1] - a.roff[p];
+            const uint64_t ns = a.slen ? a.slen[p] : a.soff[p + 1] - a.soff[p];
             if (nr && ns) n = (uint32_t)((ns + a.split - 1) / a.split);
         }
         a.task_cnt[p] = n;
@@ -139,13 +146,15 @@ __global__ void k_task_fill(JoinArgs a) {
          t += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t p = last_le<uint32_t>(a.task_base, a.max_parts + 1, (uint32_t)t);
         const uint32_t j = (uint32_t)t - a.task_base[p];
+        const uint64_t pr = a.nreal && p >= a.nreal ? p - a.nreal : p;
         const uint32_t seg =
             a.nseg == 1 ? 0u
                         : (uint32_t)last_le<uint32_t>(a.part_seg_base, (uint64_t)a.nseg + 1,
-                                                      (uint32_t)p);
-        const uint64_t r0 = a.roff[p], nr = a.roff[p + 1] - r0;
+                                                      (uint32_t)pr);
+        const uint64_t r0 = a.roff[p], nr = a.rlen ? a.rlen[p] : a.roff[p + 1] - r0;
+        const uint64_t s_end = a.slen ? a.soff[p] + a.slen[p] : a.soff[p + 1];
         const uint64_t s0 = a.soff[p] + (uint64_t)j * a.split;
-        const uint64_t ns = min<uint64_t>(a.split, a.soff[p + 1] - s0);
+        const uint64_t ns = min<uint64_t>(a.split, s_end - s0);
         const uint32_t sh = min(31u, a.seg_bits ? a.seg_bits[seg] : (uint32_t)a.hash_shift);
         a.task_info[2 * t] = make_uint4((uint32_t)r0, (uint32_t)(r0 >> 32), (uint32_t)nr, seg);
         a.task_info[2 * t + 1] = make_uint4((uint32_t)s0, (uint32_t)(s0 >> 32), (uint32_t)ns, sh);
@@ -578,9 +587,12 @@ int run_join(const JoinDesc &d, hipStream_t st, const Marks *mk, int m0) {
     a.S = d.S;
     a.roff = d.roff;
     a.soff = d.soff;
+    a.rlen = d.rlen;
+    a.slen = d.slen;
     a.part_seg_base = d.part_seg_base;
     a.nseg = d.nseg;
     a.max_parts = d.max_parts;
+    a.nreal = d.nreal;
     a.split = d.split;
     a.seg_count = d.seg_count;
     a.seg_sum = d.seg_sum;

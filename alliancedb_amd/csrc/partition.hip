@@ -26,8 +26,11 @@
 //
 // Bytes per tuple: k_hist reads 8, k_scatter reads 8 and writes 8.
 
+#include <cmath>
+
 #include "aj_internal.h"
 #include "device_common.h"
+#include "stage_tiles.h"
 #include "tiles.h"
 
 namespace aj {
@@ -55,7 +58,12 @@ struct PassArgs {
     uint64_t *poff0, *poff1;
     uint64_t max_hist;
     uint64_t max_parts;
+    // nullable: the histogram, scatter and offset kernels return at once while
+    // *guard is 0 (a pass that runs only when an earlier attempt overflowed)
+    const uint32_t *guard;
 };
+
+__device__ __forceinline__ bool guard_off(const uint32_t *guard) { return guard && !*guard; }
 
 // ---------------------------------------------------------------- k_plan
 constexpr int PLAN_BLOCK = 1024;
@@ -136,7 +144,9 @@ __global__ void k_plan_single(PassArgs a, uint64_t nR, uint64_t nS, uint64_t c0,
 
 // Zero the histogram tail beyond the entries this launch uses (so the flat
 // scan over the host-side bound stays correct).
-__global__ void k_zero_tail(uint32_t *hist, const uint64_t *header, uint64_t max_hist) {
+__global__ void k_zero_tail(uint32_t *hist, const uint64_t *header, uint64_t max_hist,
+                            const uint32_t *guard) {
+    if (guard_off(guard)) return;
     const uint64_t used = header[2] + header[3];
     for (uint64_t i = used + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < max_hist;
          i += (uint64_t)gridDim.x * blockDim.x)
@@ -170,6 +180,7 @@ __device__ __forceinline__ bool locate_chunk(const PassArgs &a, uint64_t g, Chun
 // ---------------------------------------------------------------- k_hist
 template <int STRIDE, int PBLOCK>
 __global__ __launch_bounds__(PBLOCK) void k_hist(PassArgs a) {
+    if (guard_off(a.guard)) return;
     constexpr int PTILE = PBLOCK * PITEMS;
     extern __shared__ uint32_t cnt[];
     ChunkLoc L;
@@ -197,6 +208,7 @@ __global__ __launch_bounds__(PBLOCK) void k_hist(PassArgs a) {
 // ---------------------------------------------------------------- k_scatter
 template <int STRIDE, int PBLOCK>
 __global__ __launch_bounds__(PBLOCK) void k_scatter(PassArgs a) {
+    if (guard_off(a.guard)) return;
     constexpr int PTILE = PBLOCK * PITEMS;
     extern __shared__ uint64_t smem[];
     __shared__ uint32_t scan_tmp[PBLOCK / 64];
@@ -371,6 +383,7 @@ __device__ __forceinline__ void scatter_tiles(const void *in, uint64_t begin, ui
 
 template <int STRIDE, int PBLOCK>
 __global__ __launch_bounds__(PBLOCK, 4) void k_scatter_pipe(PassArgs a) {
+    if (guard_off(a.guard)) return;
     constexpr int PTILE = PBLOCK * PITEMS;
     extern __shared__ uint64_t smem[];
     __shared__ uint32_t s_len;
@@ -638,6 +651,7 @@ __global__ __launch_bounds__(PBLOCK, 4) void k_scatter_p2(P2Args p) {
 
 // ---------------------------------------------------------------- k_part_off
 __global__ void k_part_off(PassArgs a) {
+    if (guard_off(a.guard)) return;
     const uint64_t parts = a.header[4];
     for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p <= a.max_parts;
          p += (uint64_t)gridDim.x * blockDim.x) {
@@ -665,6 +679,182 @@ __global__ void k_part_off(PassArgs a) {
     }
 }
 
+// ---------------------------------------------------------------- staged window pass
+// (run_window_stage / run_window_parts)
+struct WStageArgs {
+    uint32_t *cap;
+    uint64_t *base;
+    uint32_t *cur0, *cur1;
+    uint32_t *flag;       // the histogram pass must run (set before or during the scatter)
+    uint32_t *lflag;      // the layout does not fit (set before the scatter starts)
+    uint64_t lim0, lim1;  // layout limits: the output buffer less a trash tile
+    int W;
+    uint64_t *pstart0, *pstart1;
+    uint32_t *plen0, *plen1;
+    // S runs past their partition's capacity: spilled, then gathered into one
+    // extension range per partition
+    uint32_t *strad;
+    uint32_t *sp_cur, *sp_n;
+    uint4 *sp_list;
+    uint32_t sp_base, sp_cap, sp_max;
+    uint64_t ext_base;
+    uint32_t *ext_cur, *ext_off;
+};
+
+constexpr int WPLAN_BLOCK = 1024;
+constexpr int WSTAGE_MAXF = 1 << WSTAGE_MAX_BITS;
+
+// A partition of e expected tuples: e + max(e/8, 8 sqrt(e)) + 64, in whole
+// 64-tuple lines (as the staged PRO's regions).
+__device__ __forceinline__ uint32_t wstage_cap(uint64_t n, uint32_t F) {
+    const double e = (double)n / F;
+    const double slack = fmax(e / 8.0, 8.0 * sqrt(e));
+    const uint64_t c = (uint64_t)ceil(e + slack) + 64;
+    return (uint32_t)((c + 63) / 64 * 64);
+}
+
+__global__ __launch_bounds__(WPLAN_BLOCK) void k_wstage_caps(PassArgs a, WStageArgs w) {
+    __shared__ uint64_t tmp[WPLAN_BLOCK / 64];
+    for (int rel = 0; rel < 2; ++rel) {
+        const uint64_t *ib = rel ? a.in_begin1 : a.in_begin0;
+        const uint64_t *ie = rel ? a.in_end1 : a.in_end0;
+        uint64_t run = 0;
+        for (int b = 0; b < a.nseg; b += WPLAN_BLOCK) {
+            const int k = b + threadIdx.x;
+            uint64_t size = 0;
+            if (k < a.nseg) {
+                const uint32_t F = 1u << a.bits[k];
+                const uint32_t c = wstage_cap(ie[k] - ib[k], F);
+                w.cap[rel * w.W + k] = c;
+                size = (uint64_t)F * c;
+            }
+            uint64_t tot;
+            const uint64_t pre = block_exclusive_scan<WPLAN_BLOCK>(size, tmp, &tot);
+            if (k < a.nseg) w.base[rel * w.W + k] = run + pre;
+            run += tot;
+        }
+        if (threadIdx.x == 0 && run > (rel ? w.lim1 : w.lim0)) {
+            atomicOr(w.flag, 1u);
+            atomicOr(w.lflag, 1u);
+        }
+        __syncthreads();
+    }
+}
+
+// One work-group per chunk of a window: the chunk's tiles counting-sorted by
+// digit in LDS, each run placed by one atomic reservation in its partition.
+template <int PBLOCK>
+__global__ __launch_bounds__(PBLOCK, 4) void k_wscatter(PassArgs a, WStageArgs w) {
+    constexpr int PTILE = PBLOCK * PITEMS;
+    extern __shared__ uint64_t smem[];
+    __shared__ uint32_t s_len;
+    // the layout does not fit: the histogram pass runs instead.  (Only the flag
+    // written before this launch is read here: the overflow flag that work-groups
+    // of this launch set would end some waves of a work-group and not others.)
+    if (*w.lflag) return;
+    ChunkLoc L;
+    if (!locate_chunk(a, blockIdx.x, L)) return;
+    const uint32_t F = 1u << a.bits[L.seg];
+    uint64_t *buf = smem;                                                // PTILE + NSPARE
+    uint32_t *cnt = reinterpret_cast<uint32_t *>(buf + PTILE + NSPARE);  // F + NSPARE
+    uint32_t *dst = cnt + F + NSPARE;                                    // F
+    for (uint32_t d = threadIdx.x; d < F + NSPARE; d += PBLOCK) cnt[d] = 0;
+    __syncthreads();
+    struct {
+        StageSeg s;
+        __device__ bool get(uint32_t i, StageSeg &o) const {
+            o = s;
+            return i == 0;
+        }
+    } src;
+    const uint32_t c = w.cap[L.rel * w.W + L.seg];
+    src.s.in = L.rel ? a.in1 : a.in0;
+    src.s.begin = L.begin;
+    src.s.end = L.end;
+    src.s.r.cur = (L.rel ? w.cur1 : w.cur0) + a.pbase[L.seg];
+    src.s.r.base = (uint32_t)w.base[L.rel * w.W + L.seg];
+    src.s.r.stride = c;
+    src.s.r.cap = c;
+    src.s.r.trash = (uint32_t)(L.rel ? w.lim1 : w.lim0);
+    src.s.r.flag = w.flag;
+    StageSpill sp{};  // S runs that do not fit are spilled; R runs set the flag
+    if (L.rel) {
+        sp.strad = w.strad;
+        sp.cur = w.sp_cur;
+        sp.n = w.sp_n;
+        sp.list = w.sp_list;
+        sp.base = w.sp_base;
+        sp.cap = w.sp_cap;
+        sp.max = w.sp_max;
+        sp.part0 = a.pbase[L.seg];
+    }
+    stage_tiles<PBLOCK, true, false, false, decltype(src), 12, WSTAGE_MAXF, true>(
+        src, L.rel ? a.out1 : a.out0, a.shift, F, buf, cnt, dst, &s_len, &sp);
+}
+
+// Partition p's start and length per relation, from the staged layout, or from
+// the exact offsets of the histogram pass after an overflow.  Entry p is the
+// partition's main range; entry max_parts + p its S extension (with the same R
+// range), non-empty only when S runs of the partition were spilled: those are
+// given one range of the extension area here and gathered into it by
+// k_wspill_gather.
+__global__ void k_wstage_parts(PassArgs a, WStageArgs w) {
+    const bool over = *w.flag != 0;
+    const uint64_t parts = a.header[4];
+    const uint64_t M = a.max_parts;
+    for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < M;
+         p += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t rs = 0, ss = 0, es = 0;
+        uint32_t rl = 0, sl = 0, el = 0;
+        if (p < parts) {
+            if (over) {
+                rs = a.poff0[p];
+                rl = (uint32_t)(a.poff0[p + 1] - a.poff0[p]);
+                ss = a.poff1[p];
+                sl = (uint32_t)(a.poff1[p + 1] - a.poff1[p]);
+            } else {
+                const int k = (int)last_le<uint32_t>(a.pbase, (uint64_t)a.nseg + 1, (uint32_t)p);
+                const uint64_t d = p - a.pbase[k];
+                const uint32_t c0 = w.cap[k], c1 = w.cap[w.W + k];
+                const uint32_t n0 = w.cur0[p], n1 = w.cur1[p];
+                rs = w.base[k] + d * c0;
+                rl = n0 < c0 ? n0 : c0;  // R never spills: n0 > c0 set the flag
+                ss = w.base[w.W + k] + d * c1;
+                sl = n1;
+                if (n1 > c1) {
+                    sl = w.strad[p];  // the runs from here on were spilled
+                    el = n1 - sl;
+                    const uint32_t off = atomicAdd(w.ext_cur, el);
+                    w.ext_off[p] = off;
+                    es = w.ext_base + off;
+                }
+            }
+        }
+        w.pstart0[p] = rs;
+        w.plen0[p] = rl;
+        w.pstart1[p] = ss;
+        w.plen1[p] = sl;
+        w.pstart0[M + p] = rs;
+        w.plen0[M + p] = el ? rl : 0u;
+        w.pstart1[M + p] = es;
+        w.plen1[M + p] = el;
+    }
+}
+
+// The spilled S runs, each moved to its place in its partition's extension
+// range (the runs of a partition past its capacity cover [strad, cur) of the
+// partition's positions, so a run's place follows from its start alone).
+__global__ __launch_bounds__(256) void k_wspill_gather(PassArgs a, WStageArgs w) {
+    if (*w.flag) return;
+    const uint32_t n = min(*w.sp_n, w.sp_max);
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+        const uint4 e = w.sp_list[i];
+        const uint64_t *src = a.out1 + w.sp_base + e.y;
+        uint64_t *dst = a.out1 + w.ext_base + w.ext_off[e.x] + (e.z - w.strad[e.x]);
+        for (uint32_t j = threadIdx.x; j < e.w; j += blockDim.x) dst[j] = src[j];
+    }
+}
+
 // ---------------------------------------------------------------- device scan
 // Prefix sums of n counters (u32: the histograms of a pass, the join's task
 // counts; u64: the window generator's arrival times), modulo the type's width,
@@ -676,7 +866,9 @@ constexpr int SCAN_BLOCK = 256, SCAN_ITEMS = 16, SCAN_TILE = SCAN_BLOCK * SCAN_I
 constexpr int SUMS_BLOCK = 1024;
 
 template <class T>
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tiles(const T *in, uint64_t n, T *sums) {
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tiles(const T *in, uint64_t n, T *sums,
+                                                          const uint32_t *guard) {
+    if (guard_off(guard)) return;
     __shared__ T red[SCAN_BLOCK / 64];
     const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
     T s = 0;
@@ -690,7 +882,9 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tiles(const T *in, uint64_t
 }
 
 template <class T>
-__global__ __launch_bounds__(SUMS_BLOCK) void k_scan_sums(T *sums, uint32_t tiles) {
+__global__ __launch_bounds__(SUMS_BLOCK) void k_scan_sums(T *sums, uint32_t tiles,
+                                                         const uint32_t *guard) {
+    if (guard_off(guard)) return;
     __shared__ T tmp[SUMS_BLOCK / 64];
     T carry = 0;
     for (uint32_t b = 0; b < tiles; b += SUMS_BLOCK) {
@@ -707,7 +901,8 @@ __device__ __forceinline__ uint32_t scan_slot(uint32_t k) { return k + (k >> 5);
 
 template <class T, bool INCLUSIVE>
 __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(const T *in, T *out, uint64_t n,
-                                                         const T *sums) {
+                                                         const T *sums, const uint32_t *guard) {
+    if (guard_off(guard)) return;
     __shared__ T sh[SCAN_TILE + SCAN_TILE / 32];  // padded: lane t's run spans banks
     __shared__ T tmp[SCAN_BLOCK / 64];
     const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
@@ -740,7 +935,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(const T *in, T *out, 
 
 template <class T, bool INCLUSIVE>
 static int device_scan(const T *in, T *out, uint64_t n, void *tmp, size_t tmp_bytes,
-                       hipStream_t st) {
+                       hipStream_t st, const uint32_t *guard = nullptr) {
     if (!n) return AJ_OK;
     const uint64_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
     if (tmp_bytes < sizeof(T) * (tiles + 1) || !tmp || tiles >= (1ull << 31) ||
@@ -751,10 +946,11 @@ static int device_scan(const T *in, T *out, uint64_t n, void *tmp, size_t tmp_by
     }
     T *sums = static_cast<T *>(tmp);
     hipLaunchKernelGGL(k_scan_tiles<T>, dim3((unsigned)tiles), dim3(SCAN_BLOCK), 0, st, in, n,
-                       sums);
-    hipLaunchKernelGGL(k_scan_sums<T>, dim3(1), dim3(SUMS_BLOCK), 0, st, sums, (uint32_t)tiles);
+                       sums, guard);
+    hipLaunchKernelGGL(k_scan_sums<T>, dim3(1), dim3(SUMS_BLOCK), 0, st, sums, (uint32_t)tiles,
+                       guard);
     hipLaunchKernelGGL((k_scan_apply<T, INCLUSIVE>), dim3((unsigned)tiles), dim3(SCAN_BLOCK), 0,
-                       st, in, out, n, (const T *)sums);
+                       st, in, out, n, (const T *)sums, guard);
     AJ_HIP(hipGetLastError());
     return AJ_OK;
 }
@@ -764,8 +960,8 @@ size_t scan_tmp_bytes(uint64_t n, size_t elem) {
 }
 
 int exclusive_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, void *tmp,
-                       size_t tmp_bytes, hipStream_t st) {
-    return device_scan<uint32_t, false>(in, out, n, tmp, tmp_bytes, st);
+                       size_t tmp_bytes, hipStream_t st, const uint32_t *guard) {
+    return device_scan<uint32_t, false>(in, out, n, tmp, tmp_bytes, st, guard);
 }
 
 int inclusive_scan_u64(const uint64_t *in, uint64_t *out, uint64_t n, void *tmp,
@@ -830,7 +1026,7 @@ static int launch_pass_kernels(const PassDesc &d, const PassWork &w, const PassA
     if (mk) mk->mark(m0 + 1);
     if (d.max_hist)
         AJ_TRY(exclusive_scan_u32(a.hist, a.hist_scan, d.max_hist, w.scan_tmp, w.scan_tmp_bytes,
-                                  st));
+                                  st, a.guard));
     if (mk) mk->mark(m0 + 2);
     if (d.max_chunks) {
         if (PBLOCK == 512 && (d.hist2d || STRIDE == 12) && d.max_bits <= PIPE_MAX_BITS) {
@@ -852,6 +1048,40 @@ static int launch_pass_kernels(const PassDesc &d, const PassWork &w, const PassA
     }
     if (mk) mk->mark(m0 + 3);
     return AJ_OK;
+}
+
+static PassArgs pass_args(const PassDesc &d, const PassWork &w) {
+    PassArgs a;
+    a.in0 = d.in[0];
+    a.in1 = d.in[1];
+    a.stride = d.in_stride;
+    a.out0 = d.out[0];
+    a.out1 = d.out[1];
+    a.in_begin0 = d.in_begin[0];
+    a.in_begin1 = d.in_begin[1];
+    a.in_end0 = d.in_end[0];
+    a.in_end1 = d.in_end[1];
+    a.bits = d.bits;
+    a.nseg = d.nseg;
+    a.shift = d.shift;
+    a.chunk = d.chunk;
+    a.n_total0 = d.n_total[0];
+    a.hist = w.hist;
+    a.hist_scan = w.hist + d.max_hist;
+    a.cpre0 = w.seg_chunk_pre[0];
+    a.cpre1 = w.seg_chunk_pre[1];
+    a.hbase0 = w.seg_hist_base[0];
+    a.hbase1 = w.seg_hist_base[1];
+    a.obase0 = w.seg_out_base[0];
+    a.obase1 = w.seg_out_base[1];
+    a.pbase = w.seg_part_base;
+    a.header = w.header;
+    a.poff0 = w.part_off[0];
+    a.poff1 = w.part_off[1];
+    a.max_hist = d.max_hist;
+    a.max_parts = d.max_parts;
+    a.guard = d.guard;
+    return a;
 }
 
 int run_partition_pass(const PassDesc &d, const PassWork &w, hipStream_t st, const Marks *mk,
@@ -888,36 +1118,7 @@ int run_partition_pass(const PassDesc &d, const PassWork &w, hipStream_t st, con
         set_error(AJ_ERR_ARG, "histogram too large (%llu entries)", (unsigned long long)d.max_hist);
         return AJ_ERR_ARG;
     }
-    PassArgs a;
-    a.in0 = d.in[0];
-    a.in1 = d.in[1];
-    a.stride = d.in_stride;
-    a.out0 = d.out[0];
-    a.out1 = d.out[1];
-    a.in_begin0 = d.in_begin[0];
-    a.in_begin1 = d.in_begin[1];
-    a.in_end0 = d.in_end[0];
-    a.in_end1 = d.in_end[1];
-    a.bits = d.bits;
-    a.nseg = d.nseg;
-    a.shift = d.shift;
-    a.chunk = d.chunk;
-    a.n_total0 = d.n_total[0];
-    a.hist = w.hist;
-    a.hist_scan = w.hist + d.max_hist;
-    a.cpre0 = w.seg_chunk_pre[0];
-    a.cpre1 = w.seg_chunk_pre[1];
-    a.hbase0 = w.seg_hist_base[0];
-    a.hbase1 = w.seg_hist_base[1];
-    a.obase0 = w.seg_out_base[0];
-    a.obase1 = w.seg_out_base[1];
-    a.pbase = w.seg_part_base;
-    a.header = w.header;
-    a.poff0 = w.part_off[0];
-    a.poff1 = w.part_off[1];
-    a.max_hist = d.max_hist;
-    a.max_parts = d.max_parts;
-
+    const PassArgs a = pass_args(d, w);
     if (d.exact_single && d.nseg == 1) {
         const uint32_t F = 1u << d.max_bits;
         const uint64_t c0 = (d.n_total[0] + d.chunk - 1) / d.chunk;
@@ -926,11 +1127,11 @@ int run_partition_pass(const PassDesc &d, const PassWork &w, hipStream_t st, con
                            d.n_total[1], c0, c1, F);
         if ((c0 + c1) * F < d.max_hist)  // the scan covers max_hist entries
             hipLaunchKernelGGL(k_zero_tail, dim3(256), dim3(256), 0, st, a.hist, a.header,
-                               d.max_hist);
+                               d.max_hist, a.guard);
     } else {
-        hipLaunchKernelGGL(k_plan, dim3(1), dim3(PLAN_BLOCK), 0, st, a);
+        if (!d.skip_plan) hipLaunchKernelGGL(k_plan, dim3(1), dim3(PLAN_BLOCK), 0, st, a);
         hipLaunchKernelGGL(k_zero_tail, dim3(256), dim3(256), 0, st, a.hist, a.header,
-                           d.max_hist);
+                           d.max_hist, a.guard);
     }
     const bool s8 = d.in_stride == 8;
     int rc;
@@ -943,6 +1144,84 @@ int run_partition_pass(const PassDesc &d, const PassWork &w, hipStream_t st, con
     AJ_TRY(rc);
     const unsigned pgrid = (unsigned)std::min<uint64_t>((d.max_parts + 256) / 256, 4096);
     hipLaunchKernelGGL(k_part_off, dim3(pgrid), dim3(256), 0, st, a);
+    AJ_HIP(hipGetLastError());
+    return AJ_OK;
+}
+
+int run_partition_plan(const PassDesc &d, const PassWork &w, hipStream_t st) {
+    const PassArgs a = pass_args(d, w);
+    hipLaunchKernelGGL(k_plan, dim3(1), dim3(PLAN_BLOCK), 0, st, a);
+    AJ_HIP(hipGetLastError());
+    return AJ_OK;
+}
+
+
+
+// ---------------------------------------------------------------- staged window pass, host
+uint64_t window_stage_bound(uint64_t n, uint64_t parts) {
+    // sum over windows of F * cap <= 1.125 n + 8 sqrt(F n) + 128 F, and the
+    // sum of sqrt(F_w n_w) is at most sqrt(parts * n)
+    const double nn = (double)n, pp = (double)std::max<uint64_t>(parts, 1);
+    return (uint64_t)std::ceil(1.125 * nn + 8.0 * std::sqrt(pp * nn) + 128.0 * pp) +
+           2ull * 1024 * PITEMS;
+}
+
+static WStageArgs wstage_args(const WStageDesc &ws) {
+    WStageArgs w;
+    w.cap = ws.cap;
+    w.base = ws.base;
+    w.cur0 = ws.cur[0];
+    w.cur1 = ws.cur[1];
+    w.flag = ws.flag;
+    w.lflag = ws.lflag;
+    // the last tile of each buffer is the trash area
+    w.lim0 = ws.out_cap[0] - 1024 * PITEMS;
+    w.lim1 = ws.out_cap[1] - 1024 * PITEMS;
+    w.W = ws.W;
+    w.pstart0 = ws.pstart[0];
+    w.pstart1 = ws.pstart[1];
+    w.plen0 = ws.plen[0];
+    w.plen1 = ws.plen[1];
+    w.strad = ws.strad;
+    w.sp_cur = ws.spill_cur;
+    w.sp_n = ws.spill_n;
+    w.sp_list = ws.spill_list;
+    w.sp_base = (uint32_t)ws.out_cap[1];  // the spill area follows the S layout
+    w.sp_cap = (uint32_t)ws.spill_cap;
+    w.sp_max = ws.spill_max;
+    w.ext_base = ws.out_cap[1] + ws.spill_cap;  // then the extension ranges
+    w.ext_cur = ws.ext_cur;
+    w.ext_off = ws.ext_off;
+    return w;
+}
+
+int run_window_stage(const PassDesc &d, const PassWork &wk, const WStageDesc &ws,
+                     hipStream_t st) {
+    if (d.in_stride != 12 || d.max_bits > WSTAGE_MAX_BITS || d.nseg > ws.W ||
+        ws.out_cap[0] >= (1ull << 32) || ws.out_cap[1] + ws.spill_cap >= (1ull << 32) ||
+        ws.out_cap[0] < 2 * 1024 * PITEMS || ws.out_cap[1] < 2 * 1024 * PITEMS) {
+        set_error(AJ_ERR_ARG, "staged window pass: unsupported geometry");
+        return AJ_ERR_ARG;
+    }
+    const PassArgs a = pass_args(d, wk);
+    const WStageArgs w = wstage_args(ws);
+    hipLaunchKernelGGL(k_wstage_caps, dim3(1), dim3(WPLAN_BLOCK), 0, st, a, w);
+    constexpr int B = 1024;
+    const size_t lds = stage_lds_bytes(B, 1u << d.max_bits);
+    AJ_TRY(lds_limit_set((const void *)k_wscatter<B>, lds));
+    if (d.max_chunks)
+        hipLaunchKernelGGL(k_wscatter<B>, dim3((unsigned)d.max_chunks), dim3(B), lds, st, a, w);
+    AJ_HIP(hipGetLastError());
+    return AJ_OK;
+}
+
+int run_window_parts(const PassDesc &d, const PassWork &wk, const WStageDesc &ws,
+                     hipStream_t st) {
+    const PassArgs a = pass_args(d, wk);
+    const WStageArgs w = wstage_args(ws);
+    const unsigned g = (unsigned)std::min<uint64_t>((d.max_parts + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_wstage_parts, dim3(g), dim3(256), 0, st, a, w);
+    hipLaunchKernelGGL(k_wspill_gather, dim3(1024), dim3(256), 0, st, a, w);
     AJ_HIP(hipGetLastError());
     return AJ_OK;
 }

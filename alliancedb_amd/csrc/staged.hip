@@ -38,176 +38,11 @@
 
 #include "aj_internal.h"
 #include "device_common.h"
+#include "stage_tiles.h"
 #include "tiles.h"
 
 namespace aj {
 
-constexpr int STAGE_MAXF = 256;  // digits per pass (8 bits)
-
-// Keys replaced by their murmur3 finalisation (a bijection, undone by
-// fmix32_inv): the hashed partitioning of NPO's region build.
-template <int N> __device__ __forceinline__ void hash_keys(uint64_t (&t)[N]) {
-#pragma unroll
-    for (int j = 0; j < N; ++j) t[j] = ((uint64_t)fmix32(key_of(t[j])) << 32) | (uint32_t)t[j];
-}
-
-// Where the runs of a tile go: region d starts at out index base + d * stride,
-// holds cap tuples and has filled *cur[d] of them.
-struct StageRes {
-    uint32_t *cur;
-    uint32_t base, stride, cap;
-    uint32_t trash;  // out index of a tile-sized area for runs that do not fit
-    uint32_t *flag;
-};
-
-// One contiguous input range of the tile loop and where its runs go.
-struct StageSeg {
-    const uint64_t *in;
-    uint64_t begin, end;
-    StageRes r;
-};
-
-// The pipelined tile loop of the partition scatter (four barriers per tile,
-// branch-free phases, the next tile's loads in flight during the write), with
-// the write bases taken from atomic reservations instead of running cursors.
-// Wave 0 scans the tile's F counters and issues one atomic add per non-empty
-// digit; the returned offsets are needed only for the write phase, so they are
-// turned into write bases after wave 0 has placed its own tuples.
-// The input is a sequence of segments src.get(0), src.get(1), ... (each with
-// its own destination regions); the loop runs through all of them as one
-// pipeline, so the first tile of a segment is loaded while the last tile of
-// the one before is written.  Every value src.get returns is wave-uniform.
-template <int PBLOCK, bool NT_LOAD, bool NT_STORE, bool HASH, class Src>
-__device__ __forceinline__ void stage_tiles(const Src &src, uint64_t *out, int shift, uint32_t F,
-                                            uint64_t *buf, uint32_t *cnt, uint32_t *dst,
-                                            uint32_t *s_len) {
-    constexpr int PTILE = PBLOCK * PITEMS;
-    constexpr int MAXPER = STAGE_MAXF / 64;
-    constexpr int WB = 8;
-    static_assert(PITEMS % WB == 0, "a tile's items split into whole batches");
-    const uint32_t mask = F - 1;
-    StageSeg sg;
-    uint32_t si = 0;
-    uint64_t base;
-    for (;; ++si) {  // the first segment with a tuple
-        if (!src.get(si, sg)) return;
-        base = tile_start(sg.begin, 8);
-        if (base < sg.end) break;
-    }
-    uint64_t t[PITEMS];
-    bool v[PITEMS];
-    load_tile<PBLOCK, PITEMS, NT_LOAD>(sg.in, 8, base, sg.begin, sg.end, t, v);
-    if constexpr (HASH) hash_keys<PITEMS>(t);
-    for (;;) {
-        const StageRes &r = sg.r;
-        uint32_t rank2[PITEMS / 2];
-#pragma unroll
-        for (int j = 0; j < PITEMS; ++j) {
-            const uint32_t d = (key_of(t[j]) >> shift) & mask;
-            const uint32_t rk =
-                atomicAdd(&cnt[v[j] ? d : F + (threadIdx.x & (NSPARE - 1))], 1u) & 0xffffu;
-            if (j & 1)
-                rank2[j / 2] |= rk << 16;
-            else
-                rank2[j / 2] = rk;
-        }
-        __syncthreads();
-        uint32_t o[MAXPER], pk[MAXPER], ck[MAXPER];
-        const uint32_t lane = threadIdx.x;
-        const uint32_t per = (F + 63) >> 6;
-        const uint32_t b = lane * per;
-        if (threadIdx.x < 64) {
-            uint32_t sum = 0;
-#pragma unroll
-            for (int k = 0; k < MAXPER; ++k) {
-                ck[k] = ((uint32_t)k < per && b + k < F) ? cnt[b + k] : 0u;
-                sum += ck[k];
-            }
-            const uint32_t inc = wave_inclusive_scan(sum);
-            uint32_t pre = inc - sum;
-#pragma unroll
-            for (int k = 0; k < MAXPER; ++k) {
-                pk[k] = pre;
-                if ((uint32_t)k < per && b + k < F) cnt[b + k] = pre;
-                o[k] = ck[k] ? atomicAdd(&r.cur[b + k], ck[k]) : 0u;
-                pre += ck[k];
-            }
-            if (lane == 63) *s_len = inc;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int j0 = 0; j0 < PITEMS; j0 += WB) {
-            uint32_t st[WB];
-#pragma unroll
-            for (int j = 0; j < WB; ++j) st[j] = cnt[(key_of(t[j0 + j]) >> shift) & mask];
-#pragma unroll
-            for (int j = 0; j < WB; ++j)
-                buf[v[j0 + j] ? st[j] + ((rank2[(j0 + j) / 2] >> (16 * ((j0 + j) & 1))) & 0xffffu)
-                              : (uint32_t)PTILE + (threadIdx.x & (NSPARE - 1))] = t[j0 + j];
-        }
-        if (threadIdx.x < 64) {
-#pragma unroll
-            for (int k = 0; k < MAXPER; ++k) {
-                if (ck[k]) {
-                    const bool fits = o[k] <= r.cap && ck[k] <= r.cap - o[k];
-                    if (!fits) atomicOr(r.flag, 1u);
-                    dst[b + k] = fits ? r.base + (b + k) * r.stride + o[k] - pk[k] : r.trash - pk[k];
-                }
-            }
-        }
-        __syncthreads();
-        const uint32_t len = *s_len;
-        // the next tile: in this segment, else the first of a later one
-        uint64_t nbase = base + PTILE;
-        bool more = nbase < sg.end;
-        StageSeg ns = sg;
-        while (!more && src.get(++si, ns)) {
-            nbase = tile_start(ns.begin, 8);
-            more = nbase < ns.end;
-        }
-        if (more) {
-            load_tile<PBLOCK, PITEMS, NT_LOAD>(ns.in, 8, nbase, ns.begin, ns.end, t, v);
-            if constexpr (HASH) hash_keys<PITEMS>(t);
-        }
-        // the first lane of this wave: batches past the tile's end are skipped
-        // (a wave-uniform branch; only a partly filled tile has them)
-        const uint32_t wave0 = __builtin_amdgcn_readfirstlane(threadIdx.x);
-#pragma unroll
-        for (int k0 = 0; k0 < PITEMS; k0 += WB) {
-            if (wave0 + (uint32_t)k0 * PBLOCK >= len) break;
-            uint64_t u[WB];
-            uint32_t w[WB];
-#pragma unroll
-            for (int k = 0; k < WB; ++k) {
-                const uint32_t i = threadIdx.x + (uint32_t)(k0 + k) * PBLOCK;
-                u[k] = buf[i < len ? i : (uint32_t)PTILE];
-            }
-#pragma unroll
-            for (int k = 0; k < WB; ++k) w[k] = dst[(key_of(u[k]) >> shift) & mask];
-#pragma unroll
-            for (int k = 0; k < WB; ++k) {
-                const uint32_t i = threadIdx.x + (uint32_t)(k0 + k) * PBLOCK;
-                if (i < len) {
-                    uint64_t *q = out + (uint64_t)(uint32_t)(w[k] + i);
-                    if constexpr (NT_STORE)
-                        __builtin_nontemporal_store(u[k], q);
-                    else
-                        *q = u[k];
-                }
-            }
-        }
-        for (uint32_t d = threadIdx.x; d < F; d += PBLOCK) cnt[d] = 0;
-        __syncthreads();
-        if (!more) break;
-        base = nbase;
-        sg = ns;
-    }
-}
-
-constexpr size_t stage_lds_bytes(int pblock, uint32_t F) {
-    return sizeof(uint64_t) * ((size_t)pblock * PITEMS + NSPARE) +
-           sizeof(uint32_t) * (2 * (size_t)F + NSPARE);
-}
 
 struct Stage1Args {
     const uint64_t *in;

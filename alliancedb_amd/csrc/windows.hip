@@ -94,6 +94,11 @@ int batch_plan_init(BatchPlan *p) {
     // partitions per window: 2^bits with (n >> (bits-1)) > target, so at most
     // 2n/target + 1 per window
     p->max_parts = std::min<uint64_t>(W << b, 2 * p->max_R / p->target + W);
+    // the staged pass's S spill area: half of max_S, runs of 64 tuples on average
+    p->spill_cap = std::max<uint64_t>(p->max_S / 2, 2 * WTILE);
+    p->spill_max = (uint32_t)std::min<uint64_t>(p->spill_cap / 64 + 4096, 1u << 30);
+    // join entries: each partition, and (staged) each partition's S extension
+    const uint64_t jparts = 2 * p->max_parts;
     BatchLayout &L = p->L;
     size_t off = 0;
     // window ranges: rb | re | sb | se, W entries each, in one block
@@ -110,19 +115,45 @@ int batch_plan_init(BatchPlan *p) {
     }
     L.pb = lay(off, 4 * (W + 1));
     L.hdr = lay(off, 64);
-    L.tcnt = lay(off, 4 * (p->max_parts + 1));
-    L.tbase = lay(off, 4 * (p->max_parts + 1));
-    L.tinfo = lay(off, 2 * sizeof(uint4) * join_max_tasks(p->max_parts, p->max_S, p->split));
-    L.tres = lay(off, sizeof(ulonglong2) * join_max_tasks(p->max_parts, p->max_S, p->split));
+    L.tcnt = lay(off, 4 * (jparts + 1));
+    L.tbase = lay(off, 4 * (jparts + 1));
+    L.tinfo = lay(off, 2 * sizeof(uint4) * join_max_tasks(jparts, p->max_S, p->split));
+    L.tres = lay(off, sizeof(ulonglong2) * join_max_tasks(jparts, p->max_S, p->split));
     L.scal = lay(off, 64);
+    L.wcap = lay(off, 2 * 4 * W);
+    L.wbase = lay(off, 2 * 8 * W);
+    for (int r = 0; r < 2; ++r) {
+        L.wcur[r] = lay(off, 4 * (p->max_parts + 1));
+        L.pstart[r] = lay(off, 8 * (jparts + 2));
+        L.plen[r] = lay(off, 4 * (jparts + 2));
+    }
+    L.strad = lay(off, 4 * (p->max_parts + 1));
+    L.extoff = lay(off, 4 * (p->max_parts + 1));
+    L.splist = lay(off, sizeof(uint4) * (uint64_t)p->spill_max);
     L.bytes = off;
     p->scan_bytes = std::max(pass_scan_tmp_bytes(std::max<uint64_t>(p->max_hist, 1)),
-                             join_scan_tmp_bytes(p->max_parts));
+                             join_scan_tmp_bytes(jparts));
     AJ_TRY(p->work.ensure(L.bytes));
     AJ_TRY(p->hist.ensure(sizeof(uint32_t) * 2 * p->max_hist + 256));
     AJ_TRY(p->scan_tmp.ensure(p->scan_bytes));
-    AJ_TRY(p->outR.ensure(8 * p->max_R + 256));
-    AJ_TRY(p->outS.ensure(8 * p->max_S + 256));
+    // the staged pass needs room for its partitions' slack; windows of more
+    // radix bits than its LDS holds, or layouts past 32-bit offsets, use the
+    // histogram pass
+    p->out_cap[0] = p->max_R;
+    p->out_cap[1] = p->max_S;
+    if (p->staged) {
+        const uint64_t cR = window_stage_bound(p->max_R, p->max_parts);
+        const uint64_t cS = window_stage_bound(p->max_S, p->max_parts);
+        if (p->max_bits <= WSTAGE_MAX_BITS && cR < (1ull << 32) &&
+            cS + p->spill_cap < (1ull << 32)) {
+            p->out_cap[0] = cR;
+            p->out_cap[1] = cS;
+        } else {
+            p->staged = false;
+        }
+    }
+    AJ_TRY(p->outR.ensure(8 * p->out_cap[0] + 256));
+    AJ_TRY(p->outS.ensure(8 * (p->out_cap[1] + (p->staged ? 2 * p->spill_cap : 0)) + 256));
     return AJ_OK;
 }
 
@@ -153,6 +184,12 @@ int enqueue_ranges(BatchPlan *p, const aj_stream_tuple_t *dR, uint64_t nR,
     AJ_TRY(fill_u64(cnt, nwin, 0, st));
     AJ_TRY(fill_u64(sum, nwin, 0, st));
     AJ_TRY(fill_u64(W_<uint64_t>(p, L.scal), 8, 0, st));
+    // scalars: [2] pair cursor, [3] task counter, [5] fallback / layout flags,
+    // [6] spill area cursor / spilled runs, [7] extension cursor
+    uint32_t *flag = reinterpret_cast<uint32_t *>(W_<uint64_t>(p, L.scal) + 5);
+    uint32_t *spill = reinterpret_cast<uint32_t *>(W_<uint64_t>(p, L.scal) + 6);
+    if (p->staged)
+        for (int r = 0; r < 2; ++r) AJ_TRY(fill_u32(W_<uint32_t>(p, L.wcur[r]), p->max_parts, 0, st));
     const unsigned g = (unsigned)((nwin + 255) / 256);
     hipLaunchKernelGGL(k_window_bits, dim3(g), dim3(256), 0, st, W_<uint64_t>(p, L.rb),
                        W_<uint64_t>(p, L.re), nwin, W_<uint32_t>(p, L.bits), p->target,
@@ -189,14 +226,51 @@ int enqueue_ranges(BatchPlan *p, const aj_stream_tuple_t *dR, uint64_t nR,
     w.header = W_<uint64_t>(p, L.hdr);
     w.scan_tmp = p->scan_tmp.p;
     w.scan_tmp_bytes = p->scan_bytes;
-    AJ_TRY(run_partition_pass(d, w, st));
     JoinDesc j;
+    if (p->staged) {
+        // the staged pass first; the histogram pass runs (and rewrites the
+        // partitions exactly) only when the staged one set the overflow flag
+        WStageDesc ws;
+        ws.cap = W_<uint32_t>(p, L.wcap);
+        ws.base = W_<uint64_t>(p, L.wbase);
+        ws.flag = flag;
+        ws.lflag = flag + 1;
+        ws.W = p->max_win;
+        ws.strad = W_<uint32_t>(p, L.strad);
+        ws.spill_cur = spill;
+        ws.spill_n = spill + 1;
+        ws.ext_cur = spill + 2;
+        ws.spill_list = W_<uint4>(p, L.splist);
+        ws.spill_cap = p->spill_cap;
+        ws.spill_max = p->spill_max;
+        ws.ext_off = W_<uint32_t>(p, L.extoff);
+        for (int r = 0; r < 2; ++r) {
+            ws.cur[r] = W_<uint32_t>(p, L.wcur[r]);
+            ws.out_cap[r] = p->out_cap[r];
+            ws.pstart[r] = W_<uint64_t>(p, L.pstart[r]);
+            ws.plen[r] = W_<uint32_t>(p, L.plen[r]);
+        }
+        AJ_TRY(run_partition_plan(d, w, st));
+        AJ_TRY(run_window_stage(d, w, ws, st));
+        d.guard = flag;
+        d.skip_plan = true;
+        AJ_TRY(run_partition_pass(d, w, st));
+        AJ_TRY(run_window_parts(d, w, ws, st));
+        j.roff = ws.pstart[0];
+        j.soff = ws.pstart[1];
+        j.rlen = ws.plen[0];
+        j.slen = ws.plen[1];
+        j.max_parts = 2 * p->max_parts;
+        j.nreal = p->max_parts;
+    } else {
+        AJ_TRY(run_partition_pass(d, w, st));
+        j.roff = w.part_off[0];
+        j.soff = w.part_off[1];
+        j.max_parts = p->max_parts;
+    }
     j.R = p->outR.as<uint64_t>();
     j.S = p->outS.as<uint64_t>();
-    j.roff = w.part_off[0];
-    j.soff = w.part_off[1];
     j.nparts_dev = nullptr;
-    j.max_parts = p->max_parts;
     j.part_seg_base = w.seg_part_base;
     j.nseg = nwin;
     j.hash_shift = 0;
@@ -213,7 +287,7 @@ int enqueue_ranges(BatchPlan *p, const aj_stream_tuple_t *dR, uint64_t nR,
     j.task_next = reinterpret_cast<uint32_t *>(W_<uint64_t>(p, L.scal) + 3);
     j.task_info = W_<uint4>(p, L.tinfo);
     j.task_res = W_<ulonglong2>(p, L.tres);
-    j.max_tasks = join_max_tasks(p->max_parts, p->max_S, p->split);
+    j.max_tasks = join_max_tasks(j.max_parts, p->max_S, p->split);
     j.scan_tmp = p->scan_tmp.p;
     j.scan_tmp_bytes = p->scan_bytes;
     j.num_cu = p->ctx->num_cu;
@@ -253,6 +327,7 @@ int batch_plan_setup(BatchPlan *p, Ctx *c, uint64_t max_R, uint64_t max_S, int m
     }
     p->chunk = (chunk + WTILE - 1) / WTILE * WTILE;
     p->use_graph = !(opts && opts->no_graph);
+    p->staged = staged_enabled();
     if (p->cap != 4096 && p->cap != 8192 && p->cap != 8704 && p->cap != 12288) {
         set_error(AJ_ERR_ARG, "cap must be 4096, 8192, 8704 or 12288");
         return AJ_ERR_ARG;

@@ -150,3 +150,73 @@ def test_batch_autotuner_picks_a_geometry_with_identical_results(aj, oracle):
         assert got == [(int(c), int(x)) for c, x in want]
         for d in (cnt, chk, out["R"][0], out["R"][2], out["S"][0], out["S"][2]):
             d.free()
+
+
+@pytest.mark.parametrize("staged,case", [(1, "RS"), (1, "S"), (1, "zipf"), (1, "S_all"),
+                                         (0, "RS"), (0, "zipf")])
+def test_batch_windows_overflow_falls_back_exactly(aj, oracle, staged, case):
+    """Skewed windows overflow the staged pass's fixed partition capacities.
+    RS: windows 2 and 6 have all keys equal on both sides; an R partition that
+    overflows sends the batch to the histogram pass on the device.  S: only their
+    S keys are equal; the S runs past the capacity are spilled and gathered
+    into the partition's extension range.  zipf: every window's S keys are Zipf
+    distributed (many partitions spill).  S_all: every window's S keys are equal,
+    more than the spill area holds, so the histogram pass runs again.  Every
+    window stays exact, with the staged pass on (option "staged" 1) or off, and
+    so do an empty side and the ordinary windows of the same batch."""
+    rng = np.random.default_rng(77)
+    nwin = 10
+    hot = (2, 6)
+    sizes = {"R": [], "S": []}
+    for w in range(nwin):
+        for side in ("R", "S"):
+            n = 1 << 14 if w in hot else int(rng.integers(1 << 11, 1 << 15))
+            sizes[side].append(0 if (w == 4 and side == "S") else n)
+    bounds = np.arange(nwin + 1, dtype=np.uint32) * 100000
+    host = {}
+    for side in ("R", "S"):
+        parts = []
+        for w in range(nwin):
+            n = sizes[side][w]
+            t = np.zeros(n, dtype=aj.STREAM_DTYPE)
+            t["ts"] = bounds[w] + np.sort(rng.integers(0, 100000, n)).astype(np.uint32)
+            dom = max(2, sizes["R"][w] // 4)
+            keys = rng.integers(1, dom, n)
+            if side == "S" and case == "zipf":
+                keys = np.minimum(rng.zipf(1.3, n), dom - 1)
+            if (w in hot and (case == "RS" or (case == "S" and side == "S"))) or \
+                    (case == "S_all" and side == "S"):
+                keys = np.full(n, 42)
+            t["key"] = keys
+            t["payload"] = rng.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32)
+            parts.append(t)
+        host[side] = np.concatenate(parts)
+    want = []
+    for w in range(nwin):
+        win = []
+        for side in ("R", "S"):
+            t = host[side]
+            lo, hi = np.searchsorted(t["ts"], [bounds[w], bounds[w + 1]], side="left")
+            win.append(aj.as_tuples(t["key"][lo:hi].view(np.int32),
+                                    t["payload"][lo:hi].view(np.int32)))
+        want.append(oracle.aggregate(*win))
+    assert want[4] == (0, 0)
+    if case == "RS":
+        assert want[2][0] == (1 << 28)
+    aj.set_option("staged", staged)
+    try:
+        with aj.Context(0) as ctx:
+            dR, dS = aj.DeviceArray.from_host(host["R"]), aj.DeviceArray.from_host(host["S"])
+            db = aj.DeviceArray.from_host(bounds)
+            cnt, chk = aj.DeviceArray(8 * nwin), aj.DeviceArray(8 * nwin)
+            nR, nS = len(host["R"]), len(host["S"])
+            print(f"case {case} staged={staged} nR={nR} nS={nS}", flush=True)
+            with aj.BatchPlan(ctx, nR, nS, nwin, 1 << 15) as plan:
+                for rep in range(2):
+                    plan.run(dR.ptr, nR, dS.ptr, nS, db.ptr, nwin, cnt.ptr, chk.ptr)
+                    aj.device_sync()
+                    got = list(zip(cnt.to_host(np.uint64, nwin).tolist(),
+                                   chk.to_host(np.uint64, nwin).tolist()))
+                    assert got == [(int(c), int(x)) for c, x in want], rep
+    finally:
+        aj.set_option("staged", 1)
