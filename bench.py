@@ -43,7 +43,9 @@ NPO_BYTES_PER_TUPLE = 32
 # SHJ per tuple: read 8 + its own table's slot 16 + a slot of the other table
 # 16 + 32 of its own table's slots initialised
 SHJ_BYTES_PER_TUPLE = 72
-WINDOW_BYTES_PER_TUPLE = 40  # 12 hist read + 12 scatter read + 8 write + 8 build/probe read
+# window batches, per tuple: the staged pass reads 12 and writes 8, the join
+# reads 8; the histogram pass (--no-staged) reads the 12 bytes once more
+WINDOW_BYTES_PER_TUPLE = {True: 28, False: 40}
 # algorithmic bytes per input tuple of each kernel (8-byte tuples), credited for
 # the tuples that kernel reads: NPO's build reads R only, its probe S only.  With
 # the histogram passes pass 1's histogram kernel (k_hist2d) also counts the
@@ -83,6 +85,11 @@ def parse():
     ap.add_argument("--log2n", type=int, default=27)
     ap.add_argument("--theta", type=float, default=1.0)
     ap.add_argument("--nwin", type=int, default=1024)
+    ap.add_argument("--no-graph", action="store_true",
+                    help="windows workload, for counter runs: enqueue the batch's kernels "
+                         "directly instead of replaying a captured hipGraph, and wait for "
+                         "every launch (rocprofv3 --pmc ended the process with launches "
+                         "queued back to back); the times include those waits")
     ap.add_argument("--radix-bits", type=int, default=0)
     ap.add_argument("--passes", type=int, default=0)
     ap.add_argument("--cap", type=int, default=0)
@@ -357,6 +364,16 @@ def random_access_ceiling(aj, line, algo, nR, nS):
     return out
 
 
+def calib_copy(aj):
+    """One aj_copy_kernel over 1 GiB: the traffic counters' reading for a copy of
+    known size, which scales the counters of the other kernels."""
+    src, dst = aj.DeviceArray(1 << 30), aj.DeviceArray(1 << 30)
+    aj.check(aj.lib.aj_copy_d2d(dst.ptr, src.ptr, 1 << 30))
+    aj.device_sync()
+    src.free()
+    dst.free()
+
+
 def run_pro(aj, args):
     n = 1 << args.log2n
     t0 = time.time()
@@ -368,9 +385,7 @@ def run_pro(aj, args):
     dR, dS = aj.DeviceArray.from_host(R), aj.DeviceArray.from_host(S)
     aj.device_sync()
     if args.calib:
-        dst = aj.DeviceArray(dR.nbytes)
-        aj.check(aj.lib.aj_copy_d2d(dst.ptr, dR.ptr, dR.nbytes))
-        dst.free()
+        calib_copy(aj)
     algo = {"PRO": aj.AJ_ALGO_PRO, "NPO": aj.AJ_ALGO_NPO, "SHJ": aj.AJ_ALGO_SHJ}[args.algo]
     kw = dict(radix_bits=args.radix_bits, passes=args.passes, cap=args.cap)
     if algo == aj.AJ_ALGO_SHJ:
@@ -453,6 +468,8 @@ def run_windows(aj, args):
     rng = np.random.default_rng(2024)
     sizes = np.exp(rng.uniform(np.log(4096), np.log(4194304), nwin)).astype(np.uint32)
     cap = int(sizes.sum() * 1.001 + 8 * np.sqrt(sizes.sum()) + 4096)
+    if args.calib:
+        calib_copy(aj)
     with aj.Context(0) as ctx:
         streams = {}
         for side, seed in (("R", 101), ("S", 202)):
@@ -472,13 +489,18 @@ def run_windows(aj, args):
                                            int(sizes.max() * 1.05), reps=2)
             print(f"[bench] batch tuned in {time.perf_counter() - t0:.1f}s: {geom} "
                   f"({tuned_ms:.2f} ms)", file=sys.stderr, flush=True)
-        with aj.BatchPlan(ctx, nR, nS, nwin, int(sizes.max() * 1.05), **geom) as plan:
-            for _ in range(args.warmup):
+        with aj.BatchPlan(ctx, nR, nS, nwin, int(sizes.max() * 1.05), graph=not args.no_graph,
+                          **geom) as plan:
+            def batch():
                 plan.run(dR.ptr, nR, dS.ptr, nS, bR.ptr, nwin, cnt.ptr, chk.ptr)
+                if args.no_graph:
+                    aj.device_sync()
+            for _ in range(args.warmup):
+                batch()
             aj.device_sync()
             t0 = time.perf_counter()
             for _ in range(args.steps):
-                plan.run(dR.ptr, nR, dS.ptr, nS, bR.ptr, nwin, cnt.ptr, chk.ptr)
+                batch()
             aj.device_sync()
             t_batch = (time.perf_counter() - t0) / args.steps
         batch_res = np.stack([cnt.to_host(np.uint64, nwin), chk.to_host(np.uint64, nwin)])
@@ -496,6 +518,8 @@ def run_windows(aj, args):
             def launch(w):
                 one.run(dR.ptr + 12 * int(rng_r[w]), int(per_r[w]), dS.ptr + 12 * int(rng_s[w]),
                         int(per_s[w]), bR.ptr + 4 * w, 1, cnt1.ptr + 8 * w, chk1.ptr + 8 * w)
+                if args.no_graph:
+                    aj.device_sync()
             for w in range(min(nwin, 8)):  # warm
                 launch(w)
             aj.device_sync()
@@ -509,7 +533,8 @@ def run_windows(aj, args):
         in_r = int(rng_r[-1] - rng_r[0])
         in_s = int(rng_s[-1] - rng_s[0])
     tuples = in_r + in_s
-    gbs = WINDOW_BYTES_PER_TUPLE * tuples / t_batch / 1e9
+    bpt = WINDOW_BYTES_PER_TUPLE[not args.no_staged]
+    gbs = bpt * tuples / t_batch / 1e9
     return {
         "metric": "batched window join throughput M tuples/s (|R|+|S| over all windows)",
         "value": round(tuples / t_batch / 1e6, 1),
@@ -537,7 +562,7 @@ def run_windows(aj, args):
                                        f"one sync at the end"},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "whole batched pipeline", "bytes_per_tuple": WINDOW_BYTES_PER_TUPLE},
+                     "kernel": "whole batched pipeline", "bytes_per_tuple": bpt},
         "parity": {"batched_equals_per_window": True},
     }
 

@@ -182,7 +182,13 @@ int aj_write_result_file(const result_t *r, const char *path);
  *                        with atomically reserved, over-allocated partitions (no
  *                        histogram pass); an input whose partitions overflow is
  *                        detected on the device and joined again with the
- *                        histogram passes (default 1)
+ *                        histogram passes.  Window batches (plans of at most 11
+ *                        radix bits per window): one histogram-free pass into
+ *                        over-allocated partitions, S runs past a partition's
+ *                        capacity spilled and gathered into one extension range
+ *                        per partition; R overflow, or a full spill area, runs the
+ *                        histogram pass on the device instead.  Read when a plan
+ *                        or server is created (default 1)
  *   "stage_sets"   0..64 staged: cursor sets of the staging regions (0 = 4)
  *   "stage_chunk"  >=0   staged: tuples per super-chunk (0 = 2^25)
  *   "stage_block"  0, 512, 1024  staged: lanes per work-group (0 = 1024)
@@ -400,6 +406,9 @@ int aj_join_dev_noalloc(const tuple_t *dR, uint64_t nR, const tuple_t *dS, uint6
  * one window per process, benchmark.cpp:164-257; there is no batched
  * counterpart in the reference).  The plan owns the scratch for its maximum
  * sizes; the kernel sequence is captured once into a hipGraph and replayed.
+ * With the option "staged" (default) the windows are partitioned without a
+ * histogram pass; whether a batch fell back to the histogram pass (skewed R
+ * keys) is decided on the device and never changes a result.
  */
 typedef struct aj_batch_opts_t {
     int cap;          /* R tuples per LDS round (4096/8192/8704/12288), 0 = 8704 */
