@@ -396,6 +396,87 @@ __device__ __forceinline__ void probe_count(const uint64_t (&sv)[N], uint32_t ns
     }
 }
 
+// Direct table of a round: when its m R tuples fall into distinct buckets among
+// 2^DLG (the largest power of two <= CAP, so m <= 2^DLG), the table is the
+// bucket array itself, one tuple per slot, and a probe reads one slot.  Keys
+// that are dense above the partition bits (a permutation, as in the
+// reference's unique-key relations) fill it exactly: the reference's
+// bucket_chaining_join gets one tuple per bucket for them too, with
+// next_pow2(|R_p|) buckets (parallel_radix_join.cpp:300-312).  An empty slot h
+// holds a key whose bucket is h ^ 1, so no key probing slot h equals it; this
+// needs the partition bits plus DLG to stay within 31 bits.
+template <int CAP> __host__ __device__ constexpr int direct_lg() {
+    int l = 0;
+    while ((2 << l) <= CAP) ++l;
+    return l;
+}
+
+__device__ __forceinline__ uint64_t direct_empty(uint32_t h, uint32_t sh) {
+    return (uint64_t)((h ^ 1u) << sh) << 32;
+}
+
+// Places the round's tuples (element q of rv is tuple tid + q*JB, valid below
+// m) into their slots with one LDS compare-and-swap each.  Returns, the same
+// for every lane, whether no two tuples met in a slot (a bucket clash or a
+// duplicate key): then the table is complete; otherwise the caller builds the
+// bucket table instead.
+template <int JB, int CAP>
+__device__ __forceinline__ bool build_direct(const uint64_t (&rv)[CAP / JB], uint32_t m,
+                                             uint32_t sh, uint64_t *tab, uint32_t *s_flag,
+                                             uint32_t tid) {
+    constexpr int ITEMS = CAP / JB;
+    constexpr int DLG = direct_lg<CAP>();
+    constexpr uint32_t NSLOT = 1u << DLG;
+    for (uint32_t i = tid; i < NSLOT; i += JB) tab[i] = direct_empty(i, sh);
+    if (tid == 0) *s_flag = 0;
+    __syncthreads();
+    const uint32_t hp = (uint32_t)DLG | (sh << 8);
+    const uint32_t wave0 = __builtin_amdgcn_readfirstlane(tid);
+    bool clash = false;
+    unroll<ITEMS>([&](auto q) {
+        if (wave0 + q * JB < m && tid + q * JB < m) {
+            const uint32_t h = bucket_of(key_of(rv[q]), hp);
+            const unsigned long long e = direct_empty(h, sh);
+            clash |= atomicCAS(reinterpret_cast<unsigned long long *>(tab + h), e,
+                               (unsigned long long)rv[q]) != e;
+        }
+    });
+    if (clash) *s_flag = 1;
+    __syncthreads();
+    return *s_flag == 0;
+}
+
+// Counting probe of S tuples held in registers (element j of `sv` is tuple
+// tid + (Q0 + j)*JB of the slice, valid below ns) against a direct table: one
+// slot read and one 32-bit key test per tuple; the checksum as in probe_count.
+template <int JB, int CAP, int Q0, int N>
+__device__ __forceinline__ void probe_direct(const uint64_t (&sv)[N], uint32_t ns,
+                                             const uint64_t *tab, uint32_t sh, uint64_t &cnt,
+                                             uint64_t &sum, uint32_t tid) {
+    constexpr int DLG = direct_lg<CAP>();
+    if (__builtin_amdgcn_readfirstlane(tid) + (uint32_t)Q0 * JB >= ns) return;
+    const uint32_t hp = (uint32_t)DLG | (sh << 8);
+    const uint2 *tab2 = reinterpret_cast<const uint2 *>(tab);
+    uint2 r[N];
+    uint32_t sk[N];
+    unroll<N>([&](auto j) {
+        uint32_t k = key_of(sv[j]);
+        asm volatile("" : "+v"(k));  // a 32-bit value to the compiler: 32-bit key tests
+        sk[j] = k;
+        r[j] = tab2[bucket_of(k, hp)];
+    });
+    uint32_t mt = 0, rsum = 0;
+    uint64_t ssum = 0;
+    unroll<N>([&](auto j) {
+        const bool h = (int)(tid + (uint32_t)(Q0 + j) * JB < ns) & (int)(r[j].y == sk[j]);
+        mt += h ? 1u : 0u;
+        rsum += h ? r[j].x : 0u;
+        ssum += h ? payload_of(sv[j]) : 0u;
+    });
+    cnt += mt;
+    sum += ssum + ((uint64_t)rsum << 32);
+}
+
 // Tuples tid + (Q0 + j)*JB, j < N, of `p` (n >= 1 tuples) into registers.
 // The loads are unconditional so that all of them are in flight at once: an
 // index at or past n reads tuple n-1 again, and callers ignore those elements.
@@ -436,7 +517,12 @@ __global__ __launch_bounds__(JB, 4) void k_join(JoinArgs a) {
     __shared__ uint64_t red[2 * (JB / 64)];
     __shared__ uint32_t scan_tmp[JB / 64];
     __shared__ uint64_t s_base;
+    __shared__ uint32_t s_dflag;
     constexpr int NW = JB / 64;
+    constexpr int DLG = direct_lg<CAP>();
+    // rounds left before the direct table is tried again after a clash (so
+    // that keys that never fit it cost one attempt in 16 rounds)
+    uint32_t dskip = 0;
 
     // Static round-robin over tasks (a shared device-wide task counter costs one
     // serialised atomic per task).  The S slice (at most CAP tuples) is read
@@ -463,11 +549,20 @@ __global__ __launch_bounds__(JB, 4) void k_join(JoinArgs a) {
             // one LDS round of R: CAP tuples (more rounds only for partitions
             // larger than CAP)
             const uint32_t m = (uint32_t)min<uint64_t>(CAP, cur.r0 + cur.nr - rb);
-            uint32_t hp;
+            uint32_t hp = 0;
+            bool direct = false;
             {
                 uint64_t rv[ITEMS];
                 load_part<JB, 0, ITEMS>(rv, a.R + rb, m, opaque_tid());
-                hp = build_regs<JB, CAP>(rv, m, cur.sh, tab, bst, scan_tmp, opaque_tid());
+                if (!PAIRS && m <= (1u << DLG) && cur.sh + DLG <= 31) {
+                    if (dskip == 0) {
+                        direct = build_direct<JB, CAP>(rv, m, cur.sh, tab, &s_dflag, opaque_tid());
+                        if (!direct) dskip = 16;
+                    } else {
+                        --dskip;
+                    }
+                }
+                if (!direct) hp = build_regs<JB, CAP>(rv, m, cur.sh, tab, bst, scan_tmp, opaque_tid());
             }
             // the S slice in sub-slices of CAP (one, unless a heavy partition
             // gives the task more): the first half of the first sub-slice was
@@ -481,8 +576,13 @@ __global__ __launch_bounds__(JB, 4) void k_join(JoinArgs a) {
                 uint64_t sb[SB];
                 load_part<JB, SA, SB>(sb, a.S + cur.s0 + so, sub, opaque_tid());
                 uint64_t c = 0, x = 0, wpos = 0;
-                probe_count<JB, CAP, 0, SA>(sa, sub, m, tab, bst, hp, c, x, opaque_tid());
-                probe_count<JB, CAP, SA, SB>(sb, sub, m, tab, bst, hp, c, x, opaque_tid());
+                if (!PAIRS && direct) {
+                    probe_direct<JB, CAP, 0, SA>(sa, sub, tab, cur.sh, c, x, opaque_tid());
+                    probe_direct<JB, CAP, SA, SB>(sb, sub, tab, cur.sh, c, x, opaque_tid());
+                } else {
+                    probe_count<JB, CAP, 0, SA>(sa, sub, m, tab, bst, hp, c, x, opaque_tid());
+                    probe_count<JB, CAP, SA, SB>(sb, sub, m, tab, bst, hp, c, x, opaque_tid());
+                }
                 if (PAIRS) {
                     uint32_t total_c;
                     const uint32_t pre = block_exclusive_scan<JB>((uint32_t)c, scan_tmp, &total_c);
