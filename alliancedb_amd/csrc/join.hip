@@ -416,62 +416,85 @@ __device__ __forceinline__ uint64_t direct_empty(uint32_t h, uint32_t sh) {
 }
 
 // Places the round's tuples (element q of rv is tuple tid + q*JB, valid below
-// m) into their slots with one LDS compare-and-swap each.  Returns, the same
-// for every lane, whether no two tuples met in a slot (a bucket clash or a
-// duplicate key): then the table is complete; otherwise the caller builds the
-// bucket table instead.
+// m) into their slots with one LDS compare-and-swap each.  A tuple whose slot
+// already holds its key (a duplicate) adds its payload to the slot's payload
+// half and 1 to the slot's 8-bit count of further tuples (`ext`, 2^DLG bytes
+// packed in words; read only when a duplicate was seen).  Returns, the same for
+// every lane, 0 when two different keys met in a slot or a count passed 255
+// (the caller builds the bucket table instead), 1 for a table of distinct
+// keys, 2 for one with duplicates.
 template <int JB, int CAP>
-__device__ __forceinline__ bool build_direct(const uint64_t (&rv)[CAP / JB], uint32_t m,
-                                             uint32_t sh, uint64_t *tab, uint32_t *s_flag,
-                                             uint32_t tid) {
+__device__ __forceinline__ uint32_t build_direct(const uint64_t (&rv)[CAP / JB], uint32_t m,
+                                                 uint32_t sh, uint64_t *tab, uint32_t *ext,
+                                                 uint32_t *s_flag, uint32_t tid) {
     constexpr int ITEMS = CAP / JB;
     constexpr int DLG = direct_lg<CAP>();
     constexpr uint32_t NSLOT = 1u << DLG;
     for (uint32_t i = tid; i < NSLOT; i += JB) tab[i] = direct_empty(i, sh);
+    for (uint32_t i = tid; i < NSLOT / 4; i += JB) ext[i] = 0;
     if (tid == 0) *s_flag = 0;
     __syncthreads();
     const uint32_t hp = (uint32_t)DLG | (sh << 8);
     const uint32_t wave0 = __builtin_amdgcn_readfirstlane(tid);
-    bool clash = false;
+    uint32_t fl = 0;  // 1: clash, 2: duplicate
     unroll<ITEMS>([&](auto q) {
         if (wave0 + q * JB < m && tid + q * JB < m) {
-            const uint32_t h = bucket_of(key_of(rv[q]), hp);
+            const uint32_t k = key_of(rv[q]);
+            const uint32_t h = bucket_of(k, hp);
             const unsigned long long e = direct_empty(h, sh);
-            clash |= atomicCAS(reinterpret_cast<unsigned long long *>(tab + h), e,
-                               (unsigned long long)rv[q]) != e;
+            const unsigned long long old = atomicCAS(
+                reinterpret_cast<unsigned long long *>(tab + h), e, (unsigned long long)rv[q]);
+            if (old != e) {
+                if ((uint32_t)(old >> 32) == k) {
+                    uint32_t *slot = reinterpret_cast<uint32_t *>(tab + h);
+                    atomicAdd(slot, payload_of(rv[q]));
+                    const uint32_t sb = 8 * (h & 3u);
+                    const uint32_t c = atomicAdd(ext + (h >> 2), 1u << sb);
+                    fl |= ((c >> sb) & 0xffu) == 0xffu ? 1u : 2u;
+                } else {
+                    fl |= 1u;
+                }
+            }
         }
     });
-    if (clash) *s_flag = 1;
+    if (fl) atomicOr(s_flag, fl);
     __syncthreads();
-    return *s_flag == 0;
+    const uint32_t f = *s_flag;
+    return (f & 1u) ? 0u : (f ? 2u : 1u);
 }
 
 // Counting probe of S tuples held in registers (element j of `sv` is tuple
 // tid + (Q0 + j)*JB of the slice, valid below ns) against a direct table: one
-// slot read and one 32-bit key test per tuple; the checksum as in probe_count.
-template <int JB, int CAP, int Q0, int N>
+// slot read and one 32-bit key test per tuple (with DUPS also the slot's count
+// of further tuples); the checksum as in probe_count.
+template <int JB, int CAP, bool DUPS, int Q0, int N>
 __device__ __forceinline__ void probe_direct(const uint64_t (&sv)[N], uint32_t ns,
-                                             const uint64_t *tab, uint32_t sh, uint64_t &cnt,
-                                             uint64_t &sum, uint32_t tid) {
+                                             const uint64_t *tab, const uint32_t *ext,
+                                             uint32_t sh, uint64_t &cnt, uint64_t &sum,
+                                             uint32_t tid) {
     constexpr int DLG = direct_lg<CAP>();
     if (__builtin_amdgcn_readfirstlane(tid) + (uint32_t)Q0 * JB >= ns) return;
     const uint32_t hp = (uint32_t)DLG | (sh << 8);
     const uint2 *tab2 = reinterpret_cast<const uint2 *>(tab);
     uint2 r[N];
-    uint32_t sk[N];
+    uint32_t sk[N], hs[N];
     unroll<N>([&](auto j) {
         uint32_t k = key_of(sv[j]);
         asm volatile("" : "+v"(k));  // a 32-bit value to the compiler: 32-bit key tests
         sk[j] = k;
-        r[j] = tab2[bucket_of(k, hp)];
+        hs[j] = bucket_of(k, hp);
+        r[j] = tab2[hs[j]];
     });
     uint32_t mt = 0, rsum = 0;
     uint64_t ssum = 0;
     unroll<N>([&](auto j) {
         const bool h = (int)(tid + (uint32_t)(Q0 + j) * JB < ns) & (int)(r[j].y == sk[j]);
-        mt += h ? 1u : 0u;
+        uint32_t c = 1;
+        if constexpr (DUPS) c += (ext[hs[j] >> 2] >> (8 * (hs[j] & 3u))) & 0xffu;
+        c = h ? c : 0u;
+        mt += c;
         rsum += h ? r[j].x : 0u;
-        ssum += h ? payload_of(sv[j]) : 0u;
+        ssum += (uint64_t)c * payload_of(sv[j]);
     });
     cnt += mt;
     sum += ssum + ((uint64_t)rsum << 32);
@@ -514,6 +537,10 @@ __global__ __launch_bounds__(JB, 4) void k_join(JoinArgs a) {
     extern __shared__ uint64_t jsm[];
     uint64_t *tab = jsm;  // CAP tuples in bucket order + a spare slot
     uint16_t *bst = reinterpret_cast<uint16_t *>(jsm + CAP + 1);  // max_buckets(CAP) + 3 bounds
+    // a direct table's 8-bit counts of further tuples per slot, in bst's place
+    uint32_t *ext = reinterpret_cast<uint32_t *>(bst);
+    static_assert((1u << direct_lg<CAP>()) <= 2 * (max_buckets(CAP) + 4),
+                  "a direct table's counts fit the bucket bounds' space");
     __shared__ uint64_t red[2 * (JB / 64)];
     __shared__ uint32_t scan_tmp[JB / 64];
     __shared__ uint64_t s_base;
@@ -550,13 +577,14 @@ __global__ __launch_bounds__(JB, 4) void k_join(JoinArgs a) {
             // larger than CAP)
             const uint32_t m = (uint32_t)min<uint64_t>(CAP, cur.r0 + cur.nr - rb);
             uint32_t hp = 0;
-            bool direct = false;
+            uint32_t direct = 0;  // 1: direct table, 2: direct table with duplicates
             {
                 uint64_t rv[ITEMS];
                 load_part<JB, 0, ITEMS>(rv, a.R + rb, m, opaque_tid());
                 if (!PAIRS && m <= (1u << DLG) && cur.sh + DLG <= 31) {
                     if (dskip == 0) {
-                        direct = build_direct<JB, CAP>(rv, m, cur.sh, tab, &s_dflag, opaque_tid());
+                        direct = build_direct<JB, CAP>(rv, m, cur.sh, tab, ext, &s_dflag,
+                                                       opaque_tid());
                         if (!direct) dskip = 16;
                     } else {
                         --dskip;
@@ -576,9 +604,12 @@ __global__ __launch_bounds__(JB, 4) void k_join(JoinArgs a) {
                 uint64_t sb[SB];
                 load_part<JB, SA, SB>(sb, a.S + cur.s0 + so, sub, opaque_tid());
                 uint64_t c = 0, x = 0, wpos = 0;
-                if (!PAIRS && direct) {
-                    probe_direct<JB, CAP, 0, SA>(sa, sub, tab, cur.sh, c, x, opaque_tid());
-                    probe_direct<JB, CAP, SA, SB>(sb, sub, tab, cur.sh, c, x, opaque_tid());
+                if (!PAIRS && direct == 1) {
+                    probe_direct<JB, CAP, false, 0, SA>(sa, sub, tab, ext, cur.sh, c, x, opaque_tid());
+                    probe_direct<JB, CAP, false, SA, SB>(sb, sub, tab, ext, cur.sh, c, x, opaque_tid());
+                } else if (!PAIRS && direct == 2) {
+                    probe_direct<JB, CAP, true, 0, SA>(sa, sub, tab, ext, cur.sh, c, x, opaque_tid());
+                    probe_direct<JB, CAP, true, SA, SB>(sb, sub, tab, ext, cur.sh, c, x, opaque_tid());
                 } else {
                     probe_count<JB, CAP, 0, SA>(sa, sub, m, tab, bst, hp, c, x, opaque_tid());
                     probe_count<JB, CAP, SA, SB>(sb, sub, m, tab, bst, hp, c, x, opaque_tid());

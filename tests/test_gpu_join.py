@@ -467,3 +467,24 @@ def test_hash_tables_with_hot_keys(aj, oracle, algo, kind):
     print(f"case hot keys {algo} {kind} count={want[0]}", flush=True)
     got = getattr(aj, algo)(R, S, 8)
     assert (got.count, got.checksum) == want
+
+
+@pytest.mark.parametrize("copies", [255, 256, 257])
+def test_join_round_table_duplicate_counts(aj, oracle, copies):
+    """A round's direct table counts the further copies of a key in 8 bits: R a
+    permutation of 1..2^18 plus copies of three of its keys (one of them
+    `copies` times in all, around the 8-bit limit, past which the round goes to
+    the bucket table), S uniform over R's keys.  Counts and checksums equal the
+    oracle's."""
+    n = 1 << 18
+    rng = np.random.default_rng(copies)
+    base = rng.permutation(np.arange(1, n + 1))
+    extra = np.concatenate([np.full(copies - 1, 1000), np.full(7, 2000), np.full(3, 3000)])
+    keys = np.concatenate([base, extra]).astype(np.int64)
+    rng.shuffle(keys)
+    R = aj.as_tuples(keys.astype(np.int32), rng.integers(-2**31, 2**31, len(keys)).astype(np.int32))
+    S = aj.as_tuples(rng.integers(1, n + 1, 2 * n).astype(np.int32),
+                     rng.integers(-2**31, 2**31, 2 * n).astype(np.int32))
+    print(f"case copies={copies} nR={len(R)} nS={len(S)}", flush=True)
+    got = aj.PRO(R, S, 8)
+    assert (got.count, got.checksum) == oracle.aggregate(R, S)
