@@ -146,7 +146,8 @@ class aj_server_stats_t(ctypes.Structure):
 class aj_tune_entry_t(ctypes.Structure):
     _fields_ = [("log2_r", ctypes.c_int), ("log2_s", ctypes.c_int), ("radix_bits", ctypes.c_int),
                 ("passes", ctypes.c_int), ("cap", ctypes.c_int), ("part_block", ctypes.c_int),
-                ("part_chunk", ctypes.c_uint32), ("device_ms", ctypes.c_float)]
+                ("part_chunk", ctypes.c_uint32), ("device_ms", ctypes.c_float),
+                ("one_pass_hist", ctypes.c_int)]
 
     def as_dict(self) -> dict:
         return {n: getattr(self, n) for n, _ in self._fields_}

@@ -79,6 +79,7 @@ ProConfig choose_pro_config(uint64_t nR, uint64_t nS, const aj_join_opts_t *opts
 bool tune_lookup(uint64_t nR, uint64_t nS, aj_tune_entry_t *out);
 
 // ---------------------------------------------------------------- context
+struct BatchPlan;
 struct Ctx {
     int device = 0;
     int num_cu = 256;
@@ -96,6 +97,8 @@ struct Ctx {
     DevBuf carry, carry2;       // SHJ: carries of the per-key payload sums (R, S)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;  // none: no timing (device_ms 0)
     bool no_events = false;     // never create events (no per-kernel timing either)
+    bool arena_only = false;    // buffers come from a fixed scratch array (no allocation)
+    BatchPlan *pro1 = nullptr;  // one-pass staged PRO: a one-segment window plan
     hipEvent_t evk[AJ_NPHASES + 1] = {};  // per-kernel timing (option kernel_timing)
     std::vector<hipEvent_t> evs;          // per-launch timing of the staged passes
 };
@@ -419,11 +422,17 @@ void batch_plan_release(BatchPlan *p);
 // Device block of the plan's window ranges: rb | re | sb | se, max_win u64
 // entries each (tuple indices into the R and S arrays, end exclusive).
 uint64_t *batch_ranges(BatchPlan *p);
+// Device word that is nonzero after a run whose staged pass fell back to the
+// histogram pass (plans with staged set).
+const uint32_t *batch_fallback_flag(BatchPlan *p);
 // Enqueues the joins of nwin windows whose ranges are in batch_ranges(p);
 // nR/nS = tuples in those windows.  Per-window count/checksum into cnt/sum.
-int enqueue_ranges(BatchPlan *p, const aj_stream_tuple_t *dR, uint64_t nR,
-                   const aj_stream_tuple_t *dS, uint64_t nS, int nwin, uint64_t *cnt,
-                   uint64_t *sum, hipStream_t st);
+// The inputs are 12-byte aj_stream_tuple_t (stride 12) or 8-byte tuple_t
+// (stride 8); marks (nullable): m0+1..m0+3 after the planning, m0+4..m0+8 after
+// the partitioning, m0+9 and m0+10 from the join.
+int enqueue_ranges(BatchPlan *p, const void *dR, uint64_t nR, const void *dS, uint64_t nS,
+                   int nwin, uint64_t *cnt, uint64_t *sum, hipStream_t st, int stride = 12,
+                   const Marks *mk = nullptr, int m0 = 0);
 
 // fill kernel (graph-safe replacement of hipMemsetAsync)
 int fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t st);

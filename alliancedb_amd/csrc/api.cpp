@@ -153,6 +153,11 @@ static void ctx_fini(Ctx *c) {
                       &c->table, &c->misc, &c->hist2d, &c->steps, &c->stage,
                       &c->chain, &c->chain2, &c->heads, &c->heads2, &c->carry, &c->carry2};
     for (DevBuf *b : bufs) b->release();
+    if (c->pro1) {
+        batch_plan_release(c->pro1);
+        delete c->pro1;
+        c->pro1 = nullptr;
+    }
     for (hipEvent_t e : c->evs) (void)hipEventDestroy(e);
     c->evs.clear();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -206,6 +211,8 @@ static ProConfig config_from(uint64_t nR, uint64_t nS, const aj_join_opts_t *o) 
     if (opt_part_chunk.load()) c.chunk = (int)opt_part_chunk.load();
     c.p2d = opt_part_hist2d.load() != 0;
     c.staged = opt_staged.load() != 0;
+    // a tuned one-pass geometry that was fastest through the histogram pass
+    if (use_table && te.one_pass_hist) c.staged = false;
     c.stage_sets = (int)opt_stage_sets.load();
     c.stage_chunk = (uint64_t)opt_stage_chunk.load();
     c.stage_block = (int)opt_stage_block.load();

@@ -60,6 +60,7 @@ int noalloc_setup(NoallocState &s, int dev) {
     s.ctx.num_cu = ncu > 0 ? ncu : 256;
     s.ctx.stream = nullptr;
     s.ctx.no_events = true;
+    s.ctx.arena_only = true;
     for_each_buf(s.ctx, [&](DevBuf &b) { b.arena = &s.arena; });
     s.ready = true;
     return AJ_OK;

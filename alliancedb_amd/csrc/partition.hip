@@ -743,7 +743,7 @@ __global__ __launch_bounds__(WPLAN_BLOCK) void k_wstage_caps(PassArgs a, WStageA
 
 // One work-group per chunk of a window: the chunk's tiles counting-sorted by
 // digit in LDS, each run placed by one atomic reservation in its partition.
-template <int PBLOCK>
+template <int PBLOCK, int STRIDE>
 __global__ __launch_bounds__(PBLOCK, 4) void k_wscatter(PassArgs a, WStageArgs w) {
     constexpr int PTILE = PBLOCK * PITEMS;
     extern __shared__ uint64_t smem[];
@@ -788,7 +788,7 @@ __global__ __launch_bounds__(PBLOCK, 4) void k_wscatter(PassArgs a, WStageArgs w
         sp.max = w.sp_max;
         sp.part0 = a.pbase[L.seg];
     }
-    stage_tiles<PBLOCK, true, false, false, decltype(src), 12, WSTAGE_MAXF, true>(
+    stage_tiles<PBLOCK, true, false, false, decltype(src), STRIDE, WSTAGE_MAXF, true>(
         src, L.rel ? a.out1 : a.out0, a.shift, F, buf, cnt, dst, &s_len, &sp);
 }
 
@@ -1197,7 +1197,7 @@ static WStageArgs wstage_args(const WStageDesc &ws) {
 
 int run_window_stage(const PassDesc &d, const PassWork &wk, const WStageDesc &ws,
                      hipStream_t st) {
-    if (d.in_stride != 12 || d.max_bits > WSTAGE_MAX_BITS || d.nseg > ws.W ||
+    if ((d.in_stride != 12 && d.in_stride != 8) || d.max_bits > WSTAGE_MAX_BITS || d.nseg > ws.W ||
         ws.out_cap[0] >= (1ull << 32) || ws.out_cap[1] + ws.spill_cap >= (1ull << 32) ||
         ws.out_cap[0] < 2 * 1024 * PITEMS || ws.out_cap[1] < 2 * 1024 * PITEMS) {
         set_error(AJ_ERR_ARG, "staged window pass: unsupported geometry");
@@ -1208,9 +1208,12 @@ int run_window_stage(const PassDesc &d, const PassWork &wk, const WStageDesc &ws
     hipLaunchKernelGGL(k_wstage_caps, dim3(1), dim3(WPLAN_BLOCK), 0, st, a, w);
     constexpr int B = 1024;
     const size_t lds = stage_lds_bytes(B, 1u << d.max_bits);
-    AJ_TRY(lds_limit_set((const void *)k_wscatter<B>, lds));
-    if (d.max_chunks)
-        hipLaunchKernelGGL(k_wscatter<B>, dim3((unsigned)d.max_chunks), dim3(B), lds, st, a, w);
+    const void *fn = d.in_stride == 12 ? (const void *)k_wscatter<B, 12> : (const void *)k_wscatter<B, 8>;
+    AJ_TRY(lds_limit_set(fn, lds));
+    if (d.max_chunks && d.in_stride == 12)
+        hipLaunchKernelGGL((k_wscatter<B, 12>), dim3((unsigned)d.max_chunks), dim3(B), lds, st, a, w);
+    else if (d.max_chunks)
+        hipLaunchKernelGGL((k_wscatter<B, 8>), dim3((unsigned)d.max_chunks), dim3(B), lds, st, a, w);
     AJ_HIP(hipGetLastError());
     return AJ_OK;
 }

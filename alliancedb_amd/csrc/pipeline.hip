@@ -362,9 +362,91 @@ static int pro_staged(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t 
     return AJ_OK;
 }
 
+// One pass of at most 11 radix bits without a histogram: the join is a window
+// batch of one segment (windows.hip), whose staged pass reserves each tile
+// run's room by atomics, spills S runs that overflow a partition into the
+// partition's extension range, and reruns the pass from a histogram on the
+// device only when an R partition overflows.  Counting joins on inputs of at
+// least 2^20 tuples in total; the context's plan is kept for later calls.
+static bool staged1_applies(const Ctx *ctx, uint64_t nR, uint64_t nS, const ProConfig &c,
+                            const uint64_t *pairs) {
+    return c.staged && staged_enabled() && !ctx->arena_only && !pairs && c.bits2 == 0 &&
+           c.bits1 >= 1 && c.bits1 <= WSTAGE_MAX_BITS && nR + nS >= (1ull << 20) &&
+           (c.cap == 4096 || c.cap == 8192 || c.cap == 8704 || c.cap == 12288);
+}
+
+__global__ void k_one_range(uint64_t *rb, uint64_t nR, uint64_t nS) {
+    if (threadIdx.x == 0) {
+        rb[0] = 0;   // rb
+        rb[1] = nR;  // re
+        rb[2] = 0;   // sb
+        rb[3] = nS;  // se
+    }
+}
+
+static int pro_staged1(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS,
+                       uint64_t nS, const ProConfig &cfg, aj_join_out_t *out, hipStream_t st) {
+    // the window plan sizes a window's bits by its R tuples over `target`
+    const uint32_t target = (uint32_t)((nR + (1ull << cfg.bits1) - 1) >> cfg.bits1);
+    BatchPlan *&p = ctx->pro1;
+    if (!p || p->max_R < nR || p->max_S < nS || p->max_window != nR || p->target != target ||
+        p->cap != cfg.cap || p->split != join_task_split((uint32_t)cfg.split, cfg.cap) ||
+        p->staged != staged_enabled()) {
+        if (p) {
+            batch_plan_release(p);
+            delete p;
+            p = nullptr;
+        }
+        p = new BatchPlan();
+        aj_batch_opts_t bo;
+        memset(&bo, 0, sizeof(bo));
+        bo.cap = cfg.cap;
+        bo.split = (uint32_t)cfg.split;
+        bo.target = std::max<uint32_t>(target, 1);
+        bo.no_graph = 1;
+        const int rc = batch_plan_setup(p, ctx, nR, nS, 1, nR, &bo);
+        if (rc != AJ_OK) {
+            batch_plan_release(p);
+            delete p;
+            p = nullptr;
+            return rc;
+        }
+    }
+    AJ_TRY(ctx->acc.ensure(64));
+    uint64_t *res = ctx->acc.as<uint64_t>();
+    Marks mk;
+    const Marks *mkp = nullptr;
+    if (phase_timing(ctx)) {
+        AJ_TRY(ensure_phase_events(ctx));
+        mk.ev = ctx->evk;
+        mk.st = st;
+        mkp = &mk;
+    }
+    AJ_HIP(record_event(ctx->ev0, st));
+    mk.mark(0);
+    hipLaunchKernelGGL(k_one_range, dim3(1), dim3(64), 0, st, batch_ranges(p), nR, nS);
+    AJ_TRY(enqueue_ranges(p, dR, nR, dS, nS, 1, res, res + 1, st, 8, mkp, 0));
+    AJ_HIP(record_event(ctx->ev1, st));
+    uint64_t host[2];
+    uint32_t flag = 0;
+    AJ_HIP(hipMemcpyAsync(host, res, sizeof(host), hipMemcpyDeviceToHost, st));
+    if (p->staged)
+        AJ_HIP(hipMemcpyAsync(&flag, batch_fallback_flag(p), sizeof(flag),
+                              hipMemcpyDeviceToHost, st));
+    AJ_HIP(hipStreamSynchronize(st));
+    out->count = host[0];
+    out->checksum = host[1];
+    out->npairs = 0;
+    out->device_ms = elapsed_ms(ctx->ev0, ctx->ev1);
+    out->flags = p->staged ? (flag ? AJ_OUT_FALLBACK : AJ_OUT_STAGED) : 0;
+    if (mkp) read_phases(ctx, out, AJ_PH_PLAN1, AJ_PH_JOIN + 1);
+    return AJ_OK;
+}
+
 int pro_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, uint64_t nS,
                const ProConfig &cfg, uint64_t *pairs, uint64_t pairs_cap, int records,
                aj_join_out_t *out, hipStream_t st) {
+    if (staged1_applies(ctx, nR, nS, cfg, pairs)) return pro_staged1(ctx, dR, nR, dS, nS, cfg, out, st);
     float attempt_ms = 0;
     if (staged_applies(nR, nS, cfg)) {
         bool overflow = false;

@@ -43,7 +43,8 @@ bool valid_entry(const aj_tune_entry_t &e) {
                         e.cap == 12288;
     return e.log2_r >= 0 && e.log2_r < 32 && e.log2_s >= 0 && e.log2_s < 32 && e.radix_bits >= 0 &&
            e.radix_bits <= 24 && e.passes >= 0 && e.passes <= 2 && cap_ok &&
-           (e.part_block == 256 || e.part_block == 512) && e.part_chunk % 8192 == 0;
+           (e.part_block == 256 || e.part_block == 512) && e.part_chunk % 8192 == 0 &&
+           (e.one_pass_hist == 0 || e.one_pass_hist == 1);
 }
 
 }  // namespace
@@ -110,9 +111,10 @@ int aj_tune_save(const char *path) {
         const aj_tune_entry_t &e = es[i];
         fprintf(f,
                 "{\"log2_r\": %d, \"log2_s\": %d, \"radix_bits\": %d, \"passes\": %d, \"cap\": %d, "
-                "\"part_block\": %d, \"part_chunk\": %u, \"device_ms\": %.4f}%s\n",
+                "\"part_block\": %d, \"part_chunk\": %u, \"device_ms\": %.4f, "
+                "\"one_pass_hist\": %d}%s\n",
                 e.log2_r, e.log2_s, e.radix_bits, e.passes, e.cap, e.part_block, e.part_chunk,
-                e.device_ms, i + 1 < es.size() ? "," : "");
+                e.device_ms, e.one_pass_hist, i + 1 < es.size() ? "," : "");
     }
     fprintf(f, "]}\n");
     const int bad = ferror(f);
@@ -142,12 +144,14 @@ int aj_tune_load(const char *path) {
         if (!strstr(line, "\"log2_r\"")) continue;
         aj_tune_entry_t e;
         memset(&e, 0, sizeof(e));
-        if (sscanf(line,
-                   "{\"log2_r\": %d, \"log2_s\": %d, \"radix_bits\": %d, \"passes\": %d, \"cap\": %d, "
-                   "\"part_block\": %d, \"part_chunk\": %u, \"device_ms\": %f}",
-                   &e.log2_r, &e.log2_s, &e.radix_bits, &e.passes, &e.cap, &e.part_block,
-                   &e.part_chunk, &e.device_ms) != 8 ||
-            !valid_entry(e)) {
+        // tables written before one_pass_hist existed end after device_ms (0)
+        const int got = sscanf(line,
+                               "{\"log2_r\": %d, \"log2_s\": %d, \"radix_bits\": %d, \"passes\": %d, "
+                               "\"cap\": %d, \"part_block\": %d, \"part_chunk\": %u, \"device_ms\": %f, "
+                               "\"one_pass_hist\": %d}",
+                               &e.log2_r, &e.log2_s, &e.radix_bits, &e.passes, &e.cap,
+                               &e.part_block, &e.part_chunk, &e.device_ms, &e.one_pass_hist);
+        if (got < 8 || !valid_entry(e)) {
             fclose(f);
             set_error(AJ_ERR_ARG, "%s: malformed entry: %s", path, line);
             return AJ_ERR_ARG;
@@ -177,16 +181,22 @@ int aj_tune_pro(aj_ctx_t *ctx, const tuple_t *dR, uint64_t nR, const tuple_t *dS
     const ProConfig h = choose_pro_config(nR, nS, &o0);
     const int b0 = h.bits1 + h.bits2;
     struct Cand {
-        int bits, passes, cap, block;
+        int bits, passes, cap, block, hist;  // hist: one pass through the histogram pass
     };
     std::vector<Cand> cands;
+    const bool staged_on = aj_get_option("staged") != 0;
     for (int bits = std::max(1, b0 - 1); bits <= std::min(24, b0 + 1); ++bits)
         for (int passes = 1; passes <= 2; ++passes) {
             if (passes == 1 && bits > 12) continue;  // one pass: runs of >= 2 tuples per 8K tile
             if (passes == 2 && bits < 4) continue;
-            for (int cap : {8192, 8704, 12288}) cands.push_back({bits, passes, cap, 512});
+            for (int cap : {8192, 8704, 12288}) {
+                cands.push_back({bits, passes, cap, 512, 0});
+                // one pass of <= 11 bits runs the staged pass unless told otherwise
+                if (passes == 1 && bits <= WSTAGE_MAX_BITS && staged_on)
+                    cands.push_back({bits, passes, cap, 512, 1});
+            }
         }
-    if (b0 == 0) cands.push_back({0, 0, 8704, 512});
+    if (b0 == 0) cands.push_back({0, 0, 8704, 512, 0});
     uint64_t want_c = 0, want_x = 0;
     bool have = false;
     float best_ms = 1e30f;
@@ -199,7 +209,9 @@ int aj_tune_pro(aj_ctx_t *ctx, const tuple_t *dR, uint64_t nR, const tuple_t *dS
         o.passes = c.passes;
         o.cap = c.cap;
         const int64_t saved_block = aj_get_option("part_block");
+        const int64_t saved_staged = aj_get_option("staged");
         aj_set_option("part_block", c.block);
+        if (c.hist) aj_set_option("staged", 0);
         aj_join_out_t out;
         int rc = aj_join_dev(ctx, dR, nR, dS, nS, &o, &out, nullptr);  // warm-up
         std::vector<float> t;
@@ -208,6 +220,7 @@ int aj_tune_pro(aj_ctx_t *ctx, const tuple_t *dR, uint64_t nR, const tuple_t *dS
             t.push_back(out.device_ms);
         }
         aj_set_option("part_block", saved_block);
+        aj_set_option("staged", saved_staged);
         if (rc != AJ_OK) return rc;
         if (!have) {
             want_c = out.count;
@@ -250,6 +263,7 @@ int aj_tune_pro(aj_ctx_t *ctx, const tuple_t *dR, uint64_t nR, const tuple_t *dS
     e.part_block = win.block;
     e.part_chunk = 0;
     e.device_ms = best_ms;
+    e.one_pass_hist = win.hist;
     AJ_TRY(aj_tune_put(&e));
     *best = e;
     return AJ_OK;

@@ -94,9 +94,12 @@ int batch_plan_init(BatchPlan *p) {
     // partitions per window: 2^bits with (n >> (bits-1)) > target, so at most
     // 2n/target + 1 per window
     p->max_parts = std::min<uint64_t>(W << b, 2 * p->max_R / p->target + W);
-    // the staged pass's S spill area: half of max_S, runs of 64 tuples on average
-    p->spill_cap = std::max<uint64_t>(p->max_S / 2, 2 * WTILE);
-    p->spill_max = (uint32_t)std::min<uint64_t>(p->spill_cap / 64 + 4096, 1u << 30);
+    // the staged pass's S spill area: max_S (any skew of S fits), listed runs of
+    // 8 tuples on average (a hot key past its partition's capacity spills a run
+    // from every tile that holds it: at 11 bits, Zipf keys spill ~10^2 short runs
+    // per tile)
+    p->spill_cap = std::max<uint64_t>(p->max_S, 2 * WTILE);
+    p->spill_max = (uint32_t)std::min<uint64_t>(p->spill_cap / 8 + 4096, 1u << 30);
     // join entries: each partition, and (staged) each partition's S extension
     const uint64_t jparts = 2 * p->max_parts;
     BatchLayout &L = p->L;
@@ -174,12 +177,15 @@ static int enqueue_batch(BatchPlan *p, const aj_stream_tuple_t *dR, uint64_t nR,
 }
 
 uint64_t *batch_ranges(BatchPlan *p) { return W_<uint64_t>(p, p->L.rb); }
+const uint32_t *batch_fallback_flag(BatchPlan *p) {
+    return reinterpret_cast<const uint32_t *>(W_<uint64_t>(p, p->L.scal) + 5);
+}
 
 // Enqueues the joins of nwin windows whose tuple ranges (in tuples of dR and
 // dS) are already in the plan's range block.
-int enqueue_ranges(BatchPlan *p, const aj_stream_tuple_t *dR, uint64_t nR,
-                   const aj_stream_tuple_t *dS, uint64_t nS, int nwin, uint64_t *cnt,
-                   uint64_t *sum, hipStream_t st) {
+int enqueue_ranges(BatchPlan *p, const void *dR, uint64_t nR, const void *dS, uint64_t nS,
+                   int nwin, uint64_t *cnt, uint64_t *sum, hipStream_t st, int stride,
+                   const Marks *mk, int m0) {
     const BatchLayout &L = p->L;
     AJ_TRY(fill_u64(cnt, nwin, 0, st));
     AJ_TRY(fill_u64(sum, nwin, 0, st));
@@ -197,7 +203,7 @@ int enqueue_ranges(BatchPlan *p, const aj_stream_tuple_t *dR, uint64_t nR,
     PassDesc d;
     d.in[0] = dR;
     d.in[1] = dS;
-    d.in_stride = 12;
+    d.in_stride = stride;
     d.out[0] = p->outR.as<uint64_t>();
     d.out[1] = p->outS.as<uint64_t>();
     d.in_begin[0] = W_<uint64_t>(p, L.rb);
@@ -251,11 +257,15 @@ int enqueue_ranges(BatchPlan *p, const aj_stream_tuple_t *dR, uint64_t nR,
             ws.plen[r] = W_<uint32_t>(p, L.plen[r]);
         }
         AJ_TRY(run_partition_plan(d, w, st));
+        if (mk)
+            for (int i = 1; i < 4; ++i) mk->mark(m0 + i);
         AJ_TRY(run_window_stage(d, w, ws, st));
         d.guard = flag;
         d.skip_plan = true;
         AJ_TRY(run_partition_pass(d, w, st));
         AJ_TRY(run_window_parts(d, w, ws, st));
+        if (mk)
+            for (int i = 4; i < 9; ++i) mk->mark(m0 + i);
         j.roff = ws.pstart[0];
         j.soff = ws.pstart[1];
         j.rlen = ws.plen[0];
@@ -263,7 +273,11 @@ int enqueue_ranges(BatchPlan *p, const aj_stream_tuple_t *dR, uint64_t nR,
         j.max_parts = 2 * p->max_parts;
         j.nreal = p->max_parts;
     } else {
+        if (mk)
+            for (int i = 1; i < 4; ++i) mk->mark(m0 + i);
         AJ_TRY(run_partition_pass(d, w, st));
+        if (mk)
+            for (int i = 4; i < 9; ++i) mk->mark(m0 + i);
         j.roff = w.part_off[0];
         j.soff = w.part_off[1];
         j.max_parts = p->max_parts;
@@ -291,7 +305,7 @@ int enqueue_ranges(BatchPlan *p, const aj_stream_tuple_t *dR, uint64_t nR,
     j.scan_tmp = p->scan_tmp.p;
     j.scan_tmp_bytes = p->scan_bytes;
     j.num_cu = p->ctx->num_cu;
-    AJ_TRY(run_join(j, st));
+    AJ_TRY(run_join(j, st, mk, m0 + 9));
     return AJ_OK;
 }
 

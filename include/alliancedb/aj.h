@@ -182,8 +182,10 @@ int aj_write_result_file(const result_t *r, const char *path);
  *                        with atomically reserved, over-allocated partitions (no
  *                        histogram pass); an input whose partitions overflow is
  *                        detected on the device and joined again with the
- *                        histogram passes.  Window batches (plans of at most 11
- *                        radix bits per window): one histogram-free pass into
+ *                        histogram passes.  One PRO pass of at most 11 bits
+ *                        (counting joins of >= 2^20 tuples) and window batches
+ *                        (plans of at most 11 radix bits per window): one
+ *                        histogram-free pass into
  *                        over-allocated partitions, S runs past a partition's
  *                        capacity spilled and gathered into one extension range
  *                        per partition; R overflow, or a full spill area, runs the
@@ -564,10 +566,13 @@ typedef struct aj_tune_entry_t {
     int part_block;       /* 256 or 512 */
     uint32_t part_chunk;  /* 0 = derived */
     float device_ms;      /* measured at tuning time */
+    int one_pass_hist;    /* one pass of <= 11 bits: 1 = through the histogram pass,
+                             0 = the histogram-free staged pass (option "staged") */
 } aj_tune_entry_t;
 
 /* Times candidate geometries (radix bits around the size heuristic, 1 or 2
- * passes, LDS rounds 8192/8704/12288, then the partition work-group size) on
+ * passes, one pass with and without the staged pass, LDS rounds
+ * 8192/8704/12288, then the partition work-group size) on
  * the given device relations, `reps` timed runs each after a warm-up; every
  * candidate must return the same count and checksum (else AJ_ERR_STATE).  The
  * fastest becomes the bucket's entry and is returned in *best. */

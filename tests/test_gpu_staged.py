@@ -145,3 +145,57 @@ def test_staged_timing_and_switch(aj, oracle):
     with options(aj, staged=0):
         out = _join(aj, R, S, 14)
     assert out.flags == 0 and (out.count, out.checksum) == want
+
+
+def _join1(aj, ctx, R, S, bits, **kw):
+    dR, dS = aj.DeviceArray.from_host(R), aj.DeviceArray.from_host(S)
+    try:
+        return ctx.join(dR.ptr, len(R), dS.ptr, len(S), aj.AJ_ALGO_PRO, radix_bits=bits,
+                        passes=1, **kw)
+    finally:
+        dR.free()
+        dS.free()
+
+
+@pytest.mark.parametrize("kind", ["unique_uniform", "dups", "negative", "zipf", "all_ones",
+                                  "low_bits_zero"])
+@pytest.mark.parametrize("bits", [6, 11])
+def test_one_pass_staged_matches_oracle(aj, oracle, kind, bits):
+    """One pass of up to 11 bits runs as a one-segment window batch: no
+    histogram, S runs past a partition's capacity spilled into its extension
+    range (Zipf S stays on this path), an R partition that overflows (one hot
+    key, keys with equal low bits) reruns the pass from a histogram on the
+    device.  Exact either way; flags say which ran."""
+    nR, nS = 1 << 20, (1 << 21) + 777
+    R, S = _inputs(aj, kind, nR, nS)
+    want = oracle.aggregate(R, S)
+    print(f"case one-pass staged {kind} bits={bits}", flush=True)
+    with aj.Context(0) as ctx:
+        out = _join1(aj, ctx, R, S, bits)
+        assert (out.count, out.checksum) == want
+        hot_r = kind in ("all_ones", "low_bits_zero")
+        assert out.flags == (aj.AJ_OUT_FALLBACK if hot_r else aj.AJ_OUT_STAGED)
+        # again on the same context (its plan reused), with per-kernel timing
+        with options(aj, kernel_timing=1):
+            out = _join1(aj, ctx, R, S, bits)
+        assert (out.count, out.checksum) == want
+        if not hot_r:
+            ph = out.phases()
+            assert ph["scatter1"] > 0 and ph["join"] > 0 and ph["scatter2"] < 0.01
+
+
+def test_one_pass_staged_plan_follows_sizes(aj, oracle):
+    """The context's one-segment plan is rebuilt for larger inputs, other radix
+    bits and round sizes, and after the option "staged" changes."""
+    cases = [(1 << 20, 1 << 20, 10, 8704), ((1 << 21) + 5, (1 << 22) - 3, 11, 8704),
+             (1 << 20, 1 << 21, 11, 12288), (1 << 19, 1 << 20, 9, 8192)]
+    with aj.Context(0) as ctx:
+        for nR, nS, bits, cap in cases:
+            R, S = _inputs(aj, "unique_uniform", nR, nS)
+            want = oracle.aggregate(R, S)
+            print(f"case plan nR={nR} nS={nS} bits={bits} cap={cap}", flush=True)
+            out = _join1(aj, ctx, R, S, bits, cap=cap)
+            assert (out.count, out.checksum) == want and out.flags == aj.AJ_OUT_STAGED
+        with options(aj, staged=0):
+            out = _join1(aj, ctx, R, S, bits, cap=cap)
+        assert (out.count, out.checksum) == want and out.flags == 0

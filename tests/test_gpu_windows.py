@@ -160,8 +160,8 @@ def test_batch_windows_overflow_falls_back_exactly(aj, oracle, staged, case):
     overflows sends the batch to the histogram pass on the device.  S: only their
     S keys are equal; the S runs past the capacity are spilled and gathered
     into the partition's extension range.  zipf: every window's S keys are Zipf
-    distributed (many partitions spill).  S_all: every window's S keys are equal,
-    more than the spill area holds, so the histogram pass runs again.  Every
+    distributed (many partitions spill).  S_all: every window's S keys are equal
+    (nearly all of S spills).  Every
     window stays exact, with the staged pass on (option "staged" 1) or off, and
     so do an empty side and the ordinary windows of the same batch."""
     rng = np.random.default_rng(77)
