@@ -272,6 +272,8 @@ __global__ __launch_bounds__(PBLOCK) void k_scatter(PassArgs a) {
 //             consecutive addresses within a digit's run; the next tile's loads
 //             are issued before this step, so they are in flight meanwhile
 constexpr int PIPE_MAX_BITS = 10;
+// up to PIPE_WIDE_BITS with the block-wide scan (4 digits per lane)
+constexpr int PIPE_WIDE_BITS = 11;
 // LDS of the pipelined scatter: PTILE + NSPARE tuples, then F + NSPARE
 // counters, F write bases and F cursors.
 constexpr size_t pipe_lds_bytes(int pblock, uint32_t F) {
@@ -286,14 +288,16 @@ constexpr size_t pipe_lds_bytes(int pblock, uint32_t F) {
 // Every phase is branch-free over a lane's items: a tuple outside the range
 // counts on its lane's spare counter and is placed in its lane's spare slot,
 // so the LDS reads of a phase are issued in batches (WB items) and waited for
-// once.
-template <int STRIDE, int PBLOCK>
+// once.  WIDE (more than 2^PIPE_MAX_BITS digits): every lane scans 4 digits
+// and a block-wide scan orders them, instead of wave 0 scanning all of them.
+template <int STRIDE, int PBLOCK, bool WIDE = false>
 __device__ __forceinline__ void scatter_tiles(const void *in, uint64_t begin, uint64_t end,
                                               uint64_t *out, int shift, uint32_t F,
                                               uint64_t *buf, uint32_t *cnt, uint32_t *dst,
                                               uint32_t *cur, uint32_t *s_len) {
     constexpr int PTILE = PBLOCK * PITEMS;
-    constexpr int MAXPER = (1 << PIPE_MAX_BITS) / 64;
+    constexpr int MAXPER = WIDE ? 4 : (1 << PIPE_MAX_BITS) / 64;
+    static_assert(!WIDE || MAXPER * PBLOCK >= (1 << PIPE_WIDE_BITS), "4 digits per lane cover F");
     constexpr int WB = 8;  // items per batch of LDS reads in the place / write phases
     static_assert(PITEMS % WB == 0, "a tile's items split into whole batches");
     const uint32_t mask = F - 1;
@@ -316,7 +320,29 @@ __device__ __forceinline__ void scatter_tiles(const void *in, uint64_t begin, ui
                 rank2[j / 2] = r;
         }
         __syncthreads();
-        if (threadIdx.x < 64) {
+        if constexpr (WIDE) {
+            __shared__ uint32_t wscan[PBLOCK / 64];
+            const uint32_t b = threadIdx.x * MAXPER;
+            uint32_t c[MAXPER];
+            uint32_t sum = 0;
+#pragma unroll
+            for (int k = 0; k < MAXPER; ++k) {
+                c[k] = b + k < F ? cnt[b + k] : 0u;
+                sum += c[k];
+            }
+            uint32_t total;
+            uint32_t pre = block_exclusive_scan<PBLOCK>(sum, wscan, &total);
+#pragma unroll
+            for (int k = 0; k < MAXPER; ++k) {
+                if (b + k < F) {
+                    cnt[b + k] = pre;
+                    dst[b + k] = cur[b + k] - pre;
+                    cur[b + k] += c[k];
+                    pre += c[k];
+                }
+            }
+            if (threadIdx.x == 0) *s_len = total;
+        } else if (threadIdx.x < 64) {
             const uint32_t lane = threadIdx.x;
             const uint32_t per = (F + 63) >> 6;
             const uint32_t b = lane * per;
@@ -381,7 +407,7 @@ __device__ __forceinline__ void scatter_tiles(const void *in, uint64_t begin, ui
     }
 }
 
-template <int STRIDE, int PBLOCK>
+template <int STRIDE, int PBLOCK, bool WIDE = false>
 __global__ __launch_bounds__(PBLOCK, 4) void k_scatter_pipe(PassArgs a) {
     if (guard_off(a.guard)) return;
     constexpr int PTILE = PBLOCK * PITEMS;
@@ -403,8 +429,9 @@ __global__ __launch_bounds__(PBLOCK, 4) void k_scatter_pipe(PassArgs a) {
     }
     if (threadIdx.x < NSPARE) cnt[F + threadIdx.x] = 0;
     __syncthreads();
-    scatter_tiles<STRIDE, PBLOCK>(L.rel ? a.in1 : a.in0, L.begin, L.end, L.rel ? a.out1 : a.out0,
-                                  a.shift, F, buf, cnt, dst, cur, &s_len);
+    scatter_tiles<STRIDE, PBLOCK, WIDE>(L.rel ? a.in1 : a.in0, L.begin, L.end,
+                                        L.rel ? a.out1 : a.out0, a.shift, F, buf, cnt, dst, cur,
+                                        &s_len);
 }
 
 // ---------------------------------------------------------------- pass 2 from a 2-D histogram
@@ -1039,10 +1066,18 @@ static int launch_pass_kernels(const PassDesc &d, const PassWork &w, const PassA
             AJ_TRY(lds_limit_set((const void *)k_scatter_pipe<STRIDE, BIG>, lds_big));
             hipLaunchKernelGGL((k_scatter_pipe<STRIDE, BIG>), dim3((unsigned)d.max_chunks), dim3(BIG),
                                lds_big, st, a);
-        } else if (d.max_bits <= PIPE_MAX_BITS)
+        } else if (d.max_bits <= PIPE_MAX_BITS) {
             hipLaunchKernelGGL((k_scatter_pipe<STRIDE, PBLOCK>), dim3((unsigned)d.max_chunks),
                                dim3(PBLOCK), lds_pipe, st, a);
-        else
+        } else if (d.max_bits <= PIPE_WIDE_BITS) {
+            // 2^11 digits: 1024-lane work-groups with 16384-tuple tiles (runs
+            // of ~8 tuples per digit), the digits scanned block-wide
+            constexpr int BIG = 1024;
+            const size_t lds_big = pipe_lds_bytes(BIG, F);
+            AJ_TRY(lds_limit_set((const void *)k_scatter_pipe<STRIDE, BIG, true>, lds_big));
+            hipLaunchKernelGGL((k_scatter_pipe<STRIDE, BIG, true>), dim3((unsigned)d.max_chunks),
+                               dim3(BIG), lds_big, st, a);
+        } else
             hipLaunchKernelGGL((k_scatter<STRIDE, PBLOCK>), dim3((unsigned)d.max_chunks),
                                dim3(PBLOCK), lds_scatter, st, a);
     }

@@ -199,3 +199,16 @@ def test_one_pass_staged_plan_follows_sizes(aj, oracle):
         with options(aj, staged=0):
             out = _join1(aj, ctx, R, S, bits, cap=cap)
         assert (out.count, out.checksum) == want and out.flags == 0
+
+
+@pytest.mark.parametrize("kind", ["unique_uniform", "dups", "negative", "zipf"])
+def test_one_pass_histogram_at_11_bits(aj, oracle, kind):
+    """One histogram pass of 11 bits (option "staged" off) scatters 2048 digits
+    per tile with the block-wide digit scan; exact against the oracle."""
+    nR, nS = 1 << 20, (1 << 21) + 777
+    R, S = _inputs(aj, kind, nR, nS)
+    want = oracle.aggregate(R, S)
+    print(f"case one-pass histogram 11 bits {kind}", flush=True)
+    with options(aj, staged=0), aj.Context(0) as ctx:
+        out = _join1(aj, ctx, R, S, 11)
+    assert (out.count, out.checksum) == want and out.flags == 0

@@ -220,3 +220,28 @@ def test_batch_windows_overflow_falls_back_exactly(aj, oracle, staged, case):
                     assert got == [(int(c), int(x)) for c, x in want], rep
     finally:
         aj.set_option("staged", 1)
+
+
+@pytest.mark.parametrize("staged", [1, 0])
+def test_batch_plan_of_11_bits(aj, oracle, staged):
+    """A plan whose largest windows take 11 radix bits (windows of up to 2^18
+    tuples at 128 R tuples per partition): its scatters scan 2^11 digits
+    block-wide, smaller windows take fewer bits, and every window is exact,
+    staged pass on or off."""
+    nwin = 24
+    aj.set_option("staged", staged)
+    try:
+        with aj.Context(0) as ctx:
+            sizes, out = _streams(aj, ctx, nwin, 1 << 12, 1 << 18, seed=21)
+            bounds, host, want = _expected(aj, oracle, out, nwin)
+            cnt, chk = aj.DeviceArray(8 * nwin), aj.DeviceArray(8 * nwin)
+            (dR, nR, bR), (dS, nS, _) = out["R"], out["S"]
+            print(f"case 11-bit plan staged={staged} nR={nR} nS={nS}", flush=True)
+            with aj.BatchPlan(ctx, nR, nS, nwin, 1 << 18, target=128, graph=False) as plan:
+                plan.run(dR.ptr, nR, dS.ptr, nS, bR.ptr, nwin, cnt.ptr, chk.ptr)
+                aj.device_sync()
+                got = list(zip(cnt.to_host(np.uint64, nwin).tolist(),
+                               chk.to_host(np.uint64, nwin).tolist()))
+            assert got == [(int(c), int(x)) for c, x in want]
+    finally:
+        aj.set_option("staged", 1)
