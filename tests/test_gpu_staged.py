@@ -212,3 +212,23 @@ def test_one_pass_histogram_at_11_bits(aj, oracle, kind):
     with options(aj, staged=0), aj.Context(0) as ctx:
         out = _join1(aj, ctx, R, S, 11)
     assert (out.count, out.checksum) == want and out.flags == 0
+
+
+@pytest.mark.parametrize("nR,nS", [(0, 1 << 20), ((1 << 20) + 3, 0), (1 << 10, (1 << 20) - 1),
+                                   ((1 << 20) - 5, 9), ((1 << 19) + 1, (1 << 19) + 1)])
+def test_one_pass_staged_edge_sizes(aj, oracle, nR, nS):
+    """Empty and lopsided sides through the one-pass staged path (11 bits):
+    the result equals the oracle's (0 matches when a side is empty)."""
+    R = aj.gen_unique(nR, 12345) if nR else aj.as_tuples(np.zeros(0, np.int32))
+    S = aj.gen_uniform(nS, 1, max(nR, 1), 54321) if nS else aj.as_tuples(np.zeros(0, np.int32))
+    want = oracle.aggregate(R, S) if nR and nS else (0, 0)
+    print(f"case one-pass staged edge nR={nR} nS={nS}", flush=True)
+    with aj.Context(0) as ctx:
+        dR = aj.DeviceArray.from_host(R) if nR else aj.DeviceArray(64)
+        dS = aj.DeviceArray.from_host(S) if nS else aj.DeviceArray(64)
+        try:
+            out = ctx.join(dR.ptr, nR, dS.ptr, nS, aj.AJ_ALGO_PRO, radix_bits=11, passes=1)
+        finally:
+            dR.free()
+            dS.free()
+    assert (out.count, out.checksum) == want
