@@ -43,9 +43,10 @@ NPO_BYTES_PER_TUPLE = 32
 # SHJ per tuple: read 8 + its own table's slot 16 + a slot of the other table
 # 16 + 32 of its own table's slots initialised
 SHJ_BYTES_PER_TUPLE = 72
-# window batches, per tuple: the staged pass reads 12 and writes 8, the join
-# reads 8; the histogram pass (--no-staged) reads the 12 bytes once more
-WINDOW_BYTES_PER_TUPLE = {True: 28, False: 40}
+# window batches, credited per tuple whatever passes run (as PRO's 32): 12 read
+# for the window's histogram + 12 re-read and 8 written by the scatter + 8 read
+# by the build/probe.  The staged pass moves 28 of them (no histogram read).
+WINDOW_BYTES_PER_TUPLE = 40
 # algorithmic bytes per input tuple of each kernel (8-byte tuples), credited for
 # the tuples that kernel reads: NPO's build reads R only, its probe S only.  With
 # the histogram passes pass 1's histogram kernel (k_hist2d) also counts the
@@ -533,7 +534,7 @@ def run_windows(aj, args):
         in_r = int(rng_r[-1] - rng_r[0])
         in_s = int(rng_s[-1] - rng_s[0])
     tuples = in_r + in_s
-    bpt = WINDOW_BYTES_PER_TUPLE[not args.no_staged]
+    bpt = WINDOW_BYTES_PER_TUPLE
     gbs = bpt * tuples / t_batch / 1e9
     return {
         "metric": "batched window join throughput M tuples/s (|R|+|S| over all windows)",
