@@ -34,6 +34,7 @@ AJ_ALGO_PRO = 0
 # aj_join_out_t.flags (enum aj_out_flags)
 AJ_OUT_STAGED = 1
 AJ_OUT_FALLBACK = 2
+AJ_OUT_HEAVY = 4
 AJ_ACCESS_LOAD, AJ_ACCESS_CAS, AJ_ACCESS_LOAD_CAS = 0, 1, 2
 AJ_ALGO_NPO = 1
 AJ_ALGO_SHJ = 2

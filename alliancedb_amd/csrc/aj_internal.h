@@ -95,6 +95,7 @@ struct Ctx {
     DevBuf chain, chain2;       // NPO / SHJ with pairs: duplicate chain links (R, S)
     DevBuf heads, heads2;       // NPO / SHJ with pairs: chain heads per slot (R, S)
     DevBuf carry, carry2;       // SHJ: carries of the per-key payload sums (R, S)
+    DevBuf heavy, light;        // PRO on skewed S: heavy-key tables and sums; light S tuples
     hipEvent_t ev0 = nullptr, ev1 = nullptr;  // none: no timing (device_ms 0)
     bool no_events = false;     // never create events (no per-kernel timing either)
     bool arena_only = false;    // buffers come from a fixed scratch array (no allocation)
@@ -433,6 +434,27 @@ const uint32_t *batch_fallback_flag(BatchPlan *p);
 int enqueue_ranges(BatchPlan *p, const void *dR, uint64_t nR, const void *dS, uint64_t nS,
                    int nwin, uint64_t *cnt, uint64_t *sum, hipStream_t st, int stride = 12,
                    const Marks *mk = nullptr, int m0 = 0);
+
+// ---------------------------------------------------------------- heavy keys (heavy.hip)
+// The hottest keys of a skewed S joined by their aggregates (counting joins):
+// cR, sumR, cS, sumS per key from one read of R and S, the other S tuples
+// copied out for the partitioned join.
+constexpr int HEAVY_MAX = 1024;
+struct HeavyDesc {
+    const uint64_t *R, *S;
+    uint64_t nR, nS;
+    const uint64_t *slots;         // heavy_table's slots of nheavy keys, hottest first
+    uint32_t nheavy;
+    unsigned long long *agg;       // 4 * HEAVY_MAX: cR | sumR | cS | sumS (zeroed by the caller)
+    uint64_t *light;               // out: S tuples of the other keys (nS slots)
+    unsigned long long *light_n;   // out: their number (zeroed by the caller)
+    int num_cu;
+};
+// m keys of S at stride nS / m into d_keys.
+int heavy_sample(const uint64_t *dS, uint64_t nS, uint32_t m, uint32_t *d_keys, hipStream_t st);
+// Host: the 4 * HEAVY_MAX hash slots of n keys (key << 32 | 1 + key index, 0 empty).
+void heavy_table(const uint32_t *keys, uint32_t n, uint64_t *slots);
+int heavy_split(const HeavyDesc &d, hipStream_t st);
 
 // fill kernel (graph-safe replacement of hipMemsetAsync)
 int fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t st);

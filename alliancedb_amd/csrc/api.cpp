@@ -116,7 +116,7 @@ static std::atomic<int64_t> opt_materialize{0}, opt_radix_bits{0}, opt_passes{0}
     opt_cap{0}, opt_split{0}, opt_kernel_timing{0}, opt_part_block{0}, opt_part_chunk{0},
     opt_part_hist2d{1}, opt_tuned{1}, opt_timing{0}, opt_shj_batch{0}, opt_staged{1},
     opt_stage_sets{0}, opt_stage_chunk{0}, opt_stage_block{0}, opt_stage_cache{-1},
-    opt_npo_region_min{1ll << 22};
+    opt_npo_region_min{1ll << 22}, opt_heavy{1};
 
 static thread_local bool g_force_timing = false;  // phase timing for one drop-in call
 bool kernel_timing_enabled() { return g_force_timing || opt_kernel_timing.load() != 0; }
@@ -151,7 +151,8 @@ static void ctx_fini(Ctx *c) {
     DevBuf *bufs[] = {&c->inR, &c->inS, &c->p1R, &c->p1S, &c->p2R, &c->p2S, &c->hist,
                       &c->scan_tmp, &c->plan, &c->seg, &c->tasks, &c->acc, &c->pairs,
                       &c->table, &c->misc, &c->hist2d, &c->steps, &c->stage,
-                      &c->chain, &c->chain2, &c->heads, &c->heads2, &c->carry, &c->carry2};
+                      &c->chain, &c->chain2, &c->heads, &c->heads2, &c->carry, &c->carry2,
+                      &c->heavy, &c->light};
     for (DevBuf *b : bufs) b->release();
     if (c->pro1) {
         batch_plan_release(c->pro1);
@@ -220,6 +221,123 @@ static ProConfig config_from(uint64_t nR, uint64_t nS, const aj_join_opts_t *o) 
     return c;
 }
 
+// ---------------------------------------------------------------- heavy keys
+// A counting PRO whose S has keys that hold a large share of it (Zipf S at
+// theta >= 1): a strided sample of S names the keys seen at least HEAVY_MIN
+// times in HEAVY_SAMPLE draws (the hottest HEAVY_MAX); when they hold at least
+// HEAVY_SHARE of the sample, those keys are joined by their aggregates
+// (heavy.hip) and the partitioned join takes R and the rest of S.  The result
+// is the same sum either way.  A first sample of HEAVY_SCREEN draws ends the
+// check unless the keys drawn HEAVY_SCREEN_MIN times in it hold at least
+// HEAVY_SCREEN_SHARE of it (below that, the partitioned join of what would be
+// left costs about as much as the whole: measured at Zipf theta 1.0, where the
+// keys the full sample finds hold ~1/3 of S).  Returns with
+// *applied false (nothing joined) when the samples find too little.  Only
+// probe sides of at least 2^22 tuples and 4 times the build side are sampled
+// (the skewed foreign-key shape), so other joins pay nothing for the check.
+constexpr uint32_t HEAVY_SAMPLE = 1u << 14, HEAVY_MIN = 8;
+constexpr uint32_t HEAVY_SCREEN = 1u << 11, HEAVY_SCREEN_MIN = 4;
+constexpr double HEAVY_SCREEN_SHARE = 0.4, HEAVY_SHARE = 0.15;
+
+static bool heavy_applies(const Ctx *ctx, uint64_t nR, uint64_t nS, const uint64_t *pairs,
+                          int records) {
+    return opt_heavy.load() && !pairs && !records && !ctx->arena_only && nR > 0 &&
+           nS >= (1ull << 22) && nS >= 4 * nR;
+}
+
+// Keys drawn at least `min` times in the sample, as (draws, key), hottest
+// first, at most HEAVY_MAX (counted in an open-addressed table of 2m entries).
+static std::vector<std::pair<uint32_t, uint32_t>> heavy_select(const std::vector<uint32_t> &sample,
+                                                               uint32_t min) {
+    const size_t cap = 2 * sample.size();  // a power of two, as the sample size
+    std::vector<uint32_t> tkey(cap), tcnt(cap, 0);
+    for (uint32_t k : sample) {
+        size_t h = (size_t)(((uint64_t)k * 0x9E3779B97F4A7C15ull) >> 32) & (cap - 1);
+        while (tcnt[h] && tkey[h] != k) h = (h + 1) & (cap - 1);
+        tkey[h] = k;
+        ++tcnt[h];
+    }
+    std::vector<std::pair<uint32_t, uint32_t>> hot;
+    for (size_t i = 0; i < cap; ++i)
+        if (tcnt[i] >= min) hot.push_back({tcnt[i], tkey[i]});
+    std::sort(hot.begin(), hot.end(), [](const std::pair<uint32_t, uint32_t> &a,
+                                         const std::pair<uint32_t, uint32_t> &b) {
+        return a.first != b.first ? a.first > b.first : a.second < b.second;
+    });
+    if (hot.size() > (size_t)HEAVY_MAX) hot.resize(HEAVY_MAX);
+    return hot;
+}
+
+static int pro_heavy(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, uint64_t nS,
+                     const ProConfig &cfg, aj_join_out_t *out, hipStream_t st, bool *applied) {
+    *applied = false;
+    const uint32_t m = HEAVY_SAMPLE;
+    const size_t o_keys = 0, o_slots = 4 * (size_t)m, o_agg = o_slots + 32 * HEAVY_MAX,
+                 o_n = o_agg + 32 * HEAVY_MAX, bytes = o_n + 64;
+    AJ_TRY(ctx->heavy.ensure(bytes));
+    uint8_t *hb = ctx->heavy.as<uint8_t>();
+    uint32_t *d_sample = reinterpret_cast<uint32_t *>(hb + o_keys);
+    std::vector<uint32_t> sample(HEAVY_SCREEN);
+    AJ_TRY(heavy_sample(dS, nS, HEAVY_SCREEN, d_sample, st));
+    AJ_HIP(hipMemcpyAsync(sample.data(), d_sample, 4 * (size_t)HEAVY_SCREEN,
+                          hipMemcpyDeviceToHost, st));
+    AJ_HIP(hipStreamSynchronize(st));
+    uint64_t screened = 0;
+    for (const auto &h : heavy_select(sample, HEAVY_SCREEN_MIN)) screened += h.first;
+    if ((double)screened < HEAVY_SCREEN_SHARE * HEAVY_SCREEN) return AJ_OK;
+    sample.resize(m);
+    AJ_TRY(heavy_sample(dS, nS, m, d_sample, st));
+    AJ_HIP(hipMemcpyAsync(sample.data(), d_sample, 4 * (size_t)m, hipMemcpyDeviceToHost, st));
+    AJ_HIP(hipStreamSynchronize(st));
+    const std::vector<std::pair<uint32_t, uint32_t>> hot = heavy_select(sample, HEAVY_MIN);
+    uint64_t drawn = 0;
+    for (const auto &h : hot) drawn += h.first;
+    if (hot.empty() || (double)drawn < HEAVY_SHARE * m) return AJ_OK;
+    const uint32_t nh = (uint32_t)hot.size();
+    std::vector<uint32_t> keys(nh);
+    std::vector<uint64_t> slots(4 * HEAVY_MAX);
+    for (uint32_t i = 0; i < nh; ++i) keys[i] = hot[i].second;
+    heavy_table(keys.data(), nh, slots.data());
+    AJ_TRY(ctx->light.ensure(8 * nS + 256));
+    HeavyDesc d;
+    d.R = dR;
+    d.S = dS;
+    d.nR = nR;
+    d.nS = nS;
+    d.slots = reinterpret_cast<uint64_t *>(hb + o_slots);
+    d.nheavy = nh;
+    d.agg = reinterpret_cast<unsigned long long *>(hb + o_agg);
+    d.light = ctx->light.as<uint64_t>();
+    d.light_n = reinterpret_cast<unsigned long long *>(hb + o_n);
+    d.num_cu = ctx->num_cu;
+    AJ_HIP(hipMemcpyAsync((void *)d.slots, slots.data(), 8 * slots.size(), hipMemcpyHostToDevice,
+                          st));
+    AJ_TRY(fill_u64(reinterpret_cast<uint64_t *>(d.agg), 4 * HEAVY_MAX + 1, 0, st));
+    AJ_HIP(record_event(ctx->ev0, st));
+    AJ_TRY(heavy_split(d, st));
+    AJ_HIP(record_event(ctx->ev1, st));
+    std::vector<uint64_t> agg(4 * HEAVY_MAX + 1);
+    AJ_HIP(hipMemcpyAsync(agg.data(), d.agg, 8 * agg.size(), hipMemcpyDeviceToHost, st));
+    AJ_HIP(hipStreamSynchronize(st));
+    const float split_ms = elapsed_ms(ctx->ev0, ctx->ev1);
+    const uint64_t n_light = agg[4 * HEAVY_MAX];
+    uint64_t count = 0, sum = 0;
+    for (uint32_t i = 0; i < nh; ++i) {
+        const uint64_t cR = agg[i], sR = agg[HEAVY_MAX + i], cS = agg[2 * HEAVY_MAX + i],
+                       sS = agg[3 * HEAVY_MAX + i];
+        count += cR * cS;
+        sum += ((sR * cS) << 32) + cR * sS;
+    }
+    memset(out, 0, sizeof(*out));
+    if (n_light) AJ_TRY(pro_device(ctx, dR, nR, d.light, n_light, cfg, nullptr, 0, 0, out, st));
+    out->count += count;
+    out->checksum += sum;
+    out->device_ms += split_ms;
+    out->flags |= AJ_OUT_HEAVY;
+    *applied = true;
+    return AJ_OK;
+}
+
 // Tuples per side per eager step: the caller's, else 256 steps of >= 65536.
 static uint64_t shj_batch(uint64_t nR, uint64_t nS, const aj_join_opts_t *opts) {
     if (opts && opts->batch) return opts->batch;
@@ -255,8 +373,13 @@ static int join_dev_locked(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint
     } else if (algo == AJ_ALGO_NPO) {
         rc = npo_device(ctx, dR, nR, dS, nS, pairs, pcap, records, out, st);
     } else if (algo == AJ_ALGO_PRO) {
-        rc = pro_device(ctx, dR, nR, dS, nS, config_from(nR, nS, opts), pairs, pcap, records,
-                        out, st);
+        const ProConfig cfg = config_from(nR, nS, opts);
+        bool done = false;
+        rc = AJ_OK;
+        if (heavy_applies(ctx, nR, nS, pairs, records))
+            rc = pro_heavy(ctx, dR, nR, dS, nS, cfg, out, st, &done);
+        if (rc == AJ_OK && !done)
+            rc = pro_device(ctx, dR, nR, dS, nS, cfg, pairs, pcap, records, out, st);
     } else {
         set_error(AJ_ERR_ARG, "unknown algorithm %d", algo);
         return AJ_ERR_ARG;
@@ -648,7 +771,8 @@ int aj_set_option(const char *name, int64_t value) {
                {"stage_chunk", &opt_stage_chunk, 0, 1ll << 31},
                {"stage_block", &opt_stage_block, 0, 1024},
                {"stage_cache", &opt_stage_cache, -1, 15},
-               {"npo_region_min", &opt_npo_region_min, 0, 1ll << 32}};
+               {"npo_region_min", &opt_npo_region_min, 0, 1ll << 32},
+               {"heavy", &opt_heavy, 0, 1}};
     for (auto &e : tab) {
         if (strcmp(e.n, name) == 0) {
             if (value < e.lo || value > e.hi) {
@@ -705,6 +829,7 @@ int64_t aj_get_option(const char *name) {
     if (!strcmp(name, "stage_block")) return opt_stage_block.load();
     if (!strcmp(name, "stage_cache")) return opt_stage_cache.load();
     if (!strcmp(name, "npo_region_min")) return opt_npo_region_min.load();
+    if (!strcmp(name, "heavy")) return opt_heavy.load();
     if (!strcmp(name, "device_budget")) return (int64_t)g_dev_budget.load();
     if (!strcmp(name, "device_used")) return (int64_t)g_dev_used.load();
     return -1;

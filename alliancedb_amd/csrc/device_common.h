@@ -202,7 +202,7 @@ __device__ __forceinline__ uint64_t last_le(const T *a, uint64_t n, T x) {
     return lo;
 }
 
-__device__ __forceinline__ uint32_t fmix32(uint32_t h) {
+__host__ __device__ __forceinline__ uint32_t fmix32(uint32_t h) {
     // murmur3 finalizer: a bijective avalanche mix of the key
     h ^= h >> 16;
     h *= 0x85ebca6bu;

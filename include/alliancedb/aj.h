@@ -191,6 +191,13 @@ int aj_write_result_file(const result_t *r, const char *path);
  *                        per partition; R overflow, or a full spill area, runs the
  *                        histogram pass on the device instead.  Read when a plan
  *                        or server is created (default 1)
+ *   "heavy"        0/1   counting PRO joins of |S| >= 2^22 and >= 4 |R|: when keys
+ *                        drawn 4+ times in a 2^11-tuple strided sample of S hold >=
+ *                        40% of it, the keys drawn 8+ times in a 2^14-tuple sample
+ *                        (the hottest 1024) are joined by their aggregates (count
+ *                        and payload sum per key and side); the partitioned join
+ *                        takes R and the other S tuples.  The result is the same
+ *                        (default 1)
  *   "stage_sets"   0..64 staged: cursor sets of the staging regions (0 = 4)
  *   "stage_chunk"  >=0   staged: tuples per super-chunk (0 = 2^25)
  *   "stage_block"  0, 512, 1024  staged: lanes per work-group (0 = 1024)
@@ -358,10 +365,12 @@ typedef struct aj_join_out_t {
                             and the histogram passes joined again (device_ms covers both).
                             NPO: AJ_OUT_STAGED when the table was built region by region,
                             AJ_OUT_FALLBACK when that overflowed and the flat build ran
-                            in its place (on the device); 0 for the flat build alone. */
+                            in its place (on the device); 0 for the flat build alone.
+                            PRO also AJ_OUT_HEAVY when the hottest keys of S were joined
+                            by their aggregates (option "heavy"). */
 } aj_join_out_t;
 
-enum aj_out_flags { AJ_OUT_STAGED = 1, AJ_OUT_FALLBACK = 2 };
+enum aj_out_flags { AJ_OUT_STAGED = 1, AJ_OUT_FALLBACK = 2, AJ_OUT_HEAVY = 4 };
 
 /*
  * Join device-resident relations (8-byte tuple_t arrays in HBM) on `stream`

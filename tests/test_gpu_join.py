@@ -488,3 +488,75 @@ def test_join_round_table_duplicate_counts(aj, oracle, copies):
     print(f"case copies={copies} nR={len(R)} nS={len(S)}", flush=True)
     got = aj.PRO(R, S, 8)
     assert (got.count, got.checksum) == oracle.aggregate(R, S)
+
+
+def _heavy_inputs(aj, kind):
+    nR, nS = 1 << 20, 1 << 22
+    rng = np.random.default_rng(404)
+    if kind in ("zipf10", "zipf15"):
+        return aj.gen_unique(nR, 7), aj.gen_zipf(nS, nR, 1.0 if kind == "zipf10" else 1.5, 8)
+    if kind == "one_key":  # every S tuple the same key, held three times by R
+        keys = rng.permutation(np.arange(1, nR + 1))
+        keys[:3] = 77
+        R = aj.as_tuples(keys.astype(np.int32), rng.integers(-2**31, 2**31, nR).astype(np.int32))
+        return R, aj.as_tuples(np.full(nS, 77, np.int32),
+                               rng.integers(-2**31, 2**31, nS).astype(np.int32))
+    if kind == "mixed":  # hot keys with R duplicates, hot keys missing from R, negative keys
+        rk = rng.permutation(np.arange(1, nR + 1)).astype(np.int64)
+        rk[:40] = np.repeat([5, 6, -3, 2**31 - 1], 10)
+        hot = np.array([5, 6, -3, 2**31 - 1, 99999999, -77777])
+        sk = rng.integers(1, nR + 1, nS)
+        pick = rng.random(nS) < 0.5
+        sk[pick] = rng.choice(hot, pick.sum())
+        R = aj.as_tuples(rk.astype(np.int32), rng.integers(-2**31, 2**31, nR).astype(np.int32))
+        S = aj.as_tuples(sk.astype(np.int32), rng.integers(-2**31, 2**31, nS).astype(np.int32))
+        return R, S
+    raise ValueError(kind)
+
+
+@pytest.mark.parametrize("kind,heavy", [("zipf10", False), ("zipf15", True), ("one_key", True),
+                                        ("mixed", True)])
+def test_heavy_keys_joined_by_aggregates(aj, oracle, kind, heavy):
+    """A skewed S (|S| >= 4|R|): when a few keys hold most of it (Zipf 1.5, one
+    key, six keys holding half), they are joined by their per-key counts and
+    payload sums and the rest by the partitioned join; at Zipf 1.0 the screen
+    leaves the join to the partitioned path.  Count and checksum equal the
+    oracle's either way, and with option "heavy" 0."""
+    R, S = _heavy_inputs(aj, kind)
+    want = oracle.aggregate(R, S)
+    print(f"case heavy {kind} nR={len(R)} nS={len(S)}", flush=True)
+    dR, dS = aj.DeviceArray.from_host(R), aj.DeviceArray.from_host(S)
+    try:
+        with aj.Context(0) as ctx:
+            out = ctx.join(dR.ptr, len(R), dS.ptr, len(S), aj.AJ_ALGO_PRO)
+            assert (out.count, out.checksum) == want
+            assert bool(out.flags & aj.AJ_OUT_HEAVY) == heavy
+            for bits, passes in ((11, 1), (10, 2)):  # other geometries of the light join
+                out = ctx.join(dR.ptr, len(R), dS.ptr, len(S), aj.AJ_ALGO_PRO,
+                               radix_bits=bits, passes=passes)
+                assert (out.count, out.checksum) == want
+                assert bool(out.flags & aj.AJ_OUT_HEAVY) == heavy
+            aj.set_option("heavy", 0)
+            try:
+                out = ctx.join(dR.ptr, len(R), dS.ptr, len(S), aj.AJ_ALGO_PRO)
+            finally:
+                aj.set_option("heavy", 1)
+            assert (out.count, out.checksum) == want and not out.flags & aj.AJ_OUT_HEAVY
+    finally:
+        dR.free()
+        dS.free()
+
+
+def test_heavy_keys_not_taken_without_skew(aj, oracle):
+    """Uniform S of the same shape: the sample finds no heavy keys and the
+    partitioned join runs alone."""
+    R, S = aj.gen_unique(1 << 20, 7), aj.gen_uniform(1 << 22, 1, 1 << 20, 9)
+    dR, dS = aj.DeviceArray.from_host(R), aj.DeviceArray.from_host(S)
+    try:
+        with aj.Context(0) as ctx:
+            out = ctx.join(dR.ptr, len(R), dS.ptr, len(S), aj.AJ_ALGO_PRO)
+    finally:
+        dR.free()
+        dS.free()
+    assert (out.count, out.checksum) == oracle.aggregate(R, S)
+    assert not out.flags & aj.AJ_OUT_HEAVY
