@@ -97,6 +97,7 @@ struct Ctx {
     DevBuf carry, carry2;       // SHJ: carries of the per-key payload sums (R, S)
     DevBuf heavy, light;        // PRO on skewed S: heavy-key tables and sums; light S tuples
     hipEvent_t ev0 = nullptr, ev1 = nullptr;  // none: no timing (device_ms 0)
+    uint64_t *hres = nullptr;  // pinned host slots for a join's scalars (HRES_SLOTS); none: pageable
     bool no_events = false;     // never create events (no per-kernel timing either)
     bool arena_only = false;    // buffers come from a fixed scratch array (no allocation)
     BatchPlan *pro1 = nullptr;  // one-pass staged PRO: a one-segment window plan
@@ -108,6 +109,14 @@ bool staged_enabled();  // option "staged"
 // option npo_region_min: smallest |R| whose NPO table is built by regions (0 = never)
 uint64_t npo_region_min();
 // Events of a context that may have none: recording is skipped, times are 0.
+// A join's scalars (count, checksum, ...) are copied back into the context's
+// pinned slots, which the copy engine writes directly: a copy to pageable
+// memory goes through a staging buffer and keeps the device idle ~15-30 us
+// longer between back-to-back joins.  A context without them (the
+// allocation-free entry point's) reads into `local`.
+constexpr int HRES_SLOTS = 8;
+inline uint64_t *result_slots(Ctx *ctx, uint64_t *local) { return ctx->hres ? ctx->hres : local; }
+
 inline hipError_t record_event(hipEvent_t e, hipStream_t st) {
     return e ? hipEventRecord(e, st) : hipSuccess;
 }

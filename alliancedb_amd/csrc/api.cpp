@@ -143,6 +143,7 @@ static int ctx_init(Ctx *c, int device) {
     AJ_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     AJ_HIP(hipEventCreate(&c->ev0));
     AJ_HIP(hipEventCreate(&c->ev1));
+    AJ_HIP(hipHostMalloc((void **)&c->hres, HRES_SLOTS * sizeof(uint64_t), hipHostMallocDefault));
     return AJ_OK;
 }
 
@@ -166,6 +167,8 @@ static void ctx_fini(Ctx *c) {
     for (hipEvent_t &e : c->evk)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->hres) (void)hipHostFree(c->hres);
+    c->hres = nullptr;
 }
 
 static std::mutex g_default_mu;

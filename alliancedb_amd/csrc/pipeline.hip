@@ -330,8 +330,9 @@ static int pro_staged(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t 
     j.tasks_ready = true;
     AJ_TRY(run_join(j, st, mkp, 9));
     AJ_HIP(record_event(ctx->ev1, st));
-    uint64_t host[5];
-    AJ_HIP(hipMemcpyAsync(host, scal, sizeof(host), hipMemcpyDeviceToHost, st));
+    uint64_t local[HRES_SLOTS];
+    uint64_t *host = result_slots(ctx, local);
+    AJ_HIP(hipMemcpyAsync(host, scal, 5 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
     AJ_HIP(hipStreamSynchronize(st));
     const float ms = elapsed_ms(ctx->ev0, ctx->ev1);
     if ((uint32_t)host[4]) {
@@ -427,13 +428,15 @@ static int pro_staged1(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t
     hipLaunchKernelGGL(k_one_range, dim3(1), dim3(64), 0, st, batch_ranges(p), nR, nS);
     AJ_TRY(enqueue_ranges(p, dR, nR, dS, nS, 1, res, res + 1, st, 8, mkp, 0));
     AJ_HIP(record_event(ctx->ev1, st));
-    uint64_t host[2];
-    uint32_t flag = 0;
-    AJ_HIP(hipMemcpyAsync(host, res, sizeof(host), hipMemcpyDeviceToHost, st));
+    uint64_t local[HRES_SLOTS];
+    uint64_t *host = result_slots(ctx, local);
+    host[2] = 0;
+    AJ_HIP(hipMemcpyAsync(host, res, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
     if (p->staged)
-        AJ_HIP(hipMemcpyAsync(&flag, batch_fallback_flag(p), sizeof(flag),
+        AJ_HIP(hipMemcpyAsync(host + 2, batch_fallback_flag(p), sizeof(uint32_t),
                               hipMemcpyDeviceToHost, st));
     AJ_HIP(hipStreamSynchronize(st));
+    const uint32_t flag = (uint32_t)host[2];
     out->count = host[0];
     out->checksum = host[1];
     out->npairs = 0;
@@ -629,8 +632,9 @@ int pro_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
     j.num_cu = ctx->num_cu;
     AJ_TRY(run_join(j, st, mkp, 9));
     AJ_HIP(record_event(ctx->ev1, st));
-    uint64_t host[3];
-    AJ_HIP(hipMemcpyAsync(host, scal, sizeof(host), hipMemcpyDeviceToHost, st));
+    uint64_t local[HRES_SLOTS];
+    uint64_t *host = result_slots(ctx, local);
+    AJ_HIP(hipMemcpyAsync(host, scal, 3 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
     AJ_HIP(hipStreamSynchronize(st));
     const float ms = elapsed_ms(ctx->ev0, ctx->ev1);
     out->count = host[0];
@@ -715,8 +719,9 @@ int npo_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
     }
     AJ_TRY(run_npo(d, st, mkp, AJ_PH_NPO_BUILD));
     AJ_HIP(record_event(ctx->ev1, st));
-    uint64_t host[5];
-    AJ_HIP(hipMemcpyAsync(host, acc, sizeof(host), hipMemcpyDeviceToHost, st));
+    uint64_t local[HRES_SLOTS];
+    uint64_t *host = result_slots(ctx, local);
+    AJ_HIP(hipMemcpyAsync(host, acc, 5 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
     AJ_HIP(hipStreamSynchronize(st));
     const float ms = elapsed_ms(ctx->ev0, ctx->ev1);
     out->count = host[0];

@@ -430,8 +430,9 @@ int shj_device(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *dS, ui
                        (const uint64_t *)sum_part, prog, acc);
     AJ_HIP(hipGetLastError());
     AJ_HIP(record_event(ctx->ev1, st));
-    uint64_t host[6];
-    AJ_HIP(hipMemcpyAsync(host, acc, sizeof(host), hipMemcpyDeviceToHost, st));
+    uint64_t local[HRES_SLOTS];
+    uint64_t *host = result_slots(ctx, local);
+    AJ_HIP(hipMemcpyAsync(host, acc, 6 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
     if (progress)
         AJ_HIP(hipMemcpyAsync(progress, prog, 8 * steps, hipMemcpyDefault, st));
     AJ_HIP(hipStreamSynchronize(st));

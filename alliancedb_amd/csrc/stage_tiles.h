@@ -92,15 +92,25 @@ __device__ __forceinline__ void stage_tiles(const Src &src, uint64_t *out, int s
     for (;;) {
         const StageRes &r = sg.r;
         uint32_t rank2[PITEMS / 2];
+        // A wave whose first tuple of the tile lies past the segment's end
+        // holds none of its tuples (most waves of a segment's last tile, when
+        // that holds only a few): it skips the count here and the placement
+        // below.  Lane l's tuples are tb + 2 (j PBLOCK + l) + {0, 1} for
+        // 8-byte tuples and tb + j PBLOCK + l for 12-byte ones.
+        const uint64_t wfirst =
+            base + (STRIDE == 8 ? 2ull : 1ull) * __builtin_amdgcn_readfirstlane(threadIdx.x);
+        const bool busy = wfirst < sg.end;
+        if (busy) {
 #pragma unroll
-        for (int j = 0; j < PITEMS; ++j) {
-            const uint32_t d = (key_of(t[j]) >> shift) & mask;
-            const uint32_t rk =
-                atomicAdd(&cnt[v[j] ? d : F + (threadIdx.x & (NSPARE - 1))], 1u) & 0xffffu;
-            if (j & 1)
-                rank2[j / 2] |= rk << 16;
-            else
-                rank2[j / 2] = rk;
+            for (int j = 0; j < PITEMS; ++j) {
+                const uint32_t d = (key_of(t[j]) >> shift) & mask;
+                const uint32_t rk =
+                    atomicAdd(&cnt[v[j] ? d : F + (threadIdx.x & (NSPARE - 1))], 1u) & 0xffffu;
+                if (j & 1)
+                    rank2[j / 2] |= rk << 16;
+                else
+                    rank2[j / 2] = rk;
+            }
         }
         __syncthreads();
         uint32_t o[MAXPER], pk[MAXPER], ck[MAXPER];
@@ -144,15 +154,17 @@ __device__ __forceinline__ void stage_tiles(const Src &src, uint64_t *out, int s
             if (lane == 63) *s_len = inc;
         }
         __syncthreads();
+        if (busy) {
 #pragma unroll
-        for (int j0 = 0; j0 < PITEMS; j0 += WB) {
-            uint32_t st[WB];
+            for (int j0 = 0; j0 < PITEMS; j0 += WB) {
+                uint32_t st[WB];
 #pragma unroll
-            for (int j = 0; j < WB; ++j) st[j] = cnt[(key_of(t[j0 + j]) >> shift) & mask];
+                for (int j = 0; j < WB; ++j) st[j] = cnt[(key_of(t[j0 + j]) >> shift) & mask];
 #pragma unroll
-            for (int j = 0; j < WB; ++j)
-                buf[v[j0 + j] ? st[j] + ((rank2[(j0 + j) / 2] >> (16 * ((j0 + j) & 1))) & 0xffffu)
-                              : (uint32_t)PTILE + (threadIdx.x & (NSPARE - 1))] = t[j0 + j];
+                for (int j = 0; j < WB; ++j)
+                    buf[v[j0 + j] ? st[j] + ((rank2[(j0 + j) / 2] >> (16 * ((j0 + j) & 1))) & 0xffffu)
+                                  : (uint32_t)PTILE + (threadIdx.x & (NSPARE - 1))] = t[j0 + j];
+            }
         }
         if (WIDE || threadIdx.x < 64) {
 #pragma unroll
