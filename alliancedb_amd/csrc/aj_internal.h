@@ -108,7 +108,6 @@ bool kernel_timing_enabled();
 bool staged_enabled();  // option "staged"
 // option npo_region_min: smallest |R| whose NPO table is built by regions (0 = never)
 uint64_t npo_region_min();
-// Events of a context that may have none: recording is skipped, times are 0.
 // A join's scalars (count, checksum, ...) are copied back into the context's
 // pinned slots, which the copy engine writes directly: a copy to pageable
 // memory goes through a staging buffer and keeps the device idle ~15-30 us
@@ -117,6 +116,7 @@ uint64_t npo_region_min();
 constexpr int HRES_SLOTS = 8;
 inline uint64_t *result_slots(Ctx *ctx, uint64_t *local) { return ctx->hres ? ctx->hres : local; }
 
+// Events of a context that may have none: recording is skipped, times are 0.
 inline hipError_t record_event(hipEvent_t e, hipStream_t st) {
     return e ? hipEventRecord(e, st) : hipSuccess;
 }
@@ -280,6 +280,7 @@ struct StagedGeom {
     uint64_t tiles_per_wg = 0;
     uint32_t grid1 = 0;       // pass-1 work-groups of a full super-chunk
     uint32_t units = 1;       // pass-2 work items per staging region
+    uint32_t part = 0;        // tuples per pass-2 work item (0: a region split evenly over units)
     uint32_t per_wg = 1;      // pass-2 work items per work-group
     uint32_t grid2 = 0;       // pass-2 work-groups
     uint32_t scap = 0;        // tuples per staging region (nset * 2^b1 regions)
