@@ -109,7 +109,8 @@ class aj_join_out_t(ctypes.Structure):
     _fields_ = [("count", ctypes.c_uint64), ("checksum", ctypes.c_uint64),
                 ("npairs", ctypes.c_uint64), ("device_ms", ctypes.c_float),
                 ("phase_ms", ctypes.c_float * AJ_NPHASES), ("nrecords_r", ctypes.c_uint64),
-                ("flags", ctypes.c_uint32)]
+                ("flags", ctypes.c_uint32), ("heavy_ms", ctypes.c_float),
+                ("light", ctypes.c_uint64)]
 
     def phases(self) -> dict:
         return {n: float(self.phase_ms[i]) for i, n in enumerate(PHASE_NAMES)}

@@ -336,6 +336,8 @@ static int pro_heavy(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t *
     out->count += count;
     out->checksum += sum;
     out->device_ms += split_ms;
+    out->heavy_ms = split_ms;
+    out->light = n_light;
     out->flags |= AJ_OUT_HEAVY;
     *applied = true;
     return AJ_OK;

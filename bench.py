@@ -59,7 +59,10 @@ KERNEL_NAMES = {"hist1": "k_hist2d (pass 1)", "scatter1": "k_scatter_pipe (pass 
                 "scatter2": "k_scatter_p2 (pass 2)",
                 "join": "k_join", "npo_build": "NPO build (k_stage1 + k_stage2 hashing R into "
                 "the table's regions, k_npo_region)", "npo_probe": "k_npo_probe",
-                "shj_step": "k_shj_step (all eager steps)"}
+                "shj_step": "k_shj_step (all eager steps)",
+                "heavy_split": "k_heavy_split (R and S read once: the hottest keys of S summed, "
+                               "the rest of S written out; 8 B per tuple read + 8 B per S "
+                               "tuple written)"}
 # the staged partitioning (aj_join_out_t.flags & AJ_OUT_STAGED): one pass-1 and one
 # pass-2 launch per super-chunk of STAGE_CHUNK tuples (the library's default)
 STAGED_NAMES = {"scatter1": "k_stage1 (pass 1)", "scatter2": "k_stage2 (pass 2)"}
@@ -272,7 +275,7 @@ def timed_joins(aj, ctx, dR, nR, dS, nS, algo, kw, steps, warmup, want):
     try:
         for _ in range(steps):
             o2 = ctx.join(dR.ptr, nR, dS.ptr, nS, algo, **kw)
-            phases.append(o2.phases())
+            phases.append({**o2.phases(), "heavy_split": float(o2.heavy_ms)})
             ev_ms.append(o2.device_ms)
             assert (o2.count, o2.checksum) == want
     finally:
@@ -286,10 +289,22 @@ def join_line(aj, args, workload_name, nR, nS, algo, out, dev, ms_step, phases, 
     mean_ph = {k: float(np.mean([p[k] for p in phases])) for k in phases[0]}
     if algo == aj.AJ_ALGO_SHJ:  # one kernel type: the steps are the whole device time
         mean_ph = {"shj_step": dev}
-    dom = max(KERNEL_BYTES, key=lambda k: mean_ph.get(k, 0.0))
-    side = KERNEL_SIDE.get(dom)
-    kt = nR if side == "R" else nS if side == "S" else tuples
-    achieved = KERNEL_BYTES[dom] * kt / (mean_ph[dom] * 1e-3) / 1e9
+    # With the hottest keys of S joined by their aggregates (AJ_OUT_HEAVY), one
+    # pass reads R and S and writes out the rest of S (out.light tuples); the
+    # partitioned join then takes R and that rest only.
+    heavy = bool(getattr(out, "flags", 0) & aj.AJ_OUT_HEAVY)
+    light = int(out.light) if heavy else nS
+
+    def kernel_bytes(k):
+        if k == "heavy_split":
+            return 8 * (nR + nS) + 8 * light
+        side = KERNEL_SIDE.get(k)
+        return KERNEL_BYTES[k] * (nR if side == "R" else nS if side == "S" else nR + light)
+
+    cands = list(KERNEL_BYTES) + (["heavy_split"] if heavy else [])
+    dom = max(cands, key=lambda k: mean_ph.get(k, 0.0))
+    dom_bytes = kernel_bytes(dom)
+    achieved = dom_bytes / (mean_ph[dom] * 1e-3) / 1e9
     staged = bool(getattr(out, "flags", 0) & aj.AJ_OUT_STAGED)
     kname = (STAGED_NAMES.get(dom) if staged else None) or KERNEL_NAMES[dom]
     launches = 1
@@ -317,8 +332,8 @@ def join_line(aj, args, workload_name, nR, nS, algo, out, dev, ms_step, phases, 
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None, "kernel": kname, "launches_per_join": launches,
-                     "bytes_per_tuple": KERNEL_BYTES[dom],
-                     "bytes_per_launch": KERNEL_BYTES[dom] * kt // launches,
+                     "bytes_per_tuple": KERNEL_BYTES.get(dom, 8),
+                     "bytes_per_launch": dom_bytes // launches,
                      "mean_launch_ms": round(mean_ph[dom] / launches, 4),
                      "whole_join": {"achieved": round(whole_gbs, 1),
                                     "frac": round(whole_gbs / HBM_PEAK_GBS, 4),
@@ -328,6 +343,8 @@ def join_line(aj, args, workload_name, nR, nS, algo, out, dev, ms_step, phases, 
                      "frac_of_peak": round(dev_gbs / HBM_PEAK_GBS, 4),
                      "partitioning": ("region build" if staged else "flat build")
                      if algo == aj.AJ_ALGO_NPO else "staged" if staged else "histogram passes",
+                     "heavy_keys": {"S_by_aggregates": nS - light, "S_partitioned": light}
+                     if heavy else None,
                      "kernel_ms": {k: round(v, 4) for k, v in mean_ph.items() if v > 0.0005},
                      "kernel_ms_from": f"{args.steps} further joins with HIP events around "
                                        f"every launch (device {ev_dev:.3f} ms per join with "

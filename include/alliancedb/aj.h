@@ -368,6 +368,11 @@ typedef struct aj_join_out_t {
                             in its place (on the device); 0 for the flat build alone.
                             PRO also AJ_OUT_HEAVY when the hottest keys of S were joined
                             by their aggregates (option "heavy"). */
+    float heavy_ms;      /* PRO with AJ_OUT_HEAVY: device time of the pass that splits off
+                            the hottest keys (one read of R and S, the rest of S written
+                            out); part of device_ms.  0 otherwise. */
+    uint64_t light;      /* PRO with AJ_OUT_HEAVY: S tuples left to the partitioned join
+                            (those of the keys not joined by their aggregates); 0 otherwise. */
 } aj_join_out_t;
 
 enum aj_out_flags { AJ_OUT_STAGED = 1, AJ_OUT_FALLBACK = 2, AJ_OUT_HEAVY = 4 };

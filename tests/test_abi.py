@@ -43,6 +43,28 @@ def test_struct_layouts(aj):
     assert aj.STREAM_DTYPE.itemsize == 12
 
 
+def test_join_out_mirror_matches_header(tmp_path):
+    """The ctypes mirror of aj_join_out_t has the C header's size and offsets."""
+    import shutil
+    import subprocess
+    from alliancedb_amd import _lib
+    if not shutil.which("gcc"):
+        pytest.skip("no C compiler")
+    T = _lib.aj_join_out_t
+    names = [f[0] for f in T._fields_]
+    src = ('#include <stdio.h>\n#include <stddef.h>\n#include "alliancedb/aj.h"\n'
+           'int main(void) { printf("%zu", sizeof(aj_join_out_t));' +
+           "".join(f' printf(" %zu", offsetof(aj_join_out_t, {n}));' for n in names) +
+           " return 0; }\n")
+    (tmp_path / "layout.c").write_text(src)
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), "-o", str(exe),
+                    str(tmp_path / "layout.c")], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True,
+                                          text=True).stdout.split()]
+    assert got == [ctypes.sizeof(T)] + [getattr(T, n).offset for n in names]
+
+
 def test_null_relation_is_an_error_not_exit(aj):
     from alliancedb_amd import _lib
     res = _lib.lib.PRO(None, None, 4)
