@@ -531,6 +531,14 @@ def test_heavy_keys_joined_by_aggregates(aj, oracle, kind, heavy):
             out = ctx.join(dR.ptr, len(R), dS.ptr, len(S), aj.AJ_ALGO_PRO)
             assert (out.count, out.checksum) == want
             assert bool(out.flags & aj.AJ_OUT_HEAVY) == heavy
+            if heavy:
+                # the hottest key is always among those taken: its tuples are not
+                # left to the partitioned join
+                keys, counts = np.unique(S["key"], return_counts=True)
+                assert out.light <= len(S) - counts.max()
+                assert out.heavy_ms > 0.0
+            else:
+                assert out.light == 0 and out.heavy_ms == 0.0
             for bits, passes in ((11, 1), (10, 2)):  # other geometries of the light join
                 out = ctx.join(dR.ptr, len(R), dS.ptr, len(S), aj.AJ_ALGO_PRO,
                                radix_bits=bits, passes=passes)
