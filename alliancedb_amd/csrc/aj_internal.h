@@ -273,7 +273,8 @@ int run_pass2_2d(const P2Desc &d, hipStream_t st, const Marks *marks = nullptr, 
 struct StagedGeom {
     int b1 = 0, b2 = 0;       // radix bits of pass 1 and pass 2 (each 1..8)
     int nset = 1;             // staging cursor sets (work-group mod nset)
-    int block = 1024;         // lanes per work-group (512 or 1024)
+    int block = 1024;         // lanes per pass-2 work-group (512 or 1024)
+    int block1 = 1024;        // lanes per pass-1 work-group (512 or 1024)
     int cache_mode = 9;       // nontemporal accesses: 1 input loads, 2 staging stores,
                               // 4 staging loads, 8 final stores; 16: keys hashed in pass 1
     uint64_t chunk = 0;       // tuples per super-chunk (a multiple of the tile)
@@ -299,8 +300,9 @@ struct StagedDesc {
     uint32_t *flag;           // set when a partition overflowed
 };
 // chunk 0 = 2^25 tuples
+// block: lanes per pass-2 work-group; block1 those of pass 1 (0: block)
 int staged_geometry(StagedGeom *g, uint64_t nR, uint64_t nS, int b1, int b2, uint64_t chunk,
-                    int nset, int block, int num_cu);
+                    int nset, int block, int num_cu, int block1 = 0);
 // events recorded by run_staged when timing: before and after every pass-1
 // launch (pass 2 runs between one super-chunk's "after" and the next "before")
 int staged_events(const StagedDesc &d);

@@ -244,9 +244,16 @@ static int pro_staged(Ctx *ctx, const uint64_t *dR, uint64_t nR, const uint64_t 
     *overflow = false;
     StagedDesc sd;
     const int sets = cfg.stage_sets ? cfg.stage_sets : 4;
+    // pass 1 in 512-lane work-groups (two per CU: one's tile work overlaps the
+    // other's loads; 0.85 against 0.87 ms per 2^28 tuples), pass 2 in 1024-lane
+    // ones (16K-tuple tiles: runs of 128 tuples per digit; with 8K tiles pass 2
+    // took 1.22 against 1.02 ms); option stage_block sets both, and the other
+    // cache modes keep 1024 lanes in both
     const int block = cfg.stage_block ? cfg.stage_block : 1024;
+    const int block1 = cfg.stage_block ? cfg.stage_block
+                       : (cfg.stage_cache < 0 || cfg.stage_cache == 9) ? 512 : 1024;
     AJ_TRY(staged_geometry(&sd.g, nR, nS, cfg.bits1, cfg.bits2, cfg.stage_chunk, sets, block,
-                           ctx->num_cu));
+                           ctx->num_cu, block1));
     if (cfg.stage_cache >= 0) sd.g.cache_mode = cfg.stage_cache;
     const StagedGeom &g = sd.g;
     const uint32_t F1 = 1u << g.b1, F = F1 << g.b2;

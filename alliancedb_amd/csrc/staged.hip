@@ -212,30 +212,38 @@ static uint32_t region_cap(double expect) {
 }
 
 int staged_geometry(StagedGeom *g, uint64_t nR, uint64_t nS, int b1, int b2, uint64_t chunk,
-                    int nset, int block, int num_cu) {
+                    int nset, int block, int num_cu, int block1) {
+    if (!block1) block1 = block;
     if (b1 < 1 || b2 < 1 || (1 << b1) > STAGE_MAXF || (1 << b2) > STAGE_MAXF ||
-        (block != 512 && block != 1024) || nset < 1 || nset > 64 || nR == 0 || nS == 0) {
-        set_error(AJ_ERR_ARG, "staged partitioning: bad geometry (%d + %d bits, block %d, %d sets)",
-                  b1, b2, block, nset);
+        (block != 512 && block != 1024) || (block1 != 512 && block1 != 1024) || nset < 1 ||
+        nset > 64 || nR == 0 || nS == 0) {
+        set_error(AJ_ERR_ARG,
+                  "staged partitioning: bad geometry (%d + %d bits, blocks %d + %d, %d sets)", b1,
+                  b2, block1, block, nset);
         return AJ_ERR_ARG;
     }
-    const uint64_t ptile = (uint64_t)block * PITEMS;
+    // pass 1 tiles of ptile1 tuples, pass 2 tiles of ptile; both trash areas
+    // hold the larger tile
+    const uint64_t ptile1 = (uint64_t)block1 * PITEMS, ptile = (uint64_t)block * PITEMS;
+    const uint64_t pmax = std::max(ptile1, ptile);
     g->b1 = b1;
     g->b2 = b2;
     g->nset = nset;
     g->block = block;
+    g->block1 = block1;
     const uint64_t nmax = std::max(nR, nS);
     if (!chunk) chunk = 1ull << 25;
-    chunk = std::max<uint64_t>(ptile, chunk / ptile * ptile);
-    g->chunk = std::min<uint64_t>(chunk, (nmax + ptile - 1) / ptile * ptile);
-    const uint64_t tiles = g->chunk / ptile;
-    const int per_cu = block == 1024 ? 1 : 2;
-    const uint64_t g1 = std::min<uint64_t>(tiles, (uint64_t)num_cu * per_cu);
+    chunk = std::max<uint64_t>(pmax, chunk / pmax * pmax);
+    g->chunk = std::min<uint64_t>(chunk, (nmax + pmax - 1) / pmax * pmax);
+    const uint64_t tiles = g->chunk / ptile1;
+    const int per_cu1 = block1 == 1024 ? 1 : 2, per_cu = block == 1024 ? 1 : 2;
+    const uint64_t g1 = std::min<uint64_t>(tiles, (uint64_t)num_cu * per_cu1);
     g->tiles_per_wg = (tiles + g1 - 1) / g1;
     g->grid1 = (uint32_t)((tiles + g->tiles_per_wg - 1) / g->tiles_per_wg);
     // the largest share of a super-chunk one cursor set can receive
     const uint64_t wg_per_set = (g->grid1 + nset - 1) / nset;
-    const double set_tuples = (double)std::min<uint64_t>(g->chunk, wg_per_set * g->tiles_per_wg * ptile);
+    const double set_tuples =
+        (double)std::min<uint64_t>(g->chunk, wg_per_set * g->tiles_per_wg * ptile1);
     const uint32_t F1 = 1u << b1, F = F1 << b2;
     g->scap = region_cap(set_tuples / F1);
     g->fcap[0] = region_cap((double)nR / F);
@@ -251,8 +259,8 @@ int staged_geometry(StagedGeom *g, uint64_t nR, uint64_t nS, int b1, int b2, uin
     const uint64_t items = (uint64_t)nreg * g->units;
     g->per_wg = (uint32_t)((items + g2 - 1) / g2);
     g->grid2 = (uint32_t)((items + g->per_wg - 1) / g->per_wg);
-    g->stage_tuples = (uint64_t)nreg * g->scap + ptile;
-    for (int r = 0; r < 2; ++r) g->out_tuples[r] = (uint64_t)F * g->fcap[r] + ptile;
+    g->stage_tuples = (uint64_t)nreg * g->scap + pmax;
+    for (int r = 0; r < 2; ++r) g->out_tuples[r] = (uint64_t)F * g->fcap[r] + pmax;
     if (g->stage_tuples >= (1ull << 32) || g->out_tuples[0] >= (1ull << 32) ||
         g->out_tuples[1] >= (1ull << 32)) {
         set_error(AJ_ERR_ARG, "staged partitioning: regions exceed 32-bit offsets");
@@ -261,13 +269,13 @@ int staged_geometry(StagedGeom *g, uint64_t nR, uint64_t nS, int b1, int b2, uin
     return AJ_OK;
 }
 
-template <int PBLOCK, int MODE>
+template <int PBLOCK1, int PBLOCK, int MODE>
 static int launch_staged(const StagedDesc &d, hipStream_t st, std::vector<hipEvent_t> *evs) {
     const StagedGeom &g = d.g;
     const uint32_t F1 = 1u << g.b1, F2 = 1u << g.b2;
-    const size_t lds1 = stage_lds_bytes(PBLOCK, F1), lds2 = stage_lds_bytes(PBLOCK, F2);
+    const size_t lds1 = stage_lds_bytes(PBLOCK1, F1), lds2 = stage_lds_bytes(PBLOCK, F2);
     constexpr int MODE2 = MODE & 15;  // pass 2 reads keys as pass 1 wrote them
-    AJ_TRY(lds_limit_set((const void *)k_stage1<PBLOCK, MODE>, lds1));
+    AJ_TRY(lds_limit_set((const void *)k_stage1<PBLOCK1, MODE>, lds1));
     AJ_TRY(lds_limit_set((const void *)k_stage2<PBLOCK, MODE2>, lds2));
     const uint32_t nreg = (uint32_t)g.nset * F1;
     int k = 0, ev = 0;
@@ -286,10 +294,12 @@ static int launch_staged(const StagedDesc &d, hipStream_t st, std::vector<hipEve
             a1.scap = g.scap;
             a1.scur = cur;
             a1.flag = d.flag;
-            const uint64_t tiles = (a1.end - a1.begin + PBLOCK * PITEMS - 1) / (PBLOCK * PITEMS);
+            const uint64_t tiles =
+                (a1.end - a1.begin + PBLOCK1 * PITEMS - 1) / (PBLOCK1 * PITEMS);
             const unsigned grid1 = (unsigned)((tiles + g.tiles_per_wg - 1) / g.tiles_per_wg);
             if (evs) (void)hipEventRecord((*evs)[ev++], st);
-            hipLaunchKernelGGL((k_stage1<PBLOCK, MODE>), dim3(grid1), dim3(PBLOCK), lds1, st, a1);
+            hipLaunchKernelGGL((k_stage1<PBLOCK1, MODE>), dim3(grid1), dim3(PBLOCK1), lds1, st,
+                               a1);
             if (evs) (void)hipEventRecord((*evs)[ev++], st);
             Stage2Args a2;
             a2.stage = d.stage;
@@ -333,17 +343,18 @@ int run_staged(const StagedDesc &d, hipStream_t st, std::vector<hipEvent_t> *evs
     hipLaunchKernelGGL(k_stage_zero, dim3(64), dim3(256), 0, st, d.scur, nz);
     hipLaunchKernelGGL(k_stage_zero, dim3(64), dim3(256), 0, st, d.fcur, 2ull * F);
     hipLaunchKernelGGL(k_stage_zero, dim3(1), dim3(64), 0, st, d.flag, (uint64_t)1);
-    switch (g.block * 32 + g.cache_mode) {
-#define AJ_STAGED_CASE(B, M) \
-    case B * 32 + M: return launch_staged<B, M>(d, st, evs);
-        AJ_STAGED_CASE(1024, 0) AJ_STAGED_CASE(1024, 1) AJ_STAGED_CASE(1024, 5)
-        AJ_STAGED_CASE(1024, 9) AJ_STAGED_CASE(1024, 13) AJ_STAGED_CASE(1024, 15)
-        AJ_STAGED_CASE(512, 1) AJ_STAGED_CASE(512, 5) AJ_STAGED_CASE(512, 9)
-        AJ_STAGED_CASE(1024, 25)
+    switch ((g.block1 * 2 + g.block) * 32 + g.cache_mode) {
+#define AJ_STAGED_CASE(B1, B, M) \
+    case (B1 * 2 + B) * 32 + M: return launch_staged<B1, B, M>(d, st, evs);
+        AJ_STAGED_CASE(1024, 1024, 0) AJ_STAGED_CASE(1024, 1024, 1) AJ_STAGED_CASE(1024, 1024, 5)
+        AJ_STAGED_CASE(1024, 1024, 9) AJ_STAGED_CASE(1024, 1024, 13)
+        AJ_STAGED_CASE(1024, 1024, 15) AJ_STAGED_CASE(512, 512, 1) AJ_STAGED_CASE(512, 512, 5)
+        AJ_STAGED_CASE(512, 512, 9) AJ_STAGED_CASE(1024, 1024, 25) AJ_STAGED_CASE(512, 1024, 9)
 #undef AJ_STAGED_CASE
         default:
-            set_error(AJ_ERR_ARG, "staged partitioning: no kernels for block %d, cache mode %d",
-                      g.block, g.cache_mode);
+            set_error(AJ_ERR_ARG,
+                      "staged partitioning: no kernels for blocks %d + %d, cache mode %d",
+                      g.block1, g.block, g.cache_mode);
             return AJ_ERR_ARG;
     }
 }

@@ -79,7 +79,7 @@ def test_staged_matches_oracle(aj, oracle, kind, bits):
 
 @pytest.mark.parametrize("sets", [1, 2, 8])
 @pytest.mark.parametrize("chunk", [16384, 1 << 20, 0])  # 16384: one 1024-lane tile
-@pytest.mark.parametrize("block", [512, 1024])
+@pytest.mark.parametrize("block", [0, 512, 1024])  # 0: 512-lane pass 1, 1024-lane pass 2
 def test_staged_geometry_invariance(aj, oracle, sets, chunk, block):
     nR, nS = (1 << 21) + 12345, (1 << 21) - 999
     R, S = _inputs(aj, "unique_uniform", nR, nS)
