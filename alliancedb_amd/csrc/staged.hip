@@ -198,10 +198,15 @@ __global__ void k_stage_tasks(const uint32_t *fcur, uint32_t F, uint32_t fcapR, 
     }
 }
 
-__global__ void k_stage_zero(uint32_t *p, uint64_t n) {
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
-         i += (uint64_t)gridDim.x * blockDim.x)
-        p[i] = 0;
+// Zeroes up to three counter ranges in one launch (the staging cursors, the
+// final cursors and the overflow flag of a staged join).
+__global__ void k_stage_zero(uint32_t *p0, uint64_t n0, uint32_t *p1, uint64_t n1, uint32_t *p2,
+                             uint64_t n2) {
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t i0 = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    for (uint64_t i = i0; i < n0; i += step) p0[i] = 0;
+    for (uint64_t i = i0; i < n1; i += step) p1[i] = 0;
+    for (uint64_t i = i0; i < n2; i += step) p2[i] = 0;
 }
 
 // ---------------------------------------------------------------- host side
@@ -340,9 +345,8 @@ int run_staged(const StagedDesc &d, hipStream_t st, std::vector<hipEvent_t> *evs
     }
     // both cursor sets of the staging regions, the final cursors and the flag
     const uint64_t nz = 2ull * g.nset * F1;
-    hipLaunchKernelGGL(k_stage_zero, dim3(64), dim3(256), 0, st, d.scur, nz);
-    hipLaunchKernelGGL(k_stage_zero, dim3(64), dim3(256), 0, st, d.fcur, 2ull * F);
-    hipLaunchKernelGGL(k_stage_zero, dim3(1), dim3(64), 0, st, d.flag, (uint64_t)1);
+    hipLaunchKernelGGL(k_stage_zero, dim3(64), dim3(256), 0, st, d.scur, nz, d.fcur, 2ull * F,
+                       d.flag, (uint64_t)1);
     switch ((g.block1 * 2 + g.block) * 32 + g.cache_mode) {
 #define AJ_STAGED_CASE(B1, B, M) \
     case (B1 * 2 + B) * 32 + M: return launch_staged<B1, B, M>(d, st, evs);
