@@ -83,7 +83,7 @@ __device__ __forceinline__ void stage_tiles(const Src &src, uint64_t *out, int s
     for (;; ++si) {  // the first segment with a tuple
         if (!src.get(si, sg)) return;
         base = tile_start(sg.begin, STRIDE);
-        if (base < sg.end) break;
+        if (sg.begin < sg.end) break;  // (base may be begin - 1 of an empty segment)
     }
     uint64_t t[PITEMS];
     bool v[PITEMS];
@@ -208,7 +208,7 @@ __device__ __forceinline__ void stage_tiles(const Src &src, uint64_t *out, int s
         StageSeg ns = sg;
         while (!more && src.get(++si, ns)) {
             nbase = tile_start(ns.begin, STRIDE);
-            more = nbase < ns.end;
+            more = ns.begin < ns.end;
         }
         if (more) {
             load_tile<PBLOCK, PITEMS, NT_LOAD>(ns.in, STRIDE, nbase, ns.begin, ns.end, t, v);
